@@ -1,0 +1,153 @@
+"""Throughput of the LCODE 3D xi step on one MI355X.
+
+Workload: the 641^2 production configuration (grid_step_size 0.025,
+xi_step_size 0.005, plasma_coarseness 3, plasma_fineness 2, Variant A,
+subtraction trick 1, paddings 5/10) driven by the cosine-Gaussian beam of
+amplitude 0.05, sigma 1, with the plain FP64 HIP step.  One "step" is one
+xi slice: estimate, 3 pushes, 3 depositions, 2 six-field solves.  The beam
+slices of the timed region are generated on the device beforehand, so the
+timed loop reads only HBM-resident inputs.
+
+Prints ONE JSON line on stdout (everything else goes to stderr).
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_GRID = 641
+WORKLOAD = dict(grid_steps=641, grid_step_size=.025, xi_step_size=.005,
+                plasma_coarseness=3, plasma_fineness=2,
+                field_solver_subtraction_trick=1, field_solver_variant_A=True,
+                reflect_padding_steps=5, plasma_padding_steps=10)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_config():
+    import types
+    return types.SimpleNamespace(**WORKLOAD)
+
+
+def algorithmic_bytes_per_step(nc, n):
+    """8 (79 nc^2 + 91 N^2): every stage reads its inputs once and writes
+    its outputs once (estimate 10 + push 3x16 + deposit 3x7 coarse-array
+    touches; push 3x6 + deposit 3x5 + field solve 2x29 grid touches)."""
+    return 8 * (79 * nc * nc + 91 * n * n)
+
+
+def cpu_baseline(steps):
+    """The oracle (NumPy/SciPy restatement of the reference step) on the
+    host cores, on a bounded sample of the same workload."""
+    import numpy as np
+    from oracle import lcode_oracle as oracle
+    cores = len(os.sched_getaffinity(0))
+    oracle.set_fft_workers(cores)
+    cfg = oracle.make_config(**{k: v for k, v in dict(
+        grid_steps=641, grid_step_size=.025, xi_step_size=.005,
+        coarseness=3, fineness=2).items()})
+    xs, ys, const, vp, st = oracle.init(cfg)
+    start = 300  # a slice inside the bunch
+    st = oracle.step(cfg, const, vp, st, oracle.beam(cfg, start, xs, ys))
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st = oracle.step(cfg, const, vp, st,
+                         oracle.beam(cfg, start + 1 + k, xs, ys))
+    dt = time.perf_counter() - t0
+    assert np.isfinite(st.Ez).all()
+    return {"value": steps / dt, "unit": "xi-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{steps} xi-steps at 641^2 after 1 warm-up step "
+                      f"(slices {start + 1}-{start + steps}), scipy.fft "
+                      f"workers={cores}, NumPy bincount deposition"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): "
+              "the xi step is sequential in xi and this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import lcode3d_amd as lc
+    from lcode3d_amd.beams import cosine_gaussian
+
+    torch.cuda.set_device(0)
+    cfg = make_config()
+    xs, ys, const, vp, state = lc.init(cfg)
+    nc = const.m.shape[0]
+    n = cfg.grid_steps
+    total = args.warmup + args.steps
+    # beam slices of the run, resident in HBM before the clock starts
+    first = 100
+    beams = [cosine_gaussian(cfg, first + k) for k in range(total)]
+    for k in range(first):  # bring the plasma into a developed state
+        state = lc.step(cfg, const, vp, state, cosine_gaussian(cfg, k))
+    torch.cuda.synchronize()
+    log(f"workload 641^2, nc={nc}, nf={vp.fine_grid.numel()}, "
+        f"{args.warmup} warm-up + {args.steps} timed steps")
+    for k in range(args.warmup):
+        state = lc.step(cfg, const, vp, state, beams[k])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        state = lc.step(cfg, const, vp, state, beams[args.warmup + k])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    assert torch.isfinite(state.Ez).all().item()
+    faults = lc.fault_count(cfg, nc)
+    if faults:
+        log(f"WARNING: {faults} clamped stencil lookups")
+    steps_per_s = args.steps / dt
+    ms = dt / args.steps * 1e3
+    step_bytes = algorithmic_bytes_per_step(nc, n)
+    log(f"{steps_per_s:.1f} xi-steps/s, {ms:.3f} ms/step, "
+        f"step-level {step_bytes / (dt / args.steps) / 1e9:.1f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_steps)
+        log(f"cpu baseline {cpu['value']:.3f} xi-steps/s on {cpu['cores']} cores")
+
+    out = {
+        "metric": "xi-steps/s at 641^2 grid (plain FP64 HIP step)",
+        "value": steps_per_s,
+        "unit": "xi-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic cosine-Gaussian beam (amplitude 0.05, sigma 1), "
+                "slices 100+ of the bunch",
+        "config": {"workload": "lcode3d xi step, 641^2 grid, coarseness 3, "
+                               "fineness 2, plain FP64",
+                   "grid_steps": n, "coarse_particles": nc * nc,
+                   "fine_particles": int(vp.fine_grid.numel()) ** 2},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+/*
+ * lcode3d_hip.h - C ABI of the MI355X (gfx950) implementation of LCODE 3D's
+ * per-xi-slice step.
+ *
+ * Every entry point takes plain device pointers (FP64 arrays in C order,
+ * int64 index arrays), sizes in a config struct and a HIP stream, and
+ * returns an int status: LCODE_OK (0) or a negative LCODE_E_* code, with a
+ * human-readable message from lcode_last_error().  Nothing here allocates
+ * the caller's outputs: the caller passes them in.  All work is enqueued on
+ * `stream`; no entry point synchronises the device except
+ * lcode_workspace_fault_count().
+ *
+ * Interface map onto the reference (lotov/lcode3d, lcode.py):
+ *   lcode_workspace_create/destroy  - the cp.memoize'd eigenvalue tables
+ *                                      (lcode.py:144, :215, :316) plus the
+ *                                      scratch the fused step needs
+ *   lcode_move_estimate_wo_fields   - move_estimate_wo_fields (lcode.py:369)
+ *   lcode_move_smart                - move_smart + move_smart_kernel
+ *                                      (lcode.py:796, :704)
+ *   lcode_deposit                   - deposit + deposit_kernel
+ *                                      (lcode.py:666, :626)
+ *   lcode_calculate_Ex_Ey_Bx_By     - calculate_Ex_Ey_Bx_By (lcode.py:246)
+ *   lcode_calculate_Ez              - calculate_Ez (lcode.py:160)
+ *   lcode_calculate_Bz              - calculate_Bz (lcode.py:333)
+ *   lcode_step                      - step (lcode.py:828)
+ *
+ * Grids are (N, N) with [i, j] <-> (x, y) = ((i - N/2) h, (j - N/2) h)
+ * (lcode.py:951-953).  Coarse particle arrays hold nc*nc values, the
+ * virt_params arrays nf values each (lcode.py:566-570).
+ *
+ * Threading: a workspace may be used by one stream at a time; calls on the
+ * same workspace from two streams race on its scratch buffers.
+ */
+#ifndef LCODE3D_HIP_H
+#define LCODE3D_HIP_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LCODE_OK = 0,
+    LCODE_E_INVALID = -1,   /* bad argument or config */
+    LCODE_E_HIP = -2,       /* a HIP runtime call failed */
+    LCODE_E_UNSUPPORTED = -3 /* e.g. a transform length with a prime > 7 */
+};
+
+/* Transform back-end of the field solves. */
+enum {
+    LCODE_SOLVER_FFT64 = 0  /* hand-written mixed-radix FFT in LDS, FP64 */
+};
+
+/* The config attributes the step reads (config_example.py:1-17), with
+ * reflect_boundary as init() computes it (lcode.py:947-949). */
+typedef struct {
+    int32_t grid_steps;            /* N, odd */
+    double  grid_step_size;        /* h */
+    double  xi_step_size;          /* dxi */
+    double  subtraction_trick;     /* RHS multiplier; the matrix uses != 0 */
+    int32_t variant_A;             /* field_solver_variant_A */
+    double  reflect_boundary;
+    int32_t plasma_coarseness;
+    int32_t plasma_fineness;
+    int32_t nc;                    /* coarse particles per axis */
+    int32_t nf;                    /* fine particles per axis */
+    int32_t solver;                /* LCODE_SOLVER_* */
+} lcode_config;
+
+/* const of init() (lcode.py:964-965): nc*nc arrays, ro_initial N*N. */
+typedef struct {
+    const double *m, *q, *x_init, *y_init, *ro_initial;
+} lcode_const;
+
+/* virt_params of make_plasma() (lcode.py:566-570): nf entries each. */
+typedef struct {
+    const double  *influence_prev, *influence_next, *fine_grid;
+    const int64_t *indices_prev, *indices_next;
+} lcode_virt_params;
+
+/* The state step() consumes and returns (lcode.py:926-929). */
+typedef struct {
+    double *x_offt, *y_offt, *px, *py, *pz;      /* nc*nc */
+    double *Ex, *Ey, *Ez, *Bx, *By, *Bz;          /* N*N */
+    double *ro, *jx, *jy, *jz;                    /* N*N */
+} lcode_state;
+
+typedef struct lcode_workspace lcode_workspace;
+
+/* Last error message of the calling thread ("" if none). */
+const char *lcode_last_error(void);
+
+/* Library version string. */
+const char *lcode_version(void);
+
+/* Builds the eigenvalue and twiddle tables for cfg (N, h, trick, solver)
+ * on the current device and allocates the step's scratch. */
+int lcode_workspace_create(const lcode_config *cfg, lcode_workspace **out);
+int lcode_workspace_destroy(lcode_workspace *ws);
+
+/* Number of particle cell lookups that fell outside the grid (and were
+ * clamped to it) since the last call; resets the counter.  The reference
+ * has no such check: an escaped particle is a memory violation there
+ * (docs/overview/window_and_grids.rst).  Synchronises `stream`. */
+int lcode_workspace_fault_count(lcode_workspace *ws, hipStream_t stream,
+                                int64_t *count);
+
+/* lcode.py:369-391.  n = nc*nc particles. */
+int lcode_move_estimate_wo_fields(lcode_workspace *ws, const lcode_config *cfg,
+                                  const double *m, const double *x_init,
+                                  const double *y_init,
+                                  const double *prev_x_offt,
+                                  const double *prev_y_offt,
+                                  const double *px, const double *py,
+                                  const double *pz,
+                                  double *x_offt_out, double *y_offt_out,
+                                  hipStream_t stream);
+
+/* lcode.py:796-823.  fields = {Ex, Ey, Ez, Bx, By, Bz}_avg (N*N each);
+ * out = {x_offt, y_offt, px, py, pz}. */
+int lcode_move_smart(lcode_workspace *ws, const lcode_config *cfg,
+                     const double *m, const double *q,
+                     const double *x_init, const double *y_init,
+                     const double *x_prev_offt, const double *y_prev_offt,
+                     const double *estimated_x_offt,
+                     const double *estimated_y_offt,
+                     const double *px_prev, const double *py_prev,
+                     const double *pz_prev,
+                     const double *const fields[6],
+                     double *const out[5],
+                     hipStream_t stream);
+
+/* lcode.py:666-689.  ro_initial may be NULL (the reference's scalar 0 in
+ * initial_deposition, lcode.py:697).  out = {ro, jx, jy, jz}. */
+int lcode_deposit(lcode_workspace *ws, const lcode_config *cfg,
+                  const double *ro_initial,
+                  const double *x_offt, const double *y_offt,
+                  const double *m, const double *q,
+                  const double *px, const double *py, const double *pz,
+                  const lcode_virt_params *virt_params,
+                  double *const out[4], hipStream_t stream);
+
+/* lcode.py:246-290.  beam_ro is an N*N array or NULL, in which case the
+ * scalar beam_scalar is used everywhere (the reference's beam returns a
+ * scalar 0 outside the bunch, config_example.py:24-25). out = {Ex,Ey,Bx,By}.*/
+int lcode_calculate_Ex_Ey_Bx_By(lcode_workspace *ws, const lcode_config *cfg,
+                                const double *Ex_avg, const double *Ey_avg,
+                                const double *Bx_avg, const double *By_avg,
+                                const double *beam_ro, double beam_scalar,
+                                const double *ro, const double *jx,
+                                const double *jy, const double *jz,
+                                const double *jx_prev, const double *jy_prev,
+                                double *const out[4], hipStream_t stream);
+
+/* lcode.py:160-184. */
+int lcode_calculate_Ez(lcode_workspace *ws, const lcode_config *cfg,
+                       const double *jx, const double *jy, double *Ez,
+                       hipStream_t stream);
+
+/* lcode.py:333-364. */
+int lcode_calculate_Bz(lcode_workspace *ws, const lcode_config *cfg,
+                       const double *jx, const double *jy, double *Bz,
+                       hipStream_t stream);
+
+/* lcode.py:828-931.  `prev` is read, `out` written; they must not alias.
+ * beam_ro / beam_scalar as in lcode_calculate_Ex_Ey_Bx_By. */
+int lcode_step(lcode_workspace *ws, const lcode_config *cfg,
+               const lcode_const *cnst, const lcode_virt_params *virt_params,
+               const lcode_state *prev, const double *beam_ro,
+               double beam_scalar, const lcode_state *out,
+               hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LCODE3D_HIP_H */

@@ -1,0 +1,560 @@
+// C ABI of the gfx950 LCODE 3D step: workspace (the eigenvalue tables the
+// reference memoizes with cp.memoize, FFT twiddles, scratch), the
+// standalone operators and the fused xi step (lcode.py:828-931).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace lcode {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace lcode
+
+using namespace lcode;
+
+struct lcode_workspace {
+    int device = 0;
+    int N = 0, M = 0;
+    double h = 0;
+    int trick_on = 0;
+    int nc = 0, nf = 0;
+    SolverGeom geom{};
+    double2* W = nullptr;          // twiddles
+    double* tab_mixed = nullptr;   // [M][N]   (DST index, DCT index)
+    double* tab_mixed_t = nullptr; // [N][M]
+    double* tab_dir = nullptr;     // [M][M]
+    double* tab_neu = nullptr;     // [N][N]
+    double* colsum = nullptr;      // [N]
+    double* T = nullptr;           // 6 * N * N transform scratch
+    double* partial = nullptr;     // [N]
+    double* p1[5] = {};            // predictor particles (nc * nc)
+    double* p2[5] = {};            // corrector particles
+    double* dep[4] = {};           // ro jx jy jz of passes 1 and 2
+    double* favg[6] = {};          // field averages of the current pass
+    int* fault = nullptr;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int dev_alloc(lcode_workspace* ws, void** p, size_t bytes) {
+    LCODE_HIP(hipMalloc(p, bytes));
+    ws->allocs.push_back(*p);
+    return LCODE_OK;
+}
+
+template <typename T>
+int upload(lcode_workspace* ws, T** dst, const std::vector<T>& src) {
+    int rc = dev_alloc(ws, (void**)dst, src.size() * sizeof(T));
+    if (rc) return rc;
+    LCODE_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T),
+                        hipMemcpyHostToDevice));
+    return LCODE_OK;
+}
+
+bool factor_radices(int L, FftPlan& plan) {
+    plan.L = L;
+    plan.nstages = 0;
+    int rest = L;
+    const int radices[] = {4, 2, 3, 5};
+    for (int r : radices) {
+        while (rest % r == 0) {
+            if (plan.nstages == kMaxStages) return false;
+            plan.radix[plan.nstages++] = r;
+            rest /= r;
+        }
+    }
+    return rest == 1;
+}
+
+// Second-difference eigenvalues 4/h^2 sin^2(k pi / (2 (N - 1)))
+// (lcode.py:152-154, :227-228, :326), in the reference's FP64 arithmetic.
+double lam(int k, int N, double h) {
+    const double s = std::sin(k * M_PI / (2 * (N - 1)));
+    return 4 / (h * h) * (s * s);
+}
+
+int check_cfg(const lcode_config* c) {
+    LCODE_REQUIRE(c, "config is NULL");
+    LCODE_REQUIRE(c->grid_steps >= 5 && c->grid_steps % 2 == 1,
+                  "grid_steps must be odd and >= 5");
+    LCODE_REQUIRE(c->grid_step_size > 0 && c->xi_step_size > 0,
+                  "grid_step_size and xi_step_size must be positive");
+    LCODE_REQUIRE(c->nc >= 0 && c->nf >= 0, "negative particle counts");
+    LCODE_REQUIRE(c->plasma_coarseness > 0 && c->plasma_fineness > 0,
+                  "plasma_coarseness and plasma_fineness must be positive");
+    LCODE_REQUIRE(c->solver == LCODE_SOLVER_FFT64, "unknown solver");
+    return LCODE_OK;
+}
+
+int check_ws(const lcode_workspace* ws, const lcode_config* c) {
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    LCODE_REQUIRE(ws, "workspace is NULL");
+    LCODE_REQUIRE(ws->N == c->grid_steps && ws->h == c->grid_step_size &&
+                      ws->trick_on == (c->subtraction_trick != 0),
+                  "workspace was built for another grid/trick");
+    return LCODE_OK;
+}
+
+// Field plans ------------------------------------------------------------
+
+FieldPlan plan_for(lcode_workspace* ws, int kind, int slot) {
+    const int N = ws->N, M = ws->M;
+    FieldPlan p{};
+    p.kind = kind;
+    p.T = ws->T + (size_t)slot * N * N;
+    switch (kind) {
+        case F_EY:
+        case F_BX:  // DST along x (rows 1..N-2), DCT along y (lcode.py:275-286)
+            p.rowsA0 = 1; p.rowsA = M;
+            p.xfA = XF_DCT; p.lenA = N; p.colA0 = 0;
+            p.xfB = XF_DST; p.table = ws->tab_mixed_t;
+            break;
+        case F_EX:
+        case F_BY:  // solved on the transposed RHS (lcode.py:287-288)
+            p.rowsA0 = 0; p.rowsA = N;
+            p.xfA = XF_DST; p.lenA = M; p.colA0 = 1;
+            p.xfB = XF_DCT; p.table = ws->tab_mixed;
+            break;
+        case F_EZ:  // DST-I both ways on the interior (lcode.py:166-182)
+            p.rowsA0 = 1; p.rowsA = M;
+            p.xfA = XF_DST; p.lenA = M; p.colA0 = 1;
+            p.xfB = XF_DST; p.table = ws->tab_dir;
+            break;
+        default:    // F_BZ: DCT-I both ways, zero mode dropped (:339-362)
+            p.rowsA0 = 0; p.rowsA = N;
+            p.xfA = XF_DCT; p.lenA = N; p.colA0 = 0;
+            p.xfB = XF_DCT; p.table = ws->tab_neu;
+            p.colsum = ws->colsum;
+            p.partial = ws->partial;
+            break;
+    }
+    return p;
+}
+
+RhsInputs rhs_base(const lcode_config* c) {
+    RhsInputs in{};
+    in.trick = c->subtraction_trick;
+    in.h = c->grid_step_size;
+    in.dxi = c->xi_step_size;
+    return in;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lcode_last_error(void) { return g_last_error.c_str(); }
+
+const char* lcode_version(void) { return "lcode3d_hip 0.1 (gfx950, FP64)"; }
+
+int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    LCODE_REQUIRE(out, "out is NULL");
+    *out = nullptr;
+    auto* ws = new lcode_workspace();
+    auto fail = [&](int code) {
+        lcode_workspace_destroy(ws);
+        return code;
+    };
+    const int N = cfg->grid_steps, M = N - 2;
+    ws->N = N;
+    ws->M = M;
+    ws->h = cfg->grid_step_size;
+    ws->trick_on = cfg->subtraction_trick != 0;
+    ws->nc = cfg->nc;
+    ws->nf = cfg->nf;
+    if (hipGetDevice(&ws->device) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        return fail(LCODE_E_HIP);
+    }
+
+    // FFT geometry: one complex FFT of L = 2 (N - 1) points per line pair.
+    SolverGeom& g = ws->geom;
+    g.N = N;
+    const int L = 2 * (N - 1);
+    if (!factor_radices(L, g.fft)) {
+        set_error("2 (grid_steps - 1) = " + std::to_string(L) +
+                  " has a prime factor other than 2, 3, 5");
+        return fail(LCODE_E_UNSUPPORTED);
+    }
+    g.Tp = ((L + 3) / 4 + 63) / 64 * 64;
+    int P = 1024 / g.Tp;
+    const int by_lds = 65536 / (int)(L * sizeof(double2));
+    if (by_lds < P) P = by_lds;
+    if (P > 4) P = 4;
+    if (P < 1) {
+        set_error("grid too large for one FFT line pair in 64 KiB of LDS");
+        return fail(LCODE_E_UNSUPPORTED);
+    }
+    g.P = P;
+    if ((N + g.Tp - 1) / g.Tp > 4) {
+        set_error("internal: too many points per thread");
+        return fail(LCODE_E_UNSUPPORTED);
+    }
+
+    std::vector<double2> W(L);
+    for (int t = 0; t < L; ++t) {
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * t / L;
+        W[t] = make_double2((double)std::cos(a), (double)-std::sin(a));
+    }
+    if ((rc = upload(ws, &ws->W, W))) return fail(rc);
+    g.W = ws->W;
+
+    // Eigenvalue tables, normalisation 1 / (2 (N - 1))^2 folded in
+    // (lcode.py:144-157, :215-231, :316-330).
+    const double norm = (double)(2 * (N - 1)) * (2 * (N - 1));
+    std::vector<double> li(M), lj(N);
+    for (int k = 1; k <= M; ++k) li[k - 1] = lam(k, N, ws->h);
+    for (int k = 0; k < N; ++k) lj[k] = lam(k, N, ws->h);
+    const double add = ws->trick_on ? 1 : 0;
+    std::vector<double> mixed((size_t)M * N), mixed_t((size_t)N * M);
+    for (int a = 0; a < M; ++a)
+        for (int b = 0; b < N; ++b) {
+            const double v = 1 / (li[a] + lj[b] + add) / norm;
+            mixed[(size_t)a * N + b] = v;
+            mixed_t[(size_t)b * M + a] = v;
+        }
+    std::vector<double> dir((size_t)M * M), neu((size_t)N * N);
+    for (int a = 0; a < M; ++a)
+        for (int b = 0; b < M; ++b)
+            dir[(size_t)a * M + b] = 1 / (li[a] + li[b]) / norm;
+    for (int a = 0; a < N; ++a)
+        for (int b = 0; b < N; ++b)
+            neu[(size_t)a * N + b] = (a == 0 && b == 0) ? 0. : 1 / (lj[a] + lj[b]) / norm;
+    // Column sums of the final DCT-I of Bz: the mean of Bz is
+    // sum_k colsum[k] sum_i T[k][i] / N^2.
+    std::vector<double> colsum(N);
+    for (int k = 0; k < N; ++k) {
+        long double s = 0;
+        for (int j = 0; j < N; ++j)
+            s += std::cos((long double)3.14159265358979323846264338327950288L * k * j / (N - 1));
+        const long double c = (k == 0 || k == N - 1) ? 1 : 2;
+        colsum[k] = (double)(c * s);
+    }
+    if ((rc = upload(ws, &ws->tab_mixed, mixed))) return fail(rc);
+    if ((rc = upload(ws, &ws->tab_mixed_t, mixed_t))) return fail(rc);
+    if ((rc = upload(ws, &ws->tab_dir, dir))) return fail(rc);
+    if ((rc = upload(ws, &ws->tab_neu, neu))) return fail(rc);
+    if ((rc = upload(ws, &ws->colsum, colsum))) return fail(rc);
+
+    const size_t n2 = (size_t)N * N, np = (size_t)cfg->nc * cfg->nc;
+    if ((rc = dev_alloc(ws, (void**)&ws->T, 6 * n2 * sizeof(double)))) return fail(rc);
+    if ((rc = dev_alloc(ws, (void**)&ws->partial, N * sizeof(double)))) return fail(rc);
+    for (int k = 0; k < 5; ++k) {
+        if ((rc = dev_alloc(ws, (void**)&ws->p1[k], (np ? np : 1) * sizeof(double)))) return fail(rc);
+        if ((rc = dev_alloc(ws, (void**)&ws->p2[k], (np ? np : 1) * sizeof(double)))) return fail(rc);
+    }
+    for (int k = 0; k < 4; ++k)
+        if ((rc = dev_alloc(ws, (void**)&ws->dep[k], n2 * sizeof(double)))) return fail(rc);
+    for (int k = 0; k < 6; ++k)
+        if ((rc = dev_alloc(ws, (void**)&ws->favg[k], n2 * sizeof(double)))) return fail(rc);
+    if ((rc = dev_alloc(ws, (void**)&ws->fault, sizeof(int)))) return fail(rc);
+    if (hipMemset(ws->fault, 0, sizeof(int)) != hipSuccess) {
+        set_error("hipMemset of the fault counter failed");
+        return fail(LCODE_E_HIP);
+    }
+    *out = ws;
+    return LCODE_OK;
+}
+
+int lcode_workspace_destroy(lcode_workspace* ws) {
+    if (!ws) return LCODE_OK;
+    int rc = LCODE_OK;
+    for (void* p : ws->allocs)
+        if (hipFree(p) != hipSuccess) rc = LCODE_E_HIP;
+    delete ws;
+    if (rc) set_error("hipFree failed in lcode_workspace_destroy");
+    return rc;
+}
+
+int lcode_workspace_fault_count(lcode_workspace* ws, hipStream_t stream,
+                                int64_t* count) {
+    LCODE_REQUIRE(ws && count, "NULL argument");
+    int host = 0;
+    LCODE_HIP(hipMemcpyAsync(&host, ws->fault, sizeof(int),
+                             hipMemcpyDeviceToHost, stream));
+    LCODE_HIP(hipStreamSynchronize(stream));
+    LCODE_HIP(hipMemsetAsync(ws->fault, 0, sizeof(int), stream));
+    *count = host;
+    return LCODE_OK;
+}
+
+int lcode_move_estimate_wo_fields(lcode_workspace* ws, const lcode_config* cfg,
+                                  const double* m, const double* x_init,
+                                  const double* y_init,
+                                  const double* prev_x_offt,
+                                  const double* prev_y_offt, const double* px,
+                                  const double* py, const double* pz,
+                                  double* x_offt_out, double* y_offt_out,
+                                  hipStream_t stream) {
+    int rc = check_ws(ws, cfg);
+    if (rc) return rc;
+    EstimateArgs a{};
+    a.n = cfg->nc * cfg->nc;
+    a.dxi = cfg->xi_step_size;
+    a.reflect = cfg->reflect_boundary;
+    a.m = m; a.x_init = x_init; a.y_init = y_init;
+    a.prev_x_offt = prev_x_offt; a.prev_y_offt = prev_y_offt;
+    a.px = px; a.py = py; a.pz = pz;
+    a.out_x = x_offt_out; a.out_y = y_offt_out;
+    LCODE_REQUIRE(a.n == 0 || (m && x_init && y_init && prev_x_offt && prev_y_offt &&
+                               px && py && pz && x_offt_out && y_offt_out),
+                  "NULL array");
+    return launch_estimate(a, stream);
+}
+
+static int do_push(lcode_workspace* ws, const lcode_config* cfg,
+                   const double* m, const double* q, const double* x_init,
+                   const double* y_init, const double* xo, const double* yo,
+                   const double* ex, const double* ey, const double* px,
+                   const double* py, const double* pz,
+                   const double* const F[6], double* const out[5],
+                   hipStream_t stream) {
+    PushArgs a{};
+    a.n = cfg->nc * cfg->nc;
+    a.N = cfg->grid_steps;
+    a.h = cfg->grid_step_size;
+    a.dxi = cfg->xi_step_size;
+    a.reflect = cfg->reflect_boundary;
+    a.m = m; a.q = q; a.x_init = x_init; a.y_init = y_init;
+    a.prev_x_offt = xo; a.prev_y_offt = yo;
+    a.est_x = ex; a.est_y = ey;
+    a.px = px; a.py = py; a.pz = pz;
+    for (int k = 0; k < 6; ++k) a.F[k] = F[k];
+    a.out_x = out[0]; a.out_y = out[1];
+    a.out_px = out[2]; a.out_py = out[3]; a.out_pz = out[4];
+    a.fault = ws->fault;
+    return launch_push(a, stream);
+}
+
+int lcode_move_smart(lcode_workspace* ws, const lcode_config* cfg,
+                     const double* m, const double* q, const double* x_init,
+                     const double* y_init, const double* x_prev_offt,
+                     const double* y_prev_offt,
+                     const double* estimated_x_offt,
+                     const double* estimated_y_offt, const double* px_prev,
+                     const double* py_prev, const double* pz_prev,
+                     const double* const fields[6], double* const out[5],
+                     hipStream_t stream) {
+    int rc = check_ws(ws, cfg);
+    if (rc) return rc;
+    LCODE_REQUIRE(fields && out, "NULL fields/out");
+    LCODE_REQUIRE(estimated_x_offt && estimated_y_offt,
+                  "estimated offsets are required");
+    for (int k = 0; k < 6; ++k) LCODE_REQUIRE(fields[k], "NULL field");
+    for (int k = 0; k < 5; ++k) LCODE_REQUIRE(out[k], "NULL output");
+    return do_push(ws, cfg, m, q, x_init, y_init, x_prev_offt, y_prev_offt,
+                   estimated_x_offt, estimated_y_offt, px_prev, py_prev,
+                   pz_prev, fields, out, stream);
+}
+
+static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
+                      const double* ro_initial, const double* xo,
+                      const double* yo, const double* m, const double* q,
+                      const double* px, const double* py, const double* pz,
+                      const lcode_virt_params* vp, double* const out[4],
+                      hipStream_t stream) {
+    DepositArgs a{};
+    a.nf = cfg->nf;
+    a.nc = cfg->nc;
+    a.N = cfg->grid_steps;
+    a.h = cfg->grid_step_size;
+    const double cf = (double)cfg->plasma_coarseness * cfg->plasma_fineness;
+    a.smallness = 1 / (cf * cf);               // lcode.py:672-673
+    a.x_offt = xo; a.y_offt = yo; a.m = m; a.q = q;
+    a.px = px; a.py = py; a.pz = pz;
+    a.ip = vp->influence_prev; a.in = vp->influence_next; a.fg = vp->fine_grid;
+    a.P = vp->indices_prev; a.Nn = vp->indices_next;
+    a.ro_initial = ro_initial;
+    a.ro = out[0]; a.jx = out[1]; a.jy = out[2]; a.jz = out[3];
+    a.fault = ws->fault;
+    return launch_deposit(a, stream);
+}
+
+int lcode_deposit(lcode_workspace* ws, const lcode_config* cfg,
+                  const double* ro_initial, const double* x_offt,
+                  const double* y_offt, const double* m, const double* q,
+                  const double* px, const double* py, const double* pz,
+                  const lcode_virt_params* virt_params, double* const out[4],
+                  hipStream_t stream) {
+    int rc = check_ws(ws, cfg);
+    if (rc) return rc;
+    LCODE_REQUIRE(virt_params && out, "NULL virt_params/out");
+    for (int k = 0; k < 4; ++k) LCODE_REQUIRE(out[k], "NULL output");
+    return do_deposit(ws, cfg, ro_initial, x_offt, y_offt, m, q, px, py, pz,
+                      virt_params, out, stream);
+}
+
+int lcode_calculate_Ex_Ey_Bx_By(lcode_workspace* ws, const lcode_config* cfg,
+                                const double* Ex_avg, const double* Ey_avg,
+                                const double* Bx_avg, const double* By_avg,
+                                const double* beam_ro, double beam_scalar,
+                                const double* ro, const double* jx,
+                                const double* jy, const double* jz,
+                                const double* jx_prev, const double* jy_prev,
+                                double* const out[4], hipStream_t stream) {
+    int rc = check_ws(ws, cfg);
+    if (rc) return rc;
+    LCODE_REQUIRE(Ex_avg && Ey_avg && Bx_avg && By_avg && ro && jx && jy &&
+                      jz && jx_prev && jy_prev && out,
+                  "NULL array");
+    SolveBatch b{};
+    b.nfields = 4;
+    const int kinds[4] = {F_EX, F_EY, F_BX, F_BY};
+    for (int k = 0; k < 4; ++k) {
+        LCODE_REQUIRE(out[k], "NULL output");
+        b.f[k] = plan_for(ws, kinds[k], k);
+        b.f[k].outF = out[k];
+    }
+    b.in = rhs_base(cfg);
+    b.in.ro = ro; b.in.jx = jx; b.in.jy = jy; b.in.jz = jz;
+    b.in.jx_prev = jx_prev; b.in.jy_prev = jy_prev;
+    b.in.Ftrick[0] = Ex_avg; b.in.Ftrick[1] = Ey_avg;
+    b.in.Ftrick[2] = Bx_avg; b.in.Ftrick[3] = By_avg;
+    b.in.beam = beam_ro;
+    b.in.beam_scalar = beam_scalar;
+    b.variantA = 0;
+    return launch_solve(ws->geom, b, stream);
+}
+
+static int solve_single(lcode_workspace* ws, const lcode_config* cfg,
+                        int kind, const double* jx, const double* jy,
+                        double* outF, hipStream_t stream) {
+    int rc = check_ws(ws, cfg);
+    if (rc) return rc;
+    LCODE_REQUIRE(jx && jy && outF, "NULL array");
+    SolveBatch b{};
+    b.nfields = 1;
+    b.f[0] = plan_for(ws, kind, 0);
+    b.f[0].outF = outF;
+    b.in = rhs_base(cfg);
+    b.in.jx = jx;
+    b.in.jy = jy;
+    return launch_solve(ws->geom, b, stream);
+}
+
+int lcode_calculate_Ez(lcode_workspace* ws, const lcode_config* cfg,
+                       const double* jx, const double* jy, double* Ez,
+                       hipStream_t stream) {
+    return solve_single(ws, cfg, F_EZ, jx, jy, Ez, stream);
+}
+
+int lcode_calculate_Bz(lcode_workspace* ws, const lcode_config* cfg,
+                       const double* jx, const double* jy, double* Bz,
+                       hipStream_t stream) {
+    return solve_single(ws, cfg, F_BZ, jx, jy, Bz, stream);
+}
+
+// Field half of a predictor/corrector pass (lcode.py:862-881, :893-911):
+// the six solves in one batch; F = solved fields (or NULL), avg = (F + prev)/2.
+static int field_pass(lcode_workspace* ws, const lcode_config* cfg,
+                      const lcode_state* prev, const double* const trick[4],
+                      const double* beam_ro, double beam_scalar,
+                      double* const dep[4], double* const F[6],
+                      double* const avg[6], hipStream_t stream) {
+    SolveBatch b{};
+    b.nfields = 6;
+    const double* prevF[6] = {prev->Ex, prev->Ey, prev->Ez,
+                              prev->Bx, prev->By, prev->Bz};
+    for (int k = 0; k < 6; ++k) {
+        b.f[k] = plan_for(ws, k, k);   // slot k, kind k (F_EX..F_BZ order)
+        b.f[k].prevF = prevF[k];
+        b.f[k].outF = F ? F[k] : nullptr;
+        b.f[k].outAvg = avg[k];
+    }
+    b.in = rhs_base(cfg);
+    b.in.ro = dep[0]; b.in.jx = dep[1]; b.in.jy = dep[2]; b.in.jz = dep[3];
+    if (cfg->variant_A) {
+        b.in.ro_prev = prev->ro;
+        b.in.jz_prev = prev->jz;
+    }
+    b.in.jx_prev = prev->jx;
+    b.in.jy_prev = prev->jy;
+    for (int k = 0; k < 4; ++k) b.in.Ftrick[k] = trick[k];
+    b.in.beam = beam_ro;
+    b.in.beam_scalar = beam_scalar;
+    b.variantA = cfg->variant_A ? 1 : 0;
+    return launch_solve(ws->geom, b, stream);
+}
+
+int lcode_step(lcode_workspace* ws, const lcode_config* cfg,
+               const lcode_const* c, const lcode_virt_params* vp,
+               const lcode_state* prev, const double* beam_ro,
+               double beam_scalar, const lcode_state* out,
+               hipStream_t stream) {
+    int rc = check_ws(ws, cfg);
+    if (rc) return rc;
+    LCODE_REQUIRE(c && vp && prev && out, "NULL struct");
+    LCODE_REQUIRE(cfg->nc == ws->nc, "workspace was built for another nc");
+    LCODE_REQUIRE(prev->Ex != out->Ex && prev->x_offt != out->x_offt,
+                  "prev and out must not alias");
+    const double* prevF[6] = {prev->Ex, prev->Ey, prev->Ez,
+                              prev->Bx, prev->By, prev->Bz};
+
+    // 1-2: field-free estimate fused into the predictor push with prev
+    // fields (lcode.py:843-854).
+    if ((rc = do_push(ws, cfg, c->m, c->q, c->x_init, c->y_init,
+                      prev->x_offt, prev->y_offt, nullptr, nullptr,
+                      prev->px, prev->py, prev->pz, prevF, ws->p1, stream)))
+        return rc;
+    // 3: deposit (lcode.py:856-859)
+    if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p1[0], ws->p1[1], c->m,
+                         c->q, ws->p1[2], ws->p1[3], ws->p1[4], vp, ws->dep,
+                         stream)))
+        return rc;
+    // 4: fields with prev fields in the trick term; only the averages are
+    // used afterwards (lcode.py:862-881)
+    const double* trick_prev[4] = {prev->Ex, prev->Ey, prev->Bx, prev->By};
+    if ((rc = field_pass(ws, cfg, prev, trick_prev, beam_ro, beam_scalar,
+                         ws->dep, nullptr, ws->favg, stream)))
+        return rc;
+    // 5: corrector push with the averaged fields from the pass-1 positions
+    // (lcode.py:884-889)
+    const double* avg[6] = {ws->favg[0], ws->favg[1], ws->favg[2],
+                            ws->favg[3], ws->favg[4], ws->favg[5]};
+    if ((rc = do_push(ws, cfg, c->m, c->q, c->x_init, c->y_init,
+                      prev->x_offt, prev->y_offt, ws->p1[0], ws->p1[1],
+                      prev->px, prev->py, prev->pz, avg, ws->p2, stream)))
+        return rc;
+    // 6: deposit (lcode.py:890-891)
+    if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p2[0], ws->p2[1], c->m,
+                         c->q, ws->p2[2], ws->p2[3], ws->p2[4], vp, ws->dep,
+                         stream)))
+        return rc;
+    // 7: fields with the pass-1 averages in the trick term; the solved
+    // fields are the step's output (lcode.py:893-911, :927-928).  The
+    // trick inputs are read in pass A, before pass C overwrites favg.
+    const double* trick_avg[4] = {ws->favg[0], ws->favg[1], ws->favg[3],
+                                  ws->favg[4]};
+    double* const outF[6] = {out->Ex, out->Ey, out->Ez,
+                             out->Bx, out->By, out->Bz};
+    if ((rc = field_pass(ws, cfg, prev, trick_avg, beam_ro, beam_scalar,
+                         ws->dep, outF, ws->favg, stream)))
+        return rc;
+    // 8: final push with the pass-2 averages from the pass-2 positions
+    // (lcode.py:914-919)
+    double* const outP[5] = {out->x_offt, out->y_offt, out->px, out->py,
+                             out->pz};
+    if ((rc = do_push(ws, cfg, c->m, c->q, c->x_init, c->y_init,
+                      prev->x_offt, prev->y_offt, ws->p2[0], ws->p2[1],
+                      prev->px, prev->py, prev->pz, avg, outP, stream)))
+        return rc;
+    // 9: final deposit into the output (lcode.py:920-921)
+    double* const outD[4] = {out->ro, out->jx, out->jy, out->jz};
+    return do_deposit(ws, cfg, c->ro_initial, out->x_offt, out->y_offt, c->m,
+                      c->q, out->px, out->py, out->pz, vp, outD, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+// Shared device helpers and internal launcher declarations of the gfx950
+// LCODE 3D step library.  Everything here is FP64: positions, momenta,
+// grids and transforms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "../../include/lcode3d_hip.h"
+
+namespace lcode {
+
+// ---------------------------------------------------------------------------
+// Error reporting
+// ---------------------------------------------------------------------------
+
+void set_error(const std::string& msg);
+
+#define LCODE_HIP(expr)                                                     \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess) {                                             \
+            ::lcode::set_error(std::string(#expr) + ": " +                  \
+                               hipGetErrorString(e_));                      \
+            return LCODE_E_HIP;                                             \
+        }                                                                   \
+    } while (0)
+
+#define LCODE_REQUIRE(cond, msg)                                            \
+    do {                                                                    \
+        if (!(cond)) {                                                      \
+            ::lcode::set_error(msg);                                        \
+            return LCODE_E_INVALID;                                         \
+        }                                                                   \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// TSC-2D shape (lcode.py:396-418)
+// ---------------------------------------------------------------------------
+
+// The 9 stencil points in the reference's weight order (lcode.py:427-429):
+// (-1,+1) (0,+1) (+1,+1) (-1,0) (0,0) (+1,0) (-1,-1) (0,-1) (+1,-1).
+__device__ __forceinline__ int stencil_di(int k) { return (k % 3) - 1; }
+__device__ __forceinline__ int stencil_dj(int k) { return 1 - (k / 3); }
+
+struct Tsc {
+    int i, j;      // centre cell, clamped to [1, N-2]
+    double w[9];   // wMP w0P wPP wM0 w00 wP0 wMM w0M wPM
+};
+
+// Cell index and quadratic-B-spline weights of the point (x, y).  A centre
+// cell whose 3x3 stencil would leave the grid is clamped into it and
+// counted in *fault (the reference has no check: there it is an
+// out-of-bounds access).
+__device__ __forceinline__ void tsc_weights(double x, double y, int N,
+                                            double h, Tsc& t, int* fault) {
+    const double x_h = x / h + .5, y_h = y / h + .5;
+    const double fx = floor(x_h), fy = floor(y_h);
+    int i = (int)fx + N / 2, j = (int)fy + N / 2;
+    const double x_loc = x_h - fx - .5, y_loc = y_h - fy - .5;
+    if (!(i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2)) {
+        atomicAdd(fault, 1);
+        i = i < 1 ? 1 : (i > N - 2 ? N - 2 : i);
+        j = j < 1 ? 1 : (j > N - 2 ? N - 2 : j);
+    }
+    t.i = i;
+    t.j = j;
+    const double wx0 = .75 - x_loc * x_loc, wy0 = .75 - y_loc * y_loc;
+    const double wxP = (.5 + x_loc) * (.5 + x_loc) / 2;
+    const double wyP = (.5 + y_loc) * (.5 + y_loc) / 2;
+    const double wxM = (.5 - x_loc) * (.5 - x_loc) / 2;
+    const double wyM = (.5 - y_loc) * (.5 - y_loc) / 2;
+    t.w[0] = wxM * wyP; t.w[1] = wx0 * wyP; t.w[2] = wxP * wyP;
+    t.w[3] = wxM * wy0; t.w[4] = wx0 * wy0; t.w[5] = wxP * wy0;
+    t.w[6] = wxM * wyM; t.w[7] = wx0 * wyM; t.w[8] = wxP * wyM;
+}
+
+// 9-point gather in the reference's summation order (lcode.py:421-430).
+__device__ __forceinline__ double interp9(const double* __restrict__ a,
+                                          int N, const Tsc& t) {
+    const double* c = a + (size_t)t.i * N + t.j;
+    return c[-N + 1] * t.w[0] + c[1] * t.w[1] + c[N + 1] * t.w[2] +
+           c[-N] * t.w[3] + c[0] * t.w[4] + c[N] * t.w[5] +
+           c[-N - 1] * t.w[6] + c[-1] * t.w[7] + c[N - 1] * t.w[8];
+}
+
+// ---------------------------------------------------------------------------
+// Internal launchers (particles.hip, fields.hip)
+// ---------------------------------------------------------------------------
+
+struct PushArgs {
+    int n;                 // particles
+    int N;                 // grid_steps
+    double h, dxi, reflect;
+    const double *m, *q, *x_init, *y_init, *prev_x_offt, *prev_y_offt;
+    const double *est_x, *est_y;   // NULL: estimate inline (lcode.py:369)
+    const double *px, *py, *pz;
+    const double *F[6];            // Ex Ey Ez Bx By Bz (averaged)
+    double *out_x, *out_y, *out_px, *out_py, *out_pz;
+    int* fault;
+};
+
+struct EstimateArgs {
+    int n;
+    double dxi, reflect;
+    const double *m, *x_init, *y_init, *prev_x_offt, *prev_y_offt;
+    const double *px, *py, *pz;
+    double *out_x, *out_y;
+};
+
+struct DepositArgs {
+    int nf, nc, N;
+    double h, smallness;
+    const double *x_offt, *y_offt, *m, *q, *px, *py, *pz;
+    const double *ip, *in, *fg;
+    const int64_t *P, *Nn;
+    const double* ro_initial;      // may be NULL
+    double *ro, *jx, *jy, *jz;
+    int* fault;
+};
+
+int launch_estimate(const EstimateArgs& a, hipStream_t s);
+int launch_push(const PushArgs& a, hipStream_t s);
+int launch_deposit(const DepositArgs& a, hipStream_t s);
+
+// Field solves ---------------------------------------------------------------
+
+enum FieldKind { F_EX = 0, F_EY = 1, F_EZ = 2, F_BX = 3, F_BY = 4, F_BZ = 5 };
+enum Xform { XF_DCT = 0, XF_DST = 1 };
+
+constexpr int kMaxStages = 24;
+
+struct FftPlan {
+    int L;                       // 2 (N - 1)
+    int nstages;
+    int radix[kMaxStages];
+};
+
+// Sources of the right-hand sides (lcode.py:166-168, :258-268, :339-343).
+struct RhsInputs {
+    const double *ro, *jx, *jy, *jz;      // this pass's densities
+    const double *ro_prev, *jz_prev;      // non-NULL: Variant A averaging
+    const double *jx_prev, *jy_prev;      // previous slice (dj/dxi)
+    const double *Ftrick[4];              // Ex Ey Bx By in the trick term
+    const double* beam;                   // NULL: beam_scalar everywhere
+    double beam_scalar;
+    double trick, h, dxi;
+};
+
+struct FieldPlan {
+    int kind;                 // FieldKind
+    int rowsA0, rowsA;        // rows i of the RHS transformed along j
+    int xfA, lenA, colA0;     // transform along j: type, length, first j
+    int xfB;                  // transform along i (length rowsA)
+    const double* table;      // [lenA][rowsA] spectral multiplier
+    double* T;                // [lenA][rowsA] scratch
+    const double* colsum;     // Bz only: column sums of the last DCT
+    double* partial;          // Bz only: [lenA] mean partials
+    const double* prevF;      // Variant A / averaging source (or NULL)
+    double* outF;             // solved field (or NULL)
+    double* outAvg;           // (F + prevF) / 2 (or NULL)
+};
+
+struct SolveBatch {
+    int nfields;
+    FieldPlan f[6];
+    RhsInputs in;
+    int variantA;             // 2 F - prev for Ex Ey Bx By
+};
+
+struct SolverGeom {
+    int N;
+    FftPlan fft;
+    const double2* W;         // exp(-2 pi i t / L), t in [0, L)
+    int P, Tp;                // pairs of lines per block, threads per pair
+};
+
+int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s);
+
+}  // namespace lcode

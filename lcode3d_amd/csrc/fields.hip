@@ -1,0 +1,445 @@
+// Field solves of the xi step (lcode.py:122-364) on gfx950, FP64.
+//
+// Every solve is out = T_x T_y (Mat o (T_y T_x rhs)) with T_x, T_y type-1
+// sine or cosine transforms along x (i) and y (j).  The transforms commute,
+// so a solve runs as three passes over lines, batched over all fields:
+//
+//   pass A  one RHS row i per line: assemble the RHS on the fly, transform
+//           along j, store transposed: T[k][i]
+//   pass B  one row k of T per line: transform along i, multiply by the
+//           eigenvalue table, transform along i again, store back in place
+//   pass C  one output row i per line: gather the column T[.][i], transform
+//           along j, then the epilogue (Bz mean, Variant A, averaging)
+//
+// Each type-1 transform of length n is one complex FFT of length
+// L = 2 (n + 1) (sine) or 2 (n - 1) (cosine) of the odd or even extension;
+// two real lines ride in one complex FFT (real and imaginary parts).  All
+// six fields of a step share L = 2 (N - 1).  The FFT is a mixed-radix
+// (4, 2, 3, 5) Stockham transform in LDS.
+
+#include "common.hpp"
+
+namespace lcode {
+
+namespace {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 csub(double2 a, double2 b) {
+    return make_double2(a.x - b.x, a.y - b.y);
+}
+// multiply by -i
+__device__ __forceinline__ double2 mul_mi(double2 a) {
+    return make_double2(a.y, -a.x);
+}
+
+// Forward DFT of R points held in registers: y_p = sum_q x_q e^{-2 pi i pq/R}.
+template <int R> struct Dft;
+
+template <> struct Dft<2> {
+    static __device__ __forceinline__ void run(double2* v) {
+        const double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+};
+
+template <> struct Dft<4> {
+    static __device__ __forceinline__ void run(double2* v) {
+        const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const double2 s13 = cadd(v[1], v[3]), d13 = mul_mi(csub(v[1], v[3]));
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    }
+};
+
+template <> struct Dft<3> {
+    static __device__ __forceinline__ void run(double2* v) {
+        constexpr double s = 0.86602540378443864676;  // sin(2 pi / 3)
+        const double2 t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const double2 base = make_double2(v[0].x - .5 * t.x, v[0].y - .5 * t.y);
+        const double2 rot = make_double2(s * d.y, -s * d.x);  // -i s d
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(base, rot);
+        v[2] = csub(base, rot);
+    }
+};
+
+template <> struct Dft<5> {
+    static __device__ __forceinline__ void run(double2* v) {
+        constexpr double c1 = 0.30901699437494742410;   // cos(2 pi / 5)
+        constexpr double c2 = -0.80901699437494742410;  // cos(4 pi / 5)
+        constexpr double s1 = 0.95105651629515357212;   // sin(2 pi / 5)
+        constexpr double s2 = 0.58778525229247312917;   // sin(4 pi / 5)
+        const double2 x0 = v[0];
+        const double2 t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+        const double2 t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+        const double2 a1 = make_double2(x0.x + c1 * t1.x + c2 * t2.x,
+                                        x0.y + c1 * t1.y + c2 * t2.y);
+        const double2 a2 = make_double2(x0.x + c2 * t1.x + c1 * t2.x,
+                                        x0.y + c2 * t1.y + c1 * t2.y);
+        const double2 b1 = make_double2(s1 * d1.x + s2 * d2.x,
+                                        s1 * d1.y + s2 * d2.y);
+        const double2 b2 = make_double2(s2 * d1.x - s1 * d2.x,
+                                        s2 * d1.y - s1 * d2.y);
+        v[0] = cadd(x0, cadd(t1, t2));
+        v[1] = cadd(a1, mul_mi(b1));
+        v[4] = csub(a1, mul_mi(b1));
+        v[2] = cadd(a2, mul_mi(b2));
+        v[3] = csub(a2, mul_mi(b2));
+    }
+};
+
+// One radix-R Stockham stage over the L points of this thread's pair.
+// Thread t of Tp handles butterflies t and t + Tp (Tp >= L / 4 makes two
+// enough for every radix >= 2).  span = product of the previous radices.
+template <int R>
+__device__ __forceinline__ void fft_stage(double2* z,
+                                          const double2* __restrict__ W,
+                                          int L, int span, int t, int Tp) {
+    const int T = L / R;
+    const int tw = L / (span * R);
+    double2 v[2][R];
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int b = t + it * Tp;
+        if (b < T) {
+            const int k = b % span;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double2 x = z[b + q * T];
+                if (q) x = cmul(x, W[q * k * tw]);
+                v[it][q] = x;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const int b = t + it * Tp;
+        if (b < T) {
+            Dft<R>::run(v[it]);
+            const int k = b % span;
+            const int j = (b - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) z[j + q * span] = v[it][q];
+        }
+    }
+    __syncthreads();
+}
+
+// In-place forward FFT of every pair's buffer (block-wide barriers inside:
+// every thread of the block must call it).
+__device__ void fft_lds(double2* z, const FftPlan& plan,
+                        const double2* __restrict__ W, int t, int Tp) {
+    int span = 1;
+    for (int s = 0; s < plan.nstages; ++s) {
+        const int R = plan.radix[s];
+        switch (R) {
+            case 4: fft_stage<4>(z, W, plan.L, span, t, Tp); break;
+            case 2: fft_stage<2>(z, W, plan.L, span, t, Tp); break;
+            case 3: fft_stage<3>(z, W, plan.L, span, t, Tp); break;
+            default: fft_stage<5>(z, W, plan.L, span, t, Tp); break;
+        }
+        span *= R;
+    }
+}
+
+// Extension writers/readers.  Cosine (DCT-I, n points): z[k] and z[L-k]
+// hold input k.  Sine (DST-I, n points): z[k+1] = input k, z[L-1-k] =
+// -input k, z[0] = z[n+1] = 0.  For a pair (a real, b imaginary) the FFT
+// gives DCT(a) = Re Z[k], DCT(b) = Im Z[k]; DST(a) = -Im Z[k+1],
+// DST(b) = Re Z[k+1].
+__device__ __forceinline__ void put_ext(double2* z, int xf, int n, int len,
+                                        int L, double a, double b) {
+    const double2 v = make_double2(a, b);
+    if (xf == XF_DCT) {
+        z[n] = v;
+        if (n >= 1 && n <= len - 2) z[L - n] = v;
+    } else {
+        z[n + 1] = v;
+        z[L - 1 - n] = make_double2(-a, -b);
+    }
+}
+
+__device__ __forceinline__ void put_ext_zeros(double2* z, int xf, int len,
+                                              int t) {
+    if (xf == XF_DST && t == 0) {
+        z[0] = make_double2(0., 0.);
+        z[len + 1] = make_double2(0., 0.);
+    }
+}
+
+__device__ __forceinline__ void get_ext(const double2* z, int xf, int k,
+                                        double& a, double& b) {
+    if (xf == XF_DCT) {
+        const double2 v = z[k];
+        a = v.x;
+        b = v.y;
+    } else {
+        const double2 v = z[k + 1];
+        a = -v.y;
+        b = v.x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand sides (lcode.py:166-168, :234-268, :339-343)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double rho_src(const RhsInputs& in, size_t idx) {
+    double r = in.ro[idx];
+    if (in.ro_prev) r = (r + in.ro_prev[idx]) / 2;     // lcode.py:862
+    return r + (in.beam ? in.beam[idx] : in.beam_scalar);
+}
+
+__device__ __forceinline__ double jz_src(const RhsInputs& in, size_t idx) {
+    double r = in.jz[idx];
+    if (in.jz_prev) r = (r + in.jz_prev[idx]) / 2;     // lcode.py:863
+    return r + (in.beam ? in.beam[idx] : in.beam_scalar);
+}
+
+__device__ __forceinline__ double rhs_value(int kind, int i, int j, int N,
+                                            const RhsInputs& in) {
+    const size_t idx = (size_t)i * N + j;
+    const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
+    const double h2 = in.h * 2;
+    switch (kind) {
+        case F_EX: {
+            const double d = inner ? (rho_src(in, idx + N) - rho_src(in, idx - N)) / h2 : 0.;
+            const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
+            return -((d - dj) - in.Ftrick[0][idx] * in.trick);
+        }
+        case F_EY: {
+            const double d = inner ? (rho_src(in, idx + 1) - rho_src(in, idx - 1)) / h2 : 0.;
+            const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
+            return -((d - dj) - in.Ftrick[1][idx] * in.trick);
+        }
+        case F_BX: {
+            const double d = inner ? (jz_src(in, idx + 1) - jz_src(in, idx - 1)) / h2 : 0.;
+            const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
+            return +((d - dj) + in.Ftrick[2][idx] * in.trick);
+        }
+        case F_BY: {
+            const double d = inner ? (jz_src(in, idx + N) - jz_src(in, idx - N)) / h2 : 0.;
+            const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
+            return -((d - dj) - in.Ftrick[3][idx] * in.trick);
+        }
+        case F_EZ:  // only interior (i, j) reach here
+            return -((in.jx[idx + N] - in.jx[idx - N]) +
+                     (in.jy[idx + 1] - in.jy[idx - 1])) / h2;
+        default: {  // F_BZ
+            const double djx_dy = inner ? in.jx[idx + 1] - in.jx[idx - 1] : 0.;
+            const double djy_dx = inner ? in.jy[idx + N] - in.jy[idx - N] : 0.;
+            return -(djx_dy - djy_dx) / h2;
+        }
+    }
+}
+
+// Deterministic block sum of one value per thread, grouped by pair: the
+// result for pair p is returned to thread 0 of that pair.  red needs one
+// slot per wave.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Passes
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(1024)
+pass_a_kernel(SolverGeom g, SolveBatch b) {
+    extern __shared__ double2 lds[];
+    const FieldPlan& fp = b.f[blockIdx.y];
+    const int Tp = g.Tp, L = g.fft.L, N = g.N;
+    const int lines = fp.rowsA;
+    if ((int)blockIdx.x * g.P * 2 >= lines) return;
+    const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
+    double2* z = lds + (size_t)pair * L;
+    const int ra = ((int)blockIdx.x * g.P + pair) * 2, rb = ra + 1;
+    const bool va = ra < lines, vb = rb < lines;
+    for (int n = t; n < fp.lenA; n += Tp) {
+        const int j = fp.colA0 + n;
+        const double a = va ? rhs_value(fp.kind, fp.rowsA0 + ra, j, N, b.in) : 0.;
+        const double c = vb ? rhs_value(fp.kind, fp.rowsA0 + rb, j, N, b.in) : 0.;
+        put_ext(z, fp.xfA, n, fp.lenA, L, a, c);
+    }
+    put_ext_zeros(z, fp.xfA, fp.lenA, t);
+    __syncthreads();
+    fft_lds(z, g.fft, g.W, t, Tp);
+    for (int k = t; k < fp.lenA; k += Tp) {
+        double a, c;
+        get_ext(z, fp.xfA, k, a, c);
+        if (va) fp.T[(size_t)k * lines + ra] = a;
+        if (vb) fp.T[(size_t)k * lines + rb] = c;
+    }
+}
+
+constexpr int kMaxPerThread = 4;
+
+__global__ void __launch_bounds__(1024)
+pass_b_kernel(SolverGeom g, SolveBatch b) {
+    extern __shared__ double2 lds[];
+    __shared__ double red[2][16];
+    const FieldPlan& fp = b.f[blockIdx.y];
+    const int Tp = g.Tp, L = g.fft.L;
+    const int lines = fp.lenA, len = fp.rowsA, xf = fp.xfB;
+    if ((int)blockIdx.x * g.P * 2 >= lines) return;
+    const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
+    double2* z = lds + (size_t)pair * L;
+    const int la = ((int)blockIdx.x * g.P + pair) * 2, lb = la + 1;
+    const bool va = la < lines, vb = lb < lines;
+    double* rowa = fp.T + (size_t)la * len;
+    double* rowb = fp.T + (size_t)lb * len;
+    const double* taba = fp.table + (size_t)la * len;
+    const double* tabb = fp.table + (size_t)lb * len;
+
+    for (int n = t; n < len; n += Tp)
+        put_ext(z, xf, n, len, L, va ? rowa[n] : 0., vb ? rowb[n] : 0.);
+    put_ext_zeros(z, xf, len, t);
+    __syncthreads();
+    fft_lds(z, g.fft, g.W, t, Tp);
+
+    double ha[kMaxPerThread], hb[kMaxPerThread];
+#pragma unroll
+    for (int u = 0; u < kMaxPerThread; ++u) {
+        const int k = t + u * Tp;
+        ha[u] = hb[u] = 0.;
+        if (k < len) {
+            double a, c;
+            get_ext(z, xf, k, a, c);
+            if (va) ha[u] = a * taba[k];
+            if (vb) hb[u] = c * tabb[k];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kMaxPerThread; ++u) {
+        const int k = t + u * Tp;
+        if (k < len) put_ext(z, xf, k, len, L, ha[u], hb[u]);
+    }
+    put_ext_zeros(z, xf, len, t);
+    __syncthreads();
+    fft_lds(z, g.fft, g.W, t, Tp);
+
+    double sa = 0., sb = 0.;
+    for (int k = t; k < len; k += Tp) {
+        double a, c;
+        get_ext(z, xf, k, a, c);
+        if (va) rowa[k] = a;
+        if (vb) rowb[k] = c;
+        sa += a;
+        sb += c;
+    }
+    if (fp.partial) {
+        // Bz mean: the sum of all Bz equals sum_k colsum[k] * sum_i T[k][i].
+        const int wave = threadIdx.x >> 6;
+        sa = wave_sum(sa);
+        sb = wave_sum(sb);
+        if ((threadIdx.x & 63) == 0) {
+            red[0][wave] = sa;
+            red[1][wave] = sb;
+        }
+        __syncthreads();
+        if (t == 0) {
+            const int w0 = (pair * Tp) >> 6, nw = Tp >> 6;
+            double ta = 0., tb = 0.;
+            for (int w = 0; w < nw; ++w) {
+                ta += red[0][w0 + w];
+                tb += red[1][w0 + w];
+            }
+            if (va) fp.partial[la] = fp.colsum[la] * ta;
+            if (vb) fp.partial[lb] = fp.colsum[lb] * tb;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+pass_c_kernel(SolverGeom g, SolveBatch b) {
+    extern __shared__ double2 lds[];
+    __shared__ double red[16];
+    const FieldPlan& fp = b.f[blockIdx.y];
+    const int Tp = g.Tp, L = g.fft.L, N = g.N;
+    if ((int)blockIdx.x * g.P * 2 >= N) return;
+    const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
+    double2* z = lds + (size_t)pair * L;
+    const int ra = ((int)blockIdx.x * g.P + pair) * 2, rb = ra + 1;
+    const int ca = ra - fp.rowsA0, cb = rb - fp.rowsA0;
+    const bool ha = ra < N && ca >= 0 && ca < fp.rowsA;
+    const bool hb = rb < N && cb >= 0 && cb < fp.rowsA;
+    const int stride = fp.rowsA;
+    for (int n = t; n < fp.lenA; n += Tp) {
+        const double* col = fp.T + (size_t)n * stride;
+        put_ext(z, fp.xfA, n, fp.lenA, L, ha ? col[ca] : 0., hb ? col[cb] : 0.);
+    }
+    put_ext_zeros(z, fp.xfA, fp.lenA, t);
+    __syncthreads();
+    fft_lds(z, g.fft, g.W, t, Tp);
+
+    double mean = 0.;
+    if (fp.partial) {
+        double s = 0.;
+        for (int k = threadIdx.x; k < fp.lenA; k += blockDim.x) s += fp.partial[k];
+        s = wave_sum(s);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+        __syncthreads();
+        double tot = 0.;
+        const int nw = blockDim.x >> 6;
+        for (int w = 0; w < nw; ++w) tot += red[w];
+        mean = tot / ((double)N * N);
+    }
+
+    const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
+                            fp.kind == F_BX || fp.kind == F_BY;
+    const bool varA = b.variantA && transverse;
+    for (int j = t; j < N; j += Tp) {
+        double va = 0., vb = 0.;
+        const int k = j - fp.colA0;
+        if (k >= 0 && k < fp.lenA) get_ext(z, fp.xfA, k, va, vb);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const int i = side ? rb : ra;
+            if (i >= N) continue;
+            const size_t idx = (size_t)i * N + j;
+            double F = side ? vb : va;
+            if (fp.partial) F -= mean;                         // lcode.py:362
+            if (varA) F = 2 * F - fp.prevF[idx];               // lcode.py:869-871
+            if (fp.outF) fp.outF[idx] = F;
+            if (fp.outAvg) fp.outAvg[idx] = (F + fp.prevF[idx]) / 2;  // :876-881
+        }
+    }
+}
+
+}  // namespace
+
+int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s) {
+    if (b.nfields == 0) return LCODE_OK;
+    const size_t lds = (size_t)g.P * g.fft.L * sizeof(double2);
+    const dim3 block(g.P * g.Tp);
+    const int per_block = 2 * g.P;
+    int max_a = 0, max_b = 0;
+    for (int f = 0; f < b.nfields; ++f) {
+        max_a = max_a > b.f[f].rowsA ? max_a : b.f[f].rowsA;
+        max_b = max_b > b.f[f].lenA ? max_b : b.f[f].lenA;
+    }
+    const dim3 ga((max_a + per_block - 1) / per_block, b.nfields);
+    const dim3 gb((max_b + per_block - 1) / per_block, b.nfields);
+    const dim3 gc((g.N + per_block - 1) / per_block, b.nfields);
+    pass_a_kernel<<<ga, block, lds, s>>>(g, b);
+    LCODE_HIP(hipGetLastError());
+    pass_b_kernel<<<gb, block, lds, s>>>(g, b);
+    LCODE_HIP(hipGetLastError());
+    pass_c_kernel<<<gc, block, lds, s>>>(g, b);
+    LCODE_HIP(hipGetLastError());
+    return LCODE_OK;
+}
+
+}  // namespace lcode

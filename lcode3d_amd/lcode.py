@@ -1,0 +1,435 @@
+"""The reference's Python surface (lotov/lcode3d, lcode.py) over torch device
+tensors, with every computation in the gfx950 kernels of
+liblcode3d_hip.so.
+
+Same names, argument meaning and ownership as the reference: every operator
+allocates fresh outputs and never mutates its inputs, ``init`` writes
+``config.reflect_boundary`` (lcode.py:947-949), ``step`` returns a new
+``GPUArrays`` and leaves ``prev`` valid (lcode.py:926-931).  Arrays are
+float64 torch tensors on the current CUDA (HIP) device; host NumPy arrays
+are accepted wherever the reference accepted them and are copied over,
+like ``cp.asarray``.
+
+Differences from the reference, all on the error path: a particle whose
+3x3 stencil would leave the grid is clamped into it and counted (see
+``fault_count``) instead of reading or writing out of bounds.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+ELECTRON_CHARGE = -1   # lcode.py:46
+ELECTRON_MASS = 1      # lcode.py:47
+
+_F64 = torch.float64
+
+
+# ---------------------------------------------------------------------------
+# Device arrays (lcode.py:52-119)
+# ---------------------------------------------------------------------------
+
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("lcode3d_amd needs a HIP device (none visible)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _as_device(array, dtype=None):
+    """cp.asarray: host arrays are copied to the current device."""
+    if isinstance(array, torch.Tensor):
+        t = array
+        if t.device.type != "cuda":
+            t = t.to(_device())
+    else:
+        t = torch.as_tensor(np.asarray(array), device=_device())
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+class GPUArrays:
+    """Attribute bag of device arrays (lcode.py:52-69)."""
+
+    def __init__(self, **kwargs):
+        for name, array in kwargs.items():
+            setattr(self, name, _as_device(array))
+
+
+class GPUArraysView:
+    """Copying view of a GPUArrays (lcode.py:73-119): reading an attribute
+    copies it to a host NumPy array, assigning copies the value back."""
+
+    def __init__(self, gpu_arrays):
+        super().__setattr__("_arrs", gpu_arrays)
+
+    def __dir__(self):
+        return list(set(super().__dir__() + dir(self._arrs)))
+
+    def __getattr__(self, attrname):
+        return getattr(self._arrs, attrname).cpu().numpy()
+
+    def __setattr__(self, attrname, value):
+        dst = getattr(self._arrs, attrname)
+        dst.copy_(torch.as_tensor(np.asarray(value), dtype=dst.dtype))
+
+
+# ---------------------------------------------------------------------------
+# Workspaces (the reference memoizes its tables with cp.memoize)
+# ---------------------------------------------------------------------------
+
+_workspaces = {}
+
+
+def _workspace(cfg_struct):
+    key = (torch.cuda.current_device(), cfg_struct.grid_steps,
+           cfg_struct.grid_step_size, cfg_struct.subtraction_trick != 0,
+           cfg_struct.nc, cfg_struct.solver)
+    ws = _workspaces.get(key)
+    if ws is None:
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(lib.lcode_workspace_create(ctypes.byref(cfg_struct),
+                                              ctypes.byref(handle)),
+                   "lcode_workspace_create")
+        ws = _workspaces[key] = handle
+    return ws
+
+
+def _config_struct(config, nc=0, nf=0):
+    c = _lib.LcodeConfig()
+    c.grid_steps = int(config.grid_steps)
+    c.grid_step_size = float(config.grid_step_size)
+    c.xi_step_size = float(config.xi_step_size)
+    c.subtraction_trick = float(config.field_solver_subtraction_trick)
+    c.variant_A = 1 if config.field_solver_variant_A else 0
+    c.reflect_boundary = float(getattr(config, "reflect_boundary", math.nan))
+    c.plasma_coarseness = int(config.plasma_coarseness)
+    c.plasma_fineness = int(config.plasma_fineness)
+    c.nc = int(nc)
+    c.nf = int(nf)
+    c.solver = _lib.LCODE_SOLVER_FFT64
+    return c
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _check_grid(config, *arrays):
+    n = int(config.grid_steps)
+    out = []
+    for a in arrays:
+        t = _as_device(a, _F64)
+        if tuple(t.shape) != (n, n):
+            raise ValueError(f"grid array of shape {tuple(t.shape)}, "
+                             f"expected ({n}, {n})")
+        out.append(t)
+    return out
+
+
+def _check_particles(*arrays):
+    out = [_as_device(a, _F64) for a in arrays]
+    shape = out[0].shape
+    if len(shape) != 2 or shape[0] != shape[1]:
+        raise ValueError(f"coarse particle arrays must be (nc, nc), got {tuple(shape)}")
+    for t in out:
+        if t.shape != shape:
+            raise ValueError("coarse particle arrays differ in shape")
+    return out
+
+
+def _beam_arg(config, beam_ro):
+    """Array beam -> (tensor, 0.0); scalar beam -> (None, value)."""
+    if isinstance(beam_ro, (int, float, np.floating, np.integer)) or (
+            np.ndim(beam_ro) == 0 and not isinstance(beam_ro, torch.Tensor)):
+        return None, float(beam_ro)
+    if isinstance(beam_ro, torch.Tensor) and beam_ro.dim() == 0:
+        return None, float(beam_ro.item())
+    (t,) = _check_grid(config, beam_ro)
+    return t, 0.0
+
+
+def fault_count(config, nc):
+    """Stencil lookups clamped into the grid since the last call (0 means no
+    particle came within one cell of the grid edge).  Synchronises."""
+    lib = _lib.load()
+    ws = _workspace(_config_struct(config, nc))
+    n = ctypes.c_int64()
+    _lib.check(lib.lcode_workspace_fault_count(ws, _stream(), ctypes.byref(n)),
+               "lcode_workspace_fault_count")
+    return n.value
+
+
+# ---------------------------------------------------------------------------
+# Field solvers (lcode.py:160-364)
+# ---------------------------------------------------------------------------
+
+def calculate_Ez(config, jx, jy):
+    """lcode.py:160-184."""
+    lib = _lib.load()
+    jx, jy = _check_grid(config, jx, jy)
+    cs = _config_struct(config)
+    Ez = torch.empty_like(jx)
+    _lib.check(lib.lcode_calculate_Ez(_workspace(cs), ctypes.byref(cs),
+                                      _ptr(jx), _ptr(jy), _ptr(Ez), _stream()),
+               "lcode_calculate_Ez")
+    return Ez
+
+
+def calculate_Bz(config, jx, jy):
+    """lcode.py:333-364."""
+    lib = _lib.load()
+    jx, jy = _check_grid(config, jx, jy)
+    cs = _config_struct(config)
+    Bz = torch.empty_like(jx)
+    _lib.check(lib.lcode_calculate_Bz(_workspace(cs), ctypes.byref(cs),
+                                      _ptr(jx), _ptr(jy), _ptr(Bz), _stream()),
+               "lcode_calculate_Bz")
+    return Bz
+
+
+def calculate_Ex_Ey_Bx_By(config, Ex_avg, Ey_avg, Bx_avg, By_avg,
+                          beam_ro, ro, jx, jy, jz, jx_prev, jy_prev):
+    """lcode.py:246-290."""
+    lib = _lib.load()
+    grids = _check_grid(config, Ex_avg, Ey_avg, Bx_avg, By_avg, ro, jx, jy,
+                        jz, jx_prev, jy_prev)
+    beam, beam_scalar = _beam_arg(config, beam_ro)
+    cs = _config_struct(config)
+    outs = [torch.empty_like(grids[0]) for _ in range(4)]
+    out_pp, keep = _lib.ptr_array([o.data_ptr() for o in outs])
+    g = [_ptr(t) for t in grids]
+    _lib.check(lib.lcode_calculate_Ex_Ey_Bx_By(
+        _workspace(cs), ctypes.byref(cs), g[0], g[1], g[2], g[3],
+        _ptr(beam) if beam is not None else None, beam_scalar,
+        g[4], g[5], g[6], g[7], g[8], g[9], out_pp, _stream()),
+        "lcode_calculate_Ex_Ey_Bx_By")
+    return tuple(outs)
+
+
+# ---------------------------------------------------------------------------
+# Particles (lcode.py:369-391, :666-699, :796-823)
+# ---------------------------------------------------------------------------
+
+def move_estimate_wo_fields(config, m, x_init, y_init, prev_x_offt,
+                            prev_y_offt, px, py, pz):
+    """lcode.py:369-391."""
+    lib = _lib.load()
+    arrs = _check_particles(m, x_init, y_init, prev_x_offt, prev_y_offt,
+                            px, py, pz)
+    nc = arrs[0].shape[0]
+    cs = _config_struct(config, nc)
+    x_offt, y_offt = torch.empty_like(arrs[0]), torch.empty_like(arrs[0])
+    _lib.check(lib.lcode_move_estimate_wo_fields(
+        _workspace(cs), ctypes.byref(cs), *[_ptr(a) for a in arrs],
+        _ptr(x_offt), _ptr(y_offt), _stream()),
+        "lcode_move_estimate_wo_fields")
+    return x_offt, y_offt
+
+
+def move_smart(config, m, q, x_init, y_init, x_prev_offt, y_prev_offt,
+               estimated_x_offt, estimated_y_offt, px_prev, py_prev, pz_prev,
+               Ex_avg, Ey_avg, Ez_avg, Bx_avg, By_avg, Bz_avg):
+    """lcode.py:796-823."""
+    lib = _lib.load()
+    parts = _check_particles(m, q, x_init, y_init, x_prev_offt, y_prev_offt,
+                             estimated_x_offt, estimated_y_offt, px_prev,
+                             py_prev, pz_prev)
+    fields = _check_grid(config, Ex_avg, Ey_avg, Ez_avg, Bx_avg, By_avg,
+                         Bz_avg)
+    nc = parts[0].shape[0]
+    cs = _config_struct(config, nc)
+    outs = [torch.empty_like(parts[0]) for _ in range(5)]
+    f_pp, kf = _lib.ptr_array([f.data_ptr() for f in fields])
+    o_pp, ko = _lib.ptr_array([o.data_ptr() for o in outs])
+    _lib.check(lib.lcode_move_smart(_workspace(cs), ctypes.byref(cs),
+                                    *[_ptr(p) for p in parts], f_pp, o_pp,
+                                    _stream()), "lcode_move_smart")
+    return tuple(outs)
+
+
+def _virt_struct(virt_params):
+    vp = _lib.LcodeVirtParams()
+    arrs = {}
+    for name in ("influence_prev", "influence_next", "fine_grid"):
+        arrs[name] = _as_device(getattr(virt_params, name), _F64)
+    for name in ("indices_prev", "indices_next"):
+        arrs[name] = _as_device(getattr(virt_params, name), torch.int64)
+    nf = arrs["fine_grid"].numel()
+    for name, t in arrs.items():
+        if t.numel() != nf:
+            raise ValueError(f"virt_params.{name} has {t.numel()} entries, "
+                             f"expected {nf}")
+        setattr(vp, name, t.data_ptr())
+    return vp, nf, arrs
+
+
+def deposit(config, ro_initial, x_offt, y_offt, m, q, px, py, pz,
+            virt_params):
+    """lcode.py:666-689.  ro_initial may be the scalar 0 (lcode.py:697)."""
+    lib = _lib.load()
+    parts = _check_particles(x_offt, y_offt, m, q, px, py, pz)
+    vp, nf, keep = _virt_struct(virt_params)
+    nc = parts[0].shape[0]
+    if isinstance(ro_initial, torch.Tensor) or np.ndim(ro_initial) > 0:
+        (ro0,) = _check_grid(config, ro_initial)
+        ro0_ptr = _ptr(ro0)
+    else:
+        if float(ro_initial) != 0:
+            raise ValueError("a scalar ro_initial must be 0")
+        ro0_ptr = None
+    cs = _config_struct(config, nc, nf)
+    n = int(config.grid_steps)
+    outs = [torch.empty((n, n), dtype=_F64, device=parts[0].device)
+            for _ in range(4)]
+    o_pp, ko = _lib.ptr_array([o.data_ptr() for o in outs])
+    _lib.check(lib.lcode_deposit(_workspace(cs), ctypes.byref(cs), ro0_ptr,
+                                 *[_ptr(p) for p in parts], ctypes.byref(vp),
+                                 o_pp, _stream()), "lcode_deposit")
+    return tuple(outs)
+
+
+def initial_deposition(config, x_offt, y_offt, px, py, pz, m, q, virt_params):
+    """lcode.py:692-699."""
+    ro_electrons_initial, _, _, _ = deposit(config, 0, x_offt, y_offt, m, q,
+                                            px, py, pz, virt_params)
+    return -ro_electrons_initial
+
+
+# ---------------------------------------------------------------------------
+# The xi step (lcode.py:828-931)
+# ---------------------------------------------------------------------------
+
+def step(config, const, virt_params, prev, beam_ro):
+    """One xi step; returns the new state (lcode.py:828-931).  beam_ro is
+    an (N, N) array (host or device) or a scalar."""
+    lib = _lib.load()
+    beam, beam_scalar = _beam_arg(config, beam_ro)
+    vp, nf, keep_vp = _virt_struct(virt_params)
+    nc = prev.x_offt.shape[0]
+    cs = _config_struct(config, nc, nf)
+    c = _lib.LcodeConst(*(getattr(const, n).data_ptr()
+                          for n in ("m", "q", "x_init", "y_init",
+                                    "ro_initial")))
+    for name in ("m", "q", "x_init", "y_init"):
+        t = getattr(const, name)
+        if t.dtype != _F64 or not t.is_contiguous() or t.shape != (nc, nc):
+            raise ValueError(f"const.{name} must be contiguous float64 ({nc}, {nc})")
+    n = int(config.grid_steps)
+    for name in _lib.STATE_FIELDS:
+        t = getattr(prev, name)
+        shape = (nc, nc) if name in ("x_offt", "y_offt", "px", "py", "pz") else (n, n)
+        if t.dtype != _F64 or not t.is_contiguous() or tuple(t.shape) != shape:
+            raise ValueError(f"prev.{name} must be contiguous float64 {shape}")
+    if tuple(const.ro_initial.shape) != (n, n):
+        raise ValueError("const.ro_initial has the wrong shape")
+    p = _lib.LcodeState(*(getattr(prev, name).data_ptr()
+                          for name in _lib.STATE_FIELDS))
+    new = {name: torch.empty_like(getattr(prev, name))
+           for name in _lib.STATE_FIELDS}
+    o = _lib.LcodeState(*(new[name].data_ptr() for name in _lib.STATE_FIELDS))
+    _lib.check(lib.lcode_step(_workspace(cs), ctypes.byref(cs),
+                              ctypes.byref(c), ctypes.byref(vp),
+                              ctypes.byref(p),
+                              _ptr(beam) if beam is not None else None,
+                              beam_scalar, ctypes.byref(o), _stream()),
+               "lcode_step")
+    state = GPUArrays.__new__(GPUArrays)
+    for name, t in new.items():
+        setattr(state, name, t)
+    return state
+
+
+# ---------------------------------------------------------------------------
+# Plasma set-up and init (lcode.py:454-573, :936-975); runs once, on the host
+# like the reference's, then uploads
+# ---------------------------------------------------------------------------
+
+def make_coarse_plasma_grid(steps, step_size, coarseness=3):
+    """lcode.py:454-465."""
+    assert coarseness == int(coarseness)
+    half = np.arange(steps // (coarseness * 2)) * (step_size * coarseness)
+    grid = np.concatenate([-half[:0:-1], half])
+    assert np.array_equal(grid, -grid[::-1])
+    return grid
+
+
+def make_fine_plasma_grid(steps, step_size, fineness=2):
+    """lcode.py:468-487."""
+    assert fineness == int(fineness)
+    step = step_size / fineness
+    if fineness % 2:
+        half = np.arange(steps // 2 * fineness) * step
+        grid = np.concatenate([-half[:0:-1], half])
+    else:
+        half = (.5 + np.arange(steps // 2 * fineness)) * step
+        grid = np.concatenate([-half[::-1], half])
+    assert np.array_equal(grid, -grid[::-1])
+    return grid
+
+
+def make_plasma(steps, cell_size, coarseness=3, fineness=2):
+    """lcode.py:490-573: coarse particle arrays (device) and the
+    coarse->fine interpolation parameters (virt_params)."""
+    coarse = make_coarse_plasma_grid(steps, cell_size, coarseness)
+    fine = make_fine_plasma_grid(steps, cell_size, fineness)
+    nc = len(coarse)
+    nxt = np.clip(np.searchsorted(coarse, fine), 0, nc - 1)
+    prv = np.clip(np.searchsorted(coarse, fine) - 1, 0, nc - 1)
+    step = cell_size * coarseness
+    w_prev = (coarse[nxt] - fine) / step
+    w_next = (fine - coarse[prv]) / step
+    # clipped neighbours at both ends use the one existing particle
+    w_prev[nxt == 0], w_next[nxt == 0] = 0, 1
+    w_next[prv == nc - 1], w_prev[prv == nc - 1] = 0, 1
+    virt_params = GPUArrays(influence_prev=w_prev, influence_next=w_next,
+                            indices_prev=prv.astype(np.int64),
+                            indices_next=nxt.astype(np.int64),
+                            fine_grid=fine)
+    dev = _device()
+    shape = (nc, nc)
+    x_init = torch.as_tensor(coarse, device=dev)[:, None].expand(shape).contiguous()
+    y_init = torch.as_tensor(coarse, device=dev)[None, :].expand(shape).contiguous()
+    z = [torch.zeros(shape, dtype=_F64, device=dev) for _ in range(5)]
+    m = torch.full(shape, float(ELECTRON_MASS * coarseness**2), dtype=_F64, device=dev)
+    q = torch.full(shape, float(ELECTRON_CHARGE * coarseness**2), dtype=_F64, device=dev)
+    return (x_init, y_init, z[0], z[1], z[2], z[3], z[4], m, q, virt_params)
+
+
+def init(config):
+    """lcode.py:936-975."""
+    assert config.grid_steps % 2 == 1
+    assert config.reflect_padding_steps > config.plasma_coarseness + 1
+    config.reflect_boundary = config.grid_step_size * (
+        config.grid_steps / 2 - config.reflect_padding_steps)
+    n = config.grid_steps
+    grid = (np.arange(n) - n // 2) * config.grid_step_size
+    xs, ys = grid[:, None], grid[None, :]
+    (x_init, y_init, x_offt, y_offt, px, py, pz, m, q,
+     virt_params) = make_plasma(n - config.plasma_padding_steps * 2,
+                                config.grid_step_size,
+                                coarseness=config.plasma_coarseness,
+                                fineness=config.plasma_fineness)
+    ro_initial = initial_deposition(config, x_offt, y_offt, px, py, pz, m, q,
+                                    virt_params)
+    const = GPUArrays(m=m, q=q, x_init=x_init, y_init=y_init,
+                      ro_initial=ro_initial)
+    dev = _device()
+
+    def zeros():
+        return torch.zeros((n, n), dtype=_F64, device=dev)
+
+    state = GPUArrays(x_offt=x_offt, y_offt=y_offt, px=px, py=py, pz=pz,
+                      Ex=zeros(), Ey=zeros(), Ez=zeros(),
+                      Bx=zeros(), By=zeros(), Bz=zeros(),
+                      ro=zeros(), jx=zeros(), jy=zeros(), jz=zeros())
+    return xs, ys, const, virt_params, state
