@@ -404,6 +404,10 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
         double va = 0., vb = 0.;
         const int k = j - fp.colA0;
         if (k >= 0 && k < fp.lenA) get_ext(z, fp.xfA, k, va, vb);
+        // a row without data shares its FFT with a partner row; keep it an
+        // exact zero instead of the partner's rounding noise
+        if (!ha) va = 0.;
+        if (!hb) vb = 0.;
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const int i = side ? rb : ra;
