@@ -41,6 +41,8 @@ struct lcode_workspace {
     double* dep[4] = {};           // ro jx jy jz of passes 1 and 2
     double* favg[6] = {};          // field averages of the current pass
     int* fault = nullptr;
+    int* spill = nullptr;          // nf * nf spilled fine particle indices
+    int* spill_ctr = nullptr;      // [0] count, [1] blocks done
     std::vector<void*> allocs;
 };
 
@@ -261,8 +263,13 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     for (int k = 0; k < 6; ++k)
         if ((rc = dev_alloc(ws, (void**)&ws->favg[k], n2 * sizeof(double)))) return fail(rc);
     if ((rc = dev_alloc(ws, (void**)&ws->fault, sizeof(int)))) return fail(rc);
-    if (hipMemset(ws->fault, 0, sizeof(int)) != hipSuccess) {
-        set_error("hipMemset of the fault counter failed");
+    const size_t nfp = (size_t)cfg->nf * cfg->nf;
+    if ((rc = dev_alloc(ws, (void**)&ws->spill, (nfp ? nfp : 1) * sizeof(int)))) return fail(rc);
+    if ((rc = dev_alloc(ws, (void**)&ws->spill_ctr, 2 * sizeof(int)))) return fail(rc);
+    if (hipMemset(ws->fault, 0, sizeof(int)) != hipSuccess ||
+        hipMemset(ws->spill_ctr, 0, 2 * sizeof(int)) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+        set_error("hipMemset of the workspace counters failed");
         return fail(LCODE_E_HIP);
     }
     *out = ws;
@@ -380,6 +387,10 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.ro_initial = ro_initial;
     a.ro = out[0]; a.jx = out[1]; a.jy = out[2]; a.jz = out[3];
     a.fault = ws->fault;
+    a.spill = ws->spill;
+    a.spill_count = ws->spill_ctr;
+    a.spill_done = ws->spill_ctr + 1;
+    LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
     return launch_deposit(a, stream);
 }
 

@@ -49,10 +49,15 @@ struct Tsc {
     double w[9];   // wMP w0P wPP wM0 w00 wP0 wMM w0M wPM
 };
 
+// Cell index of the coordinate x: floor(x / h + .5) + N / 2.
+__device__ __forceinline__ int cell_of(double x, int N, double h) {
+    return (int)floor(x / h + .5) + N / 2;
+}
+
 // Cell index and quadratic-B-spline weights of the point (x, y).  A centre
-// cell whose 3x3 stencil would leave the grid is clamped into it and
-// counted in *fault (the reference has no check: there it is an
-// out-of-bounds access).
+// cell whose 3x3 stencil would leave the grid is clamped into it and, when
+// fault is not NULL, counted there (the reference has no check: there it
+// is an out-of-bounds access).
 __device__ __forceinline__ void tsc_weights(double x, double y, int N,
                                             double h, Tsc& t, int* fault) {
     const double x_h = x / h + .5, y_h = y / h + .5;
@@ -60,7 +65,7 @@ __device__ __forceinline__ void tsc_weights(double x, double y, int N,
     int i = (int)fx + N / 2, j = (int)fy + N / 2;
     const double x_loc = x_h - fx - .5, y_loc = y_h - fy - .5;
     if (!(i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2)) {
-        atomicAdd(fault, 1);
+        if (fault) atomicAdd(fault, 1);
         i = i < 1 ? 1 : (i > N - 2 ? N - 2 : i);
         j = j < 1 ? 1 : (j > N - 2 ? N - 2 : j);
     }
@@ -118,6 +123,11 @@ struct DepositArgs {
     const double* ro_initial;      // may be NULL
     double *ro, *jx, *jy, *jz;
     int* fault;
+    // particles displaced by more than the tile halo: appended to spill[]
+    // by their home tile, deposited with atomics by a second kernel
+    int* spill;                    // nf * nf capacity
+    int* spill_count;              // zero between calls
+    int* spill_done;               // zero between calls
 };
 
 int launch_estimate(const EstimateArgs& a, hipStream_t s);

@@ -109,27 +109,14 @@ push_kernel(PushArgs a) {
     a.out_pz[k] = pz;
 }
 
-__global__ void __launch_bounds__(kBlock)
-zero4_kernel(double* a, double* b, double* c, double* d, int n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    a[k] = 0; b[k] = 0; c[k] = 0; d[k] = 0;
-}
+// Coarse->fine interpolation of fine particle (fi, fj) (lcode.py:591-621)
+// and its deposition values (lcode.py:651-655).
+struct FineParticle {
+    double x, y, v[4];   // position; dro, djx, djy, djz
+};
 
-__global__ void __launch_bounds__(kBlock)
-add_kernel(double* __restrict__ a, const double* __restrict__ b, int n) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) a[k] += b[k];
-}
-
-// deposit_kernel (lcode.py:626-663): one fine particle per thread,
-// bilinear coarse->fine interpolation (lcode.py:591-621), then 4 x 9 FP64
-// atomic adds into the grids.
-__global__ void __launch_bounds__(kBlock)
-deposit_atomic_kernel(DepositArgs a) {
-    const int fk = blockIdx.x * blockDim.x + threadIdx.x;
-    if (fk >= a.nf * a.nf) return;
-    const int fi = fk / a.nf, fj = fk % a.nf;
+__device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
+                                              int fj, FineParticle& p) {
     const double ipi = a.ip[fi], ini = a.in[fi];
     const double ipj = a.ip[fj], inj = a.in[fj];
     const double A = ipi * ipj, B = ipi * inj, C = ini * ipj, D = ini * inj;
@@ -141,8 +128,8 @@ deposit_atomic_kernel(DepositArgs a) {
     const int pp = pi * nc + pj, pn = pi * nc + nj;
     const int np = ni * nc + pj, nn = ni * nc + nj;
 #define LCODE_MIX(arr) (A * arr[pp] + B * arr[pn] + C * arr[np] + D * arr[nn])
-    const double x = a.fg[fi] + LCODE_MIX(a.x_offt);
-    const double y = a.fg[fj] + LCODE_MIX(a.y_offt);
+    p.x = a.fg[fi] + LCODE_MIX(a.x_offt);
+    p.y = a.fg[fj] + LCODE_MIX(a.y_offt);
     const double s = a.smallness;
     const double m = s * LCODE_MIX(a.m);
     const double q = s * LCODE_MIX(a.q);
@@ -152,21 +139,126 @@ deposit_atomic_kernel(DepositArgs a) {
 #undef LCODE_MIX
     const double gamma_m = sqrt(m * m + px * px + py * py + pz * pz);
     const double dro = q / (1 - pz / gamma_m);
-    const double djx = px * (dro / gamma_m);
-    const double djy = py * (dro / gamma_m);
-    const double djz = pz * (dro / gamma_m);
-    Tsc t;
-    tsc_weights(x, y, a.N, a.h, t, a.fault);
+    p.v[0] = dro;
+    p.v[1] = px * (dro / gamma_m);
+    p.v[2] = py * (dro / gamma_m);
+    p.v[3] = pz * (dro / gamma_m);
+}
+
+// Tiles of the owner-computes deposition: a block owns kTile x kTile cells
+// and deposits, in LDS, every fine particle whose cell moved at most kHalo
+// cells from its initial cell and whose stencil touches the tile.
+constexpr int kTile = 32;
+constexpr int kHalo = 6;
+constexpr int kDepBlock = 256;
+
+// First fine index whose initial cell is >= c (fg is increasing).
+__device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
+                                      double h) {
+    int lo = 0, hi = nf;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cell_of(fg[mid], N, h) < c) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// deposit_kernel (lcode.py:626-663) as owner-computes tiles: each block
+// accumulates its kTile^2 cells of ro, jx, jy, jz in LDS (FP64 LDS atomics),
+// then stores them once, adding the ion background last (lcode.py:687).
+// Every cell of the grid is written by exactly one block, so the outputs
+// need no zeroing.  A particle displaced by more than kHalo cells is left
+// to deposit_spill_kernel, appended to the spill list by its home tile.
+__global__ void __launch_bounds__(kDepBlock)
+deposit_tile_kernel(DepositArgs a) {
+    __shared__ double acc[4][kTile][kTile];
+    __shared__ int range[4];
     const int N = a.N;
+    const int ci0 = blockIdx.y * kTile, cj0 = blockIdx.x * kTile;
+    for (int k = threadIdx.x; k < 4 * kTile * kTile; k += blockDim.x)
+        (&acc[0][0][0])[k] = 0.;
+    if (threadIdx.x == 0) {
+        range[0] = first_fine_at_or_after(a.fg, a.nf, ci0 - 1 - kHalo, N, a.h);
+        range[1] = first_fine_at_or_after(a.fg, a.nf, ci0 + kTile + 1 + kHalo, N, a.h);
+        range[2] = first_fine_at_or_after(a.fg, a.nf, cj0 - 1 - kHalo, N, a.h);
+        range[3] = first_fine_at_or_after(a.fg, a.nf, cj0 + kTile + 1 + kHalo, N, a.h);
+    }
+    __syncthreads();
+    const int fi_lo = range[0], ni = range[1] - range[0];
+    const int fj_lo = range[2], nj = range[3] - range[2];
+    const int total = ni * nj;
+    for (int p = threadIdx.x; p < total; p += blockDim.x) {
+        const int fi = fi_lo + p / nj, fj = fj_lo + p % nj;
+        const int ci = cell_of(a.fg[fi], N, a.h), cj = cell_of(a.fg[fj], N, a.h);
+        const bool home = ci >= ci0 && ci < ci0 + kTile &&
+                          cj >= cj0 && cj < cj0 + kTile;
+        FineParticle fp;
+        fine_particle(a, fi, fj, fp);
+        Tsc t;
+        tsc_weights(fp.x, fp.y, N, a.h, t, home ? a.fault : nullptr);
+        if (abs(t.i - ci) > kHalo || abs(t.j - cj) > kHalo) {
+            if (home) a.spill[atomicAdd(a.spill_count, 1)] = fi * a.nf + fj;
+            continue;
+        }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const size_t idx = (size_t)(t.i + stencil_di(k)) * N + (t.j + stencil_dj(k));
-        unsafeAtomicAdd(a.ro + idx, dro * t.w[k]);
-        unsafeAtomicAdd(a.jx + idx, djx * t.w[k]);
-        unsafeAtomicAdd(a.jy + idx, djy * t.w[k]);
-        unsafeAtomicAdd(a.jz + idx, djz * t.w[k]);
+        for (int k = 0; k < 9; ++k) {
+            const int si = t.i + stencil_di(k) - ci0;
+            const int sj = t.j + stencil_dj(k) - cj0;
+            if (si >= 0 && si < kTile && sj >= 0 && sj < kTile) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) atomicAdd(&acc[c][si][sj], fp.v[c] * t.w[k]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
+        const int r = k / kTile, c = k % kTile;
+        const int i = ci0 + r, j = cj0 + c;
+        if (i >= N || j >= N) continue;
+        const size_t idx = (size_t)i * N + j;
+        double ro = acc[0][r][c];
+        if (a.ro_initial) ro += a.ro_initial[idx];
+        a.ro[idx] = ro;
+        a.jx[idx] = acc[1][r][c];
+        a.jy[idx] = acc[2][r][c];
+        a.jz[idx] = acc[3][r][c];
     }
 }
+
+// Atomic deposition of the spilled particles; the last block to finish
+// resets the list for the next call (every block has read the count by
+// then).
+__global__ void __launch_bounds__(kBlock)
+deposit_spill_kernel(DepositArgs a) {
+    const int count = atomicAdd(a.spill_count, 0);
+    const int N = a.N;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count;
+         p += gridDim.x * blockDim.x) {
+        const int fk = a.spill[p];
+        FineParticle fp;
+        fine_particle(a, fk / a.nf, fk % a.nf, fp);
+        Tsc t;
+        tsc_weights(fp.x, fp.y, N, a.h, t, nullptr);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const size_t idx = (size_t)(t.i + stencil_di(k)) * N + (t.j + stencil_dj(k));
+            unsafeAtomicAdd(a.ro + idx, fp.v[0] * t.w[k]);
+            unsafeAtomicAdd(a.jx + idx, fp.v[1] * t.w[k]);
+            unsafeAtomicAdd(a.jy + idx, fp.v[2] * t.w[k]);
+            unsafeAtomicAdd(a.jz + idx, fp.v[3] * t.w[k]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(a.spill_done, 1) == (int)gridDim.x - 1) {
+            atomicExch(a.spill_count, 0);
+            atomicExch(a.spill_done, 0);
+        }
+    }
+}
+
+constexpr int kSpillBlocks = 64;
 
 inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -187,20 +279,11 @@ int launch_push(const PushArgs& a, hipStream_t s) {
 }
 
 int launch_deposit(const DepositArgs& a, hipStream_t s) {
-    const int n2 = a.N * a.N;
-    zero4_kernel<<<blocks_for(n2), kBlock, 0, s>>>(a.ro, a.jx, a.jy, a.jz, n2);
+    const dim3 grid((a.N + kTile - 1) / kTile, (a.N + kTile - 1) / kTile);
+    deposit_tile_kernel<<<grid, kDepBlock, 0, s>>>(a);
     LCODE_HIP(hipGetLastError());
-    const long nfp = (long)a.nf * a.nf;
-    if (nfp > 0) {
-        deposit_atomic_kernel<<<blocks_for(nfp), kBlock, 0, s>>>(a);
-        LCODE_HIP(hipGetLastError());
-    }
-    // The ion background is added last "to preserve more float precision"
-    // (lcode.py:686-687).
-    if (a.ro_initial) {
-        add_kernel<<<blocks_for(n2), kBlock, 0, s>>>(a.ro, a.ro_initial, n2);
-        LCODE_HIP(hipGetLastError());
-    }
+    deposit_spill_kernel<<<kSpillBlocks, kBlock, 0, s>>>(a);
+    LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
 

@@ -88,7 +88,7 @@ _workspaces = {}
 def _workspace(cfg_struct):
     key = (torch.cuda.current_device(), cfg_struct.grid_steps,
            cfg_struct.grid_step_size, cfg_struct.subtraction_trick != 0,
-           cfg_struct.nc, cfg_struct.solver)
+           cfg_struct.nc, cfg_struct.nf, cfg_struct.solver)
     ws = _workspaces.get(key)
     if ws is None:
         lib = _lib.load()
