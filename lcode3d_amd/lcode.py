@@ -158,15 +158,23 @@ def _beam_arg(config, beam_ro):
     return t, 0.0
 
 
-def fault_count(config, nc):
-    """Stencil lookups clamped into the grid since the last call (0 means no
-    particle came within one cell of the grid edge).  Synchronises."""
+def fault_count(config=None, nc=None):
+    """Stencil lookups clamped into the grid since the last call, summed over
+    the workspaces of the current device (0 means no particle came within
+    one cell of the grid edge).  Synchronises.  The arguments are accepted
+    for call-site readability and ignored."""
     lib = _lib.load()
-    ws = _workspace(_config_struct(config, nc))
-    n = ctypes.c_int64()
-    _lib.check(lib.lcode_workspace_fault_count(ws, _stream(), ctypes.byref(n)),
-               "lcode_workspace_fault_count")
-    return n.value
+    dev = torch.cuda.current_device()
+    total = 0
+    for key, ws in _workspaces.items():
+        if key[0] != dev:
+            continue
+        n = ctypes.c_int64()
+        _lib.check(lib.lcode_workspace_fault_count(ws, _stream(),
+                                                   ctypes.byref(n)),
+                   "lcode_workspace_fault_count")
+        total += n.value
+    return total
 
 
 # ---------------------------------------------------------------------------
