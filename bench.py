@@ -111,6 +111,8 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     assert torch.isfinite(state.Ez).all().item()
+    disp = max(state.x_offt.abs().max().item(), state.y_offt.abs().max().item())
+    log(f"max coarse displacement {disp / cfg.grid_step_size:.2f} cells")
     faults = lc.fault_count(cfg, nc)
     if faults:
         log(f"WARNING: {faults} clamped stencil lookups")
