@@ -44,6 +44,32 @@ def algorithmic_bytes_per_step(nc, n):
     return 8 * (79 * nc * nc + 91 * n * n)
 
 
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+
+
+def kernel_bytes(name, nc, n):
+    """Algorithmic HBM bytes of one launch of each kernel at this workload:
+    every array the kernel must read is read once, every output written
+    once (FP64, 8 B per value; c2 = nc^2 coarse, n2 = N^2 grid values).
+
+    push           11 coarse inputs + 5 outputs, 6 gathered fields
+    deposit_tile   7 coarse inputs, ro_initial, 4 grids out
+    solve_pass_a   13 RHS source grids in, 6 transposed spectra out
+    solve_pass_b   6 spectra + 6 eigenvalue tables in, 6 spectra out
+    solve_pass_c   6 spectra + 6 prev fields in, 6 averages out, plus the
+                   6 fields on the corrector pass (mean: 9 grids out)
+    """
+    c2, n2 = nc * nc, n * n
+    table = {
+        "push": 16 * c2 + 6 * n2,
+        "deposit_tile": 7 * c2 + 5 * n2,
+        "solve_pass_a": 19 * n2,
+        "solve_pass_b": 18 * n2,
+        "solve_pass_c": 21 * n2,
+    }
+    return 8 * table[name] if name in table else None
+
+
 def cpu_baseline(steps):
     """The oracle (NumPy/SciPy restatement of the reference step) on the
     host cores, on a bounded sample of the same workload."""
@@ -77,6 +103,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="steps timed per kernel with HIP events")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run a device copy of this size (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): "
@@ -122,6 +152,40 @@ def main():
     log(f"{steps_per_s:.1f} xi-steps/s, {ms:.3f} ms/step, "
         f"step-level {step_bytes / (dt / args.steps) / 1e9:.1f} GB/s algorithmic")
 
+    # per-kernel times over a separate region, HIP events on the step stream
+    roofline, kernels = None, {}
+    if args.profile_steps:
+        lc.profile_begin(64 * args.profile_steps)
+        for k in range(args.profile_steps):
+            state = lc.step(cfg, const, vp, state, beams[k % total])
+        prof = lc.profile_end()
+        for name, (calls, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+            avg_us = ms / calls * 1e3
+            b = kernel_bytes(name, nc, n)
+            kernels[name] = {"launches_per_step": calls / args.profile_steps,
+                             "avg_us": avg_us,
+                             "share": ms / sum(v[1] for v in prof.values()),
+                             "GBps": (b / (avg_us * 1e-6) / 1e9) if b else None}
+            log(f"  {name:14s} {calls / args.profile_steps:4.1f}/step "
+                f"{avg_us:8.2f} us  {kernels[name]['share'] * 100:5.1f}%"
+                + (f"  {kernels[name]['GBps']:7.0f} GB/s" if b else ""))
+        dom = max((k for k in kernels if kernels[k]["GBps"]),
+                  key=lambda k: kernels[k]["share"])
+        ach = kernels[dom]["GBps"]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": ach,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                    "bytes_per_launch": kernel_bytes(dom, nc, n),
+                    "avg_us": kernels[dom]["avg_us"]}
+    if args.copy_probe_mib:
+        src = torch.ones(args.copy_probe_mib << 17, dtype=torch.float64,
+                         device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x {args.copy_probe_mib} MiB device copies")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_steps)
@@ -145,7 +209,8 @@ def main():
                                "fineness 2, plain FP64",
                    "grid_steps": n, "coarse_particles": nc * nc,
                    "fine_particles": int(vp.fine_grid.numel()) ** 2},
-        "roofline": None,
+        "roofline": roofline,
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

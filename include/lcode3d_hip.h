@@ -107,6 +107,22 @@ int lcode_workspace_destroy(lcode_workspace *ws);
 int lcode_workspace_fault_count(lcode_workspace *ws, hipStream_t stream,
                                 int64_t *count);
 
+/* Per-kernel timing (instrumentation; no counterpart in the reference).
+ * Between begin and end every kernel the library launches is bracketed by
+ * HIP events on its stream; end waits for the stream and returns, per
+ * kernel name, the launch count and the summed event time.  max_launches
+ * bounds the number of timed launches (later ones run untimed). */
+typedef struct {
+    char name[32];
+    int64_t calls;
+    double total_ms;
+} lcode_kernel_time;
+
+int lcode_profile_begin(lcode_workspace *ws, int32_t max_launches);
+int lcode_profile_end(lcode_workspace *ws, hipStream_t stream,
+                      lcode_kernel_time *out, int32_t max_out,
+                      int32_t *n_out);
+
 /* lcode.py:369-391.  n = nc*nc particles. */
 int lcode_move_estimate_wo_fields(lcode_workspace *ws, const lcode_config *cfg,
                                   const double *m, const double *x_init,

@@ -59,6 +59,11 @@ class LcodeState(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in STATE_FIELDS]
 
 
+class LcodeKernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("calls", ctypes.c_int64),
+                ("total_ms", ctypes.c_double)]
+
+
 #: every symbol the header declares, with (restype, argtypes)
 _vp = ctypes.c_void_p
 _SIGNATURES = {
@@ -68,6 +73,10 @@ _SIGNATURES = {
                                [ctypes.POINTER(LcodeConfig),
                                 ctypes.POINTER(ctypes.c_void_p)]),
     "lcode_workspace_destroy": (ctypes.c_int, [_vp]),
+    "lcode_profile_begin": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "lcode_profile_end": (ctypes.c_int,
+                          [_vp, _vp, ctypes.POINTER(LcodeKernelTime),
+                           ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "lcode_workspace_fault_count": (ctypes.c_int,
                                     [_vp, _vp, ctypes.POINTER(ctypes.c_int64)]),
     "lcode_move_estimate_wo_fields": (ctypes.c_int,

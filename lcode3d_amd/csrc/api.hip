@@ -28,7 +28,7 @@ struct lcode_workspace {
     int trick_on = 0;
     int nc = 0, nf = 0;
     SolverGeom geom{};
-    double2* W = nullptr;          // twiddles
+    double* costab = nullptr;      // quarter-wave cosine table
     double* tab_mixed = nullptr;   // [M][N]   (DST index, DCT index)
     double* tab_mixed_t = nullptr; // [N][M]
     double* tab_dir = nullptr;     // [M][M]
@@ -43,10 +43,15 @@ struct lcode_workspace {
     int* fault = nullptr;
     int* spill = nullptr;          // nf * nf spilled fine particle indices
     int* spill_ctr = nullptr;      // [0] count, [1] blocks done
+    Profiler prof;
     std::vector<void*> allocs;
 };
 
 namespace {
+
+// Cells a fine particle may move from its initial cell and still be
+// deposited through the LDS tiles (beyond: the atomic spill path).
+constexpr int kDepositHalo = 3;
 
 int dev_alloc(lcode_workspace* ws, void** p, size_t bytes) {
     LCODE_HIP(hipMalloc(p, bytes));
@@ -193,7 +198,8 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     }
     g.Tp = ((L + 3) / 4 + 63) / 64 * 64;
     int P = 1024 / g.Tp;
-    const int by_lds = 65536 / (int)(L * sizeof(double2));
+    const int by_lds = (65536 - (L / 4 + 1) * (int)sizeof(double)) /
+                       (int)(L * sizeof(double2));
     if (by_lds < P) P = by_lds;
     if (P > 4) P = 4;
     if (P < 1) {
@@ -206,13 +212,13 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         return fail(LCODE_E_UNSUPPORTED);
     }
 
-    std::vector<double2> W(L);
-    for (int t = 0; t < L; ++t) {
-        const long double a = 2.0L * 3.14159265358979323846264338327950288L * t / L;
-        W[t] = make_double2((double)std::cos(a), (double)-std::sin(a));
+    std::vector<double> costab(L / 4 + 1);
+    for (int r = 0; r <= L / 4; ++r) {
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * r / L;
+        costab[r] = (double)std::cos(a);
     }
-    if ((rc = upload(ws, &ws->W, W))) return fail(rc);
-    g.W = ws->W;
+    if ((rc = upload(ws, &ws->costab, costab))) return fail(rc);
+    g.costab = ws->costab;
 
     // Eigenvalue tables, normalisation 1 / (2 (N - 1))^2 folded in
     // (lcode.py:144-157, :215-231, :316-330).
@@ -276,14 +282,71 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     return LCODE_OK;
 }
 
+static void profiler_free(Profiler& p) {
+    for (int k = 0; k < 2 * p.capacity; ++k) (void)hipEventDestroy(p.ev[k]);
+    delete[] p.ev;
+    delete[] p.names;
+    p = Profiler();
+}
+
 int lcode_workspace_destroy(lcode_workspace* ws) {
     if (!ws) return LCODE_OK;
+    profiler_free(ws->prof);
     int rc = LCODE_OK;
     for (void* p : ws->allocs)
         if (hipFree(p) != hipSuccess) rc = LCODE_E_HIP;
     delete ws;
     if (rc) set_error("hipFree failed in lcode_workspace_destroy");
     return rc;
+}
+
+int lcode_profile_begin(lcode_workspace* ws, int32_t max_launches) {
+    LCODE_REQUIRE(ws && max_launches > 0, "bad profile_begin arguments");
+    profiler_free(ws->prof);
+    Profiler& p = ws->prof;
+    p.ev = new hipEvent_t[2 * (size_t)max_launches];
+    p.names = new const char*[max_launches];
+    for (int k = 0; k < 2 * max_launches; ++k) {
+        if (hipEventCreate(&p.ev[k]) != hipSuccess) {
+            p.capacity = k / 2;
+            profiler_free(p);
+            set_error("hipEventCreate failed");
+            return LCODE_E_HIP;
+        }
+        p.capacity = (k + 1) / 2;
+    }
+    p.capacity = max_launches;
+    p.used = 0;
+    p.on = true;
+    return LCODE_OK;
+}
+
+int lcode_profile_end(lcode_workspace* ws, hipStream_t stream,
+                      lcode_kernel_time* out, int32_t max_out,
+                      int32_t* n_out) {
+    LCODE_REQUIRE(ws && out && n_out, "bad profile_end arguments");
+    Profiler& p = ws->prof;
+    p.on = false;
+    LCODE_HIP(hipStreamSynchronize(stream));
+    int n = 0;
+    for (int k = 0; k < p.used; ++k) {
+        float ms = 0;
+        LCODE_HIP(hipEventElapsedTime(&ms, p.ev[2 * k], p.ev[2 * k + 1]));
+        int slot = -1;
+        for (int m = 0; m < n; ++m)
+            if (std::strcmp(out[m].name, p.names[k]) == 0) slot = m;
+        if (slot < 0) {
+            if (n == max_out) continue;
+            slot = n++;
+            std::memset(&out[slot], 0, sizeof(out[slot]));
+            std::strncpy(out[slot].name, p.names[k], sizeof(out[slot].name) - 1);
+        }
+        out[slot].calls += 1;
+        out[slot].total_ms += ms;
+    }
+    *n_out = n;
+    profiler_free(p);
+    return LCODE_OK;
 }
 
 int lcode_workspace_fault_count(lcode_workspace* ws, hipStream_t stream,
@@ -319,7 +382,7 @@ int lcode_move_estimate_wo_fields(lcode_workspace* ws, const lcode_config* cfg,
     LCODE_REQUIRE(a.n == 0 || (m && x_init && y_init && prev_x_offt && prev_y_offt &&
                                px && py && pz && x_offt_out && y_offt_out),
                   "NULL array");
-    return launch_estimate(a, stream);
+    return launch_estimate(a, stream, &ws->prof);
 }
 
 static int do_push(lcode_workspace* ws, const lcode_config* cfg,
@@ -343,7 +406,7 @@ static int do_push(lcode_workspace* ws, const lcode_config* cfg,
     a.out_x = out[0]; a.out_y = out[1];
     a.out_px = out[2]; a.out_py = out[3]; a.out_pz = out[4];
     a.fault = ws->fault;
-    return launch_push(a, stream);
+    return launch_push(a, stream, &ws->prof);
 }
 
 int lcode_move_smart(lcode_workspace* ws, const lcode_config* cfg,
@@ -390,8 +453,9 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.spill = ws->spill;
     a.spill_count = ws->spill_ctr;
     a.spill_done = ws->spill_ctr + 1;
+    a.halo = kDepositHalo;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
-    return launch_deposit(a, stream);
+    return launch_deposit(a, stream, &ws->prof);
 }
 
 int lcode_deposit(lcode_workspace* ws, const lcode_config* cfg,
@@ -437,7 +501,7 @@ int lcode_calculate_Ex_Ey_Bx_By(lcode_workspace* ws, const lcode_config* cfg,
     b.in.beam = beam_ro;
     b.in.beam_scalar = beam_scalar;
     b.variantA = 0;
-    return launch_solve(ws->geom, b, stream);
+    return launch_solve(ws->geom, b, stream, &ws->prof);
 }
 
 static int solve_single(lcode_workspace* ws, const lcode_config* cfg,
@@ -453,7 +517,7 @@ static int solve_single(lcode_workspace* ws, const lcode_config* cfg,
     b.in = rhs_base(cfg);
     b.in.jx = jx;
     b.in.jy = jy;
-    return launch_solve(ws->geom, b, stream);
+    return launch_solve(ws->geom, b, stream, &ws->prof);
 }
 
 int lcode_calculate_Ez(lcode_workspace* ws, const lcode_config* cfg,
@@ -497,7 +561,7 @@ static int field_pass(lcode_workspace* ws, const lcode_config* cfg,
     b.in.beam = beam_ro;
     b.in.beam_scalar = beam_scalar;
     b.variantA = cfg->variant_A ? 1 : 0;
-    return launch_solve(ws->geom, b, stream);
+    return launch_solve(ws->geom, b, stream, &ws->prof);
 }
 
 int lcode_step(lcode_workspace* ws, const lcode_config* cfg,

@@ -36,6 +36,36 @@ void set_error(const std::string& msg);
     } while (0)
 
 // ---------------------------------------------------------------------------
+// Kernel timing with HIP events (enabled by lcode_profile_begin)
+// ---------------------------------------------------------------------------
+
+struct Profiler {
+    bool on = false;
+    int capacity = 0;                 // event pairs
+    int used = 0;
+    hipEvent_t* ev = nullptr;         // 2 * capacity events
+    const char** names = nullptr;     // name of pair k
+};
+
+// Brackets one kernel launch; a no-op when timing is off or full.
+struct ProfScope {
+    Profiler* p;
+    hipStream_t s;
+    int k = -1;
+    ProfScope(Profiler* prof, const char* name, hipStream_t stream)
+        : p(prof), s(stream) {
+        if (p && p->on && p->used < p->capacity) {
+            k = p->used++;
+            p->names[k] = name;
+            (void)hipEventRecord(p->ev[2 * k], s);
+        }
+    }
+    ~ProfScope() {
+        if (k >= 0) (void)hipEventRecord(p->ev[2 * k + 1], s);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // TSC-2D shape (lcode.py:396-418)
 // ---------------------------------------------------------------------------
 
@@ -128,11 +158,12 @@ struct DepositArgs {
     int* spill;                    // nf * nf capacity
     int* spill_count;              // zero between calls
     int* spill_done;               // zero between calls
+    int halo;                      // tile halo in cells
 };
 
-int launch_estimate(const EstimateArgs& a, hipStream_t s);
-int launch_push(const PushArgs& a, hipStream_t s);
-int launch_deposit(const DepositArgs& a, hipStream_t s);
+int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof);
+int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof);
+int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof);
 
 // Field solves ---------------------------------------------------------------
 
@@ -182,10 +213,11 @@ struct SolveBatch {
 struct SolverGeom {
     int N;
     FftPlan fft;
-    const double2* W;         // exp(-2 pi i t / L), t in [0, L)
+    const double* costab;     // cos(2 pi r / L), r in [0, L / 4]
     int P, Tp;                // pairs of lines per block, threads per pair
 };
 
-int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s);
+int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
+                 Profiler* prof);
 
 }  // namespace lcode

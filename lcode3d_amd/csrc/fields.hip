@@ -96,12 +96,28 @@ template <> struct Dft<5> {
     }
 };
 
+// Twiddle exp(-2 pi i t / L) from a quarter-wave cosine table in LDS:
+// ct[r] = cos(2 pi r / L), r in [0, L/4] (L is a multiple of 4).
+__device__ __forceinline__ double2 twiddle(const double* ct, int t, int L) {
+    const int Q = L >> 2;
+    const int quad = t / Q, r = t - quad * Q;
+    const double c = ct[r], s = ct[Q - r];   // cos and sin of 2 pi r / L
+    // cos / sin of 2 pi t / L = quad * pi / 2 + 2 pi r / L
+    double cs, sn;
+    switch (quad) {
+        case 0: cs = c; sn = s; break;
+        case 1: cs = -s; sn = c; break;
+        case 2: cs = -c; sn = -s; break;
+        default: cs = s; sn = -c; break;
+    }
+    return make_double2(cs, -sn);
+}
+
 // One radix-R Stockham stage over the L points of this thread's pair.
 // Thread t of Tp handles butterflies t and t + Tp (Tp >= L / 4 makes two
 // enough for every radix >= 2).  span = product of the previous radices.
 template <int R>
-__device__ __forceinline__ void fft_stage(double2* z,
-                                          const double2* __restrict__ W,
+__device__ __forceinline__ void fft_stage(double2* z, const double* ct,
                                           int L, int span, int t, int Tp) {
     const int T = L / R;
     const int tw = L / (span * R);
@@ -114,7 +130,7 @@ __device__ __forceinline__ void fft_stage(double2* z,
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 double2 x = z[b + q * T];
-                if (q) x = cmul(x, W[q * k * tw]);
+                if (q && k) x = cmul(x, twiddle(ct, q * k * tw, L));
                 v[it][q] = x;
             }
         }
@@ -136,19 +152,27 @@ __device__ __forceinline__ void fft_stage(double2* z,
 
 // In-place forward FFT of every pair's buffer (block-wide barriers inside:
 // every thread of the block must call it).
-__device__ void fft_lds(double2* z, const FftPlan& plan,
-                        const double2* __restrict__ W, int t, int Tp) {
+__device__ void fft_lds(double2* z, const FftPlan& plan, const double* ct,
+                        int t, int Tp) {
     int span = 1;
     for (int s = 0; s < plan.nstages; ++s) {
         const int R = plan.radix[s];
         switch (R) {
-            case 4: fft_stage<4>(z, W, plan.L, span, t, Tp); break;
-            case 2: fft_stage<2>(z, W, plan.L, span, t, Tp); break;
-            case 3: fft_stage<3>(z, W, plan.L, span, t, Tp); break;
-            default: fft_stage<5>(z, W, plan.L, span, t, Tp); break;
+            case 4: fft_stage<4>(z, ct, plan.L, span, t, Tp); break;
+            case 2: fft_stage<2>(z, ct, plan.L, span, t, Tp); break;
+            case 3: fft_stage<3>(z, ct, plan.L, span, t, Tp); break;
+            default: fft_stage<5>(z, ct, plan.L, span, t, Tp); break;
         }
         span *= R;
     }
+}
+
+// Copies the quarter-wave cosine table into LDS (after it: a barrier).
+__device__ __forceinline__ double* load_costab(double2* lds, int P, int L,
+                                               const double* __restrict__ cg) {
+    double* ct = reinterpret_cast<double*>(lds + (size_t)P * L);
+    for (int r = threadIdx.x; r <= (L >> 2); r += blockDim.x) ct[r] = cg[r];
+    return ct;
 }
 
 // Extension writers/readers.  Cosine (DCT-I, n points): z[k] and z[L-k]
@@ -264,6 +288,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     if ((int)blockIdx.x * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     double2* z = lds + (size_t)pair * L;
+    double* ct = load_costab(lds, g.P, L, g.costab);
     const int ra = ((int)blockIdx.x * g.P + pair) * 2, rb = ra + 1;
     const bool va = ra < lines, vb = rb < lines;
     for (int n = t; n < fp.lenA; n += Tp) {
@@ -274,7 +299,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     }
     put_ext_zeros(z, fp.xfA, fp.lenA, t);
     __syncthreads();
-    fft_lds(z, g.fft, g.W, t, Tp);
+    fft_lds(z, g.fft, ct, t, Tp);
     for (int k = t; k < fp.lenA; k += Tp) {
         double a, c;
         get_ext(z, fp.xfA, k, a, c);
@@ -295,6 +320,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     if ((int)blockIdx.x * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     double2* z = lds + (size_t)pair * L;
+    double* ct = load_costab(lds, g.P, L, g.costab);
     const int la = ((int)blockIdx.x * g.P + pair) * 2, lb = la + 1;
     const bool va = la < lines, vb = lb < lines;
     double* rowa = fp.T + (size_t)la * len;
@@ -306,7 +332,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
         put_ext(z, xf, n, len, L, va ? rowa[n] : 0., vb ? rowb[n] : 0.);
     put_ext_zeros(z, xf, len, t);
     __syncthreads();
-    fft_lds(z, g.fft, g.W, t, Tp);
+    fft_lds(z, g.fft, ct, t, Tp);
 
     double ha[kMaxPerThread], hb[kMaxPerThread];
 #pragma unroll
@@ -328,7 +354,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     }
     put_ext_zeros(z, xf, len, t);
     __syncthreads();
-    fft_lds(z, g.fft, g.W, t, Tp);
+    fft_lds(z, g.fft, ct, t, Tp);
 
     double sa = 0., sb = 0.;
     for (int k = t; k < len; k += Tp) {
@@ -371,6 +397,7 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     if ((int)blockIdx.x * g.P * 2 >= N) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     double2* z = lds + (size_t)pair * L;
+    double* ct = load_costab(lds, g.P, L, g.costab);
     const int ra = ((int)blockIdx.x * g.P + pair) * 2, rb = ra + 1;
     const int ca = ra - fp.rowsA0, cb = rb - fp.rowsA0;
     const bool ha = ra < N && ca >= 0 && ca < fp.rowsA;
@@ -382,7 +409,7 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     }
     put_ext_zeros(z, fp.xfA, fp.lenA, t);
     __syncthreads();
-    fft_lds(z, g.fft, g.W, t, Tp);
+    fft_lds(z, g.fft, ct, t, Tp);
 
     double mean = 0.;
     if (fp.partial) {
@@ -424,9 +451,11 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
 
 }  // namespace
 
-int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s) {
+int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
+                 Profiler* prof) {
     if (b.nfields == 0) return LCODE_OK;
-    const size_t lds = (size_t)g.P * g.fft.L * sizeof(double2);
+    const size_t lds = (size_t)g.P * g.fft.L * sizeof(double2) +
+                       ((size_t)g.fft.L / 4 + 1) * sizeof(double);
     const dim3 block(g.P * g.Tp);
     const int per_block = 2 * g.P;
     int max_a = 0, max_b = 0;
@@ -437,11 +466,20 @@ int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s) {
     const dim3 ga((max_a + per_block - 1) / per_block, b.nfields);
     const dim3 gb((max_b + per_block - 1) / per_block, b.nfields);
     const dim3 gc((g.N + per_block - 1) / per_block, b.nfields);
-    pass_a_kernel<<<ga, block, lds, s>>>(g, b);
+    {
+        ProfScope ps(prof, "solve_pass_a", s);
+        pass_a_kernel<<<ga, block, lds, s>>>(g, b);
+    }
     LCODE_HIP(hipGetLastError());
-    pass_b_kernel<<<gb, block, lds, s>>>(g, b);
+    {
+        ProfScope ps(prof, "solve_pass_b", s);
+        pass_b_kernel<<<gb, block, lds, s>>>(g, b);
+    }
     LCODE_HIP(hipGetLastError());
-    pass_c_kernel<<<gc, block, lds, s>>>(g, b);
+    {
+        ProfScope ps(prof, "solve_pass_c", s);
+        pass_c_kernel<<<gc, block, lds, s>>>(g, b);
+    }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }

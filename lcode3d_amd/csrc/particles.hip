@@ -146,10 +146,9 @@ __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
 }
 
 // Tiles of the owner-computes deposition: a block owns kTile x kTile cells
-// and deposits, in LDS, every fine particle whose cell moved at most kHalo
-// cells from its initial cell and whose stencil touches the tile.
+// and deposits, in LDS, every fine particle whose cell moved at most
+// a.halo cells from its initial cell and whose stencil touches the tile.
 constexpr int kTile = 32;
-constexpr int kHalo = 6;
 constexpr int kDepBlock = 256;
 
 // First fine index whose initial cell is >= c (fg is increasing).
@@ -167,7 +166,7 @@ __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
 // accumulates its kTile^2 cells of ro, jx, jy, jz in LDS (FP64 LDS atomics),
 // then stores them once, adding the ion background last (lcode.py:687).
 // Every cell of the grid is written by exactly one block, so the outputs
-// need no zeroing.  A particle displaced by more than kHalo cells is left
+// need no zeroing.  A particle displaced by more than a.halo cells is left
 // to deposit_spill_kernel, appended to the spill list by its home tile.
 __global__ void __launch_bounds__(kDepBlock)
 deposit_tile_kernel(DepositArgs a) {
@@ -178,10 +177,10 @@ deposit_tile_kernel(DepositArgs a) {
     for (int k = threadIdx.x; k < 4 * kTile * kTile; k += blockDim.x)
         (&acc[0][0][0])[k] = 0.;
     if (threadIdx.x == 0) {
-        range[0] = first_fine_at_or_after(a.fg, a.nf, ci0 - 1 - kHalo, N, a.h);
-        range[1] = first_fine_at_or_after(a.fg, a.nf, ci0 + kTile + 1 + kHalo, N, a.h);
-        range[2] = first_fine_at_or_after(a.fg, a.nf, cj0 - 1 - kHalo, N, a.h);
-        range[3] = first_fine_at_or_after(a.fg, a.nf, cj0 + kTile + 1 + kHalo, N, a.h);
+        range[0] = first_fine_at_or_after(a.fg, a.nf, ci0 - 1 - a.halo, N, a.h);
+        range[1] = first_fine_at_or_after(a.fg, a.nf, ci0 + kTile + 1 + a.halo, N, a.h);
+        range[2] = first_fine_at_or_after(a.fg, a.nf, cj0 - 1 - a.halo, N, a.h);
+        range[3] = first_fine_at_or_after(a.fg, a.nf, cj0 + kTile + 1 + a.halo, N, a.h);
     }
     __syncthreads();
     const int fi_lo = range[0], ni = range[1] - range[0];
@@ -196,7 +195,7 @@ deposit_tile_kernel(DepositArgs a) {
         fine_particle(a, fi, fj, fp);
         Tsc t;
         tsc_weights(fp.x, fp.y, N, a.h, t, home ? a.fault : nullptr);
-        if (abs(t.i - ci) > kHalo || abs(t.j - cj) > kHalo) {
+        if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
             if (home) a.spill[atomicAdd(a.spill_count, 1)] = fi * a.nf + fj;
             continue;
         }
@@ -264,25 +263,33 @@ inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
 
-int launch_estimate(const EstimateArgs& a, hipStream_t s) {
+int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof) {
     if (a.n == 0) return LCODE_OK;
+    ProfScope ps(prof, "estimate", s);
     estimate_kernel<<<blocks_for(a.n), kBlock, 0, s>>>(a);
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
 
-int launch_push(const PushArgs& a, hipStream_t s) {
+int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof) {
     if (a.n == 0) return LCODE_OK;
+    ProfScope ps(prof, "push", s);
     push_kernel<<<blocks_for(a.n), kBlock, 0, s>>>(a);
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
 
-int launch_deposit(const DepositArgs& a, hipStream_t s) {
+int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof) {
     const dim3 grid((a.N + kTile - 1) / kTile, (a.N + kTile - 1) / kTile);
-    deposit_tile_kernel<<<grid, kDepBlock, 0, s>>>(a);
+    {
+        ProfScope ps(prof, "deposit_tile", s);
+        deposit_tile_kernel<<<grid, kDepBlock, 0, s>>>(a);
+    }
     LCODE_HIP(hipGetLastError());
-    deposit_spill_kernel<<<kSpillBlocks, kBlock, 0, s>>>(a);
+    {
+        ProfScope ps(prof, "deposit_spill", s);
+        deposit_spill_kernel<<<kSpillBlocks, kBlock, 0, s>>>(a);
+    }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }

@@ -177,6 +177,38 @@ def fault_count(config=None, nc=None):
     return total
 
 
+def profile_begin(max_launches=8192):
+    """Start bracketing every kernel launch of this device's workspaces
+    with HIP events (instrumentation, not in the reference)."""
+    lib = _lib.load()
+    dev = torch.cuda.current_device()
+    for key, ws in _workspaces.items():
+        if key[0] == dev:
+            _lib.check(lib.lcode_profile_begin(ws, int(max_launches)),
+                       "lcode_profile_begin")
+
+
+def profile_end():
+    """Stop timing; returns {kernel name: (launches, total ms)}.
+    Synchronises."""
+    lib = _lib.load()
+    dev = torch.cuda.current_device()
+    out = {}
+    buf = (_lib.LcodeKernelTime * 64)()
+    n = ctypes.c_int32()
+    for key, ws in _workspaces.items():
+        if key[0] != dev:
+            continue
+        _lib.check(lib.lcode_profile_end(ws, _stream(), buf, 64,
+                                         ctypes.byref(n)),
+                   "lcode_profile_end")
+        for k in range(n.value):
+            name = buf[k].name.decode()
+            calls, ms = out.get(name, (0, 0.0))
+            out[name] = (calls + buf[k].calls, ms + buf[k].total_ms)
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Field solvers (lcode.py:160-364)
 # ---------------------------------------------------------------------------

@@ -115,13 +115,16 @@ def test_move_estimate_and_move_smart(lc, oracle, N, c, f):
     assert lc.fault_count(cfg, nc) == 0
 
 
-@pytest.mark.parametrize("N,c,f", [(33, 2, 2), (129, 2, 2), (641, 3, 2),
-                                   (129, 3, 3)])
-def test_deposit(lc, oracle, N, c, f):
-    say(f"deposit N={N} c={c} f={f}")
+@pytest.mark.parametrize("N,c,f,cells", [(33, 2, 2, 2), (129, 2, 2, 2),
+                                         (641, 3, 2, 2), (129, 3, 3, 2),
+                                         (129, 2, 2, 7), (641, 3, 2, 7)])
+def test_deposit(lc, oracle, N, c, f, cells):
+    """cells: amplitude of the random particle offsets in cells; at 7 most
+    particles leave the 3-cell tile halo and take the atomic spill path."""
+    say(f"deposit N={N} c={c} f={f} offsets<={cells} cells")
     cfg = _cfg(N, h=3.2 / N, c=c, f=f)
     const, vp, xo, yo, px, py, pz = _particles(oracle, cfg, N + 3,
-                                               2 * cfg.grid_step_size, .3)
+                                               cells * cfg.grid_step_size, .3)
     want = oracle.deposit(cfg, const.ro_initial, xo, yo, const.m, const.q,
                           px, py, pz, vp)
     dvp = lc.GPUArrays(**{k: getattr(vp, k) for k in (
