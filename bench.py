@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--settle", type=int, default=100,
+                    help="untimed slices run before warm-up (plasma develops)")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="steps timed per kernel with HIP events")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -125,7 +127,7 @@ def main():
     n = cfg.grid_steps
     total = args.warmup + args.steps
     # beam slices of the run, resident in HBM before the clock starts
-    first = 100
+    first = args.settle
     beams = [cosine_gaussian(cfg, first + k) for k in range(total)]
     for k in range(first):  # bring the plasma into a developed state
         state = lc.step(cfg, const, vp, state, cosine_gaussian(cfg, k))
@@ -204,7 +206,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic cosine-Gaussian beam (amplitude 0.05, sigma 1), "
-                "slices 100+ of the bunch",
+                f"slices {args.settle}+ of the bunch",
         "config": {"workload": "lcode3d xi step, 641^2 grid, coarseness 3, "
                                "fineness 2, plain FP64",
                    "grid_steps": n, "coarse_particles": nc * nc,
