@@ -161,12 +161,12 @@ def main():
         for k in range(args.profile_steps):
             state = lc.step(cfg, const, vp, state, beams[k % total])
         prof = lc.profile_end()
-        for name, (calls, ms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
-            avg_us = ms / calls * 1e3
+        for name, (calls, kms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
+            avg_us = kms / calls * 1e3
             b = kernel_bytes(name, nc, n)
             kernels[name] = {"launches_per_step": calls / args.profile_steps,
                              "avg_us": avg_us,
-                             "share": ms / sum(v[1] for v in prof.values()),
+                             "share": kms / sum(v[1] for v in prof.values()),
                              "GBps": (b / (avg_us * 1e-6) / 1e9) if b else None}
             log(f"  {name:14s} {calls / args.profile_steps:4.1f}/step "
                 f"{avg_us:8.2f} us  {kernels[name]['share'] * 100:5.1f}%"

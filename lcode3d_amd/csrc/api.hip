@@ -453,7 +453,7 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.spill = ws->spill;
     a.spill_count = ws->spill_ctr;
     a.spill_done = ws->spill_ctr + 1;
-    a.halo = kDepositHalo;
+    a.halo = kDepositHalo;   // <= 16 (LDS index tables of the tiles)
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
     return launch_deposit(a, stream, &ws->prof);
 }

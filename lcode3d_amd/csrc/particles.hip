@@ -166,48 +166,83 @@ __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
 // accumulates its kTile^2 cells of ro, jx, jy, jz in LDS (FP64 LDS atomics),
 // then stores them once, adding the ion background last (lcode.py:687).
 // Every cell of the grid is written by exactly one block, so the outputs
-// need no zeroing.  A particle displaced by more than a.halo cells is left
-// to deposit_spill_kernel, appended to the spill list by its home tile.
+// need no zeroing.  Work is split by initial cell: a thread takes the
+// fineness^2 fine particles that start in one cell, sums in registers the
+// contributions of those that land in the same cell as the first one, and
+// adds them to LDS with one set of 36 atomics.  A particle displaced by
+// more than a.halo cells is left to deposit_spill_kernel, appended to the
+// spill list by its home tile (the tile owning its initial cell).
+constexpr int kMaxExt = kTile + 2 * 16 + 3;   // extended rows for halo <= 16
+
+// Adds val[c][k] (or v[c] * w[k] when val is NULL) to the stencil points of
+// cell (i, j) that lie in this block's tile.
+__device__ __forceinline__ void lds_deposit(double (*acc)[kTile][kTile],
+                                            int i, int j, int ci0, int cj0,
+                                            const double* v, const double* w,
+                                            const double (*val)[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int si = i + stencil_di(k) - ci0;
+        const int sj = j + stencil_dj(k) - cj0;
+        if (si >= 0 && si < kTile && sj >= 0 && sj < kTile) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                atomicAdd(&acc[c][si][sj], val ? val[c][k] : v[c] * w[k]);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kDepBlock)
 deposit_tile_kernel(DepositArgs a) {
     __shared__ double acc[4][kTile][kTile];
-    __shared__ int range[4];
+    __shared__ int fsI[kMaxExt + 1], fsJ[kMaxExt + 1];
     const int N = a.N;
     const int ci0 = blockIdx.y * kTile, cj0 = blockIdx.x * kTile;
+    const int ext = kTile + 2 * a.halo + 2;       // initial cells per side
+    const int ci_lo = ci0 - 1 - a.halo, cj_lo = cj0 - 1 - a.halo;
     for (int k = threadIdx.x; k < 4 * kTile * kTile; k += blockDim.x)
         (&acc[0][0][0])[k] = 0.;
-    if (threadIdx.x == 0) {
-        range[0] = first_fine_at_or_after(a.fg, a.nf, ci0 - 1 - a.halo, N, a.h);
-        range[1] = first_fine_at_or_after(a.fg, a.nf, ci0 + kTile + 1 + a.halo, N, a.h);
-        range[2] = first_fine_at_or_after(a.fg, a.nf, cj0 - 1 - a.halo, N, a.h);
-        range[3] = first_fine_at_or_after(a.fg, a.nf, cj0 + kTile + 1 + a.halo, N, a.h);
+    for (int r = threadIdx.x; r <= ext; r += blockDim.x) {
+        fsI[r] = first_fine_at_or_after(a.fg, a.nf, ci_lo + r, N, a.h);
+        fsJ[r] = first_fine_at_or_after(a.fg, a.nf, cj_lo + r, N, a.h);
     }
     __syncthreads();
-    const int fi_lo = range[0], ni = range[1] - range[0];
-    const int fj_lo = range[2], nj = range[3] - range[2];
-    const int total = ni * nj;
-    for (int p = threadIdx.x; p < total; p += blockDim.x) {
-        const int fi = fi_lo + p / nj, fj = fj_lo + p % nj;
-        const int ci = cell_of(a.fg[fi], N, a.h), cj = cell_of(a.fg[fj], N, a.h);
+    for (int task = threadIdx.x; task < ext * ext; task += blockDim.x) {
+        const int r = task / ext, c = task - r * ext;
+        const int ci = ci_lo + r, cj = cj_lo + c;   // initial cell
         const bool home = ci >= ci0 && ci < ci0 + kTile &&
                           cj >= cj0 && cj < cj0 + kTile;
-        FineParticle fp;
-        fine_particle(a, fi, fj, fp);
-        Tsc t;
-        tsc_weights(fp.x, fp.y, N, a.h, t, home ? a.fault : nullptr);
-        if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
-            if (home) a.spill[atomicAdd(a.spill_count, 1)] = fi * a.nf + fj;
-            continue;
-        }
+        double sum[4][9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const int si = t.i + stencil_di(k) - ci0;
-            const int sj = t.j + stencil_dj(k) - cj0;
-            if (si >= 0 && si < kTile && sj >= 0 && sj < kTile) {
+        for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) atomicAdd(&acc[c][si][sj], fp.v[c] * t.w[k]);
+            for (int k = 0; k < 9; ++k) sum[q][k] = 0.;
+        int i0 = -1, j0 = -1;
+        for (int fi = fsI[r]; fi < fsI[r + 1]; ++fi) {
+            for (int fj = fsJ[c]; fj < fsJ[c + 1]; ++fj) {
+                FineParticle fp;
+                fine_particle(a, fi, fj, fp);
+                Tsc t;
+                tsc_weights(fp.x, fp.y, N, a.h, t, home ? a.fault : nullptr);
+                if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
+                    if (home) a.spill[atomicAdd(a.spill_count, 1)] = fi * a.nf + fj;
+                    continue;
+                }
+                if (i0 < 0) {
+                    i0 = t.i;
+                    j0 = t.j;
+                }
+                if (t.i == i0 && t.j == j0) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) sum[q][k] += fp.v[q] * t.w[k];
+                } else {
+                    lds_deposit(acc, t.i, t.j, ci0, cj0, fp.v, t.w, nullptr);
+                }
             }
         }
+        if (i0 >= 0) lds_deposit(acc, i0, j0, ci0, cj0, nullptr, nullptr, sum);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
