@@ -29,6 +29,7 @@ struct lcode_workspace {
     int nc = 0, nf = 0;
     SolverGeom geom{};
     double* costab = nullptr;      // quarter-wave cosine table
+    double2* stage_tw = nullptr;   // static-plan stage twiddles
     double* tab_mixed = nullptr;   // [M][N]   (DST index, DCT index)
     double* tab_mixed_t = nullptr; // [N][M]
     double* tab_dir = nullptr;     // [M][M]
@@ -197,11 +198,24 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         return fail(LCODE_E_UNSUPPORTED);
     }
     g.Tp = ((L + 3) / 4 + 63) / 64 * 64;
-    int P = 1024 / g.Tp;
-    const int by_lds = (65536 - (L / 4 + 1) * (int)sizeof(double)) /
-                       (int)(L * sizeof(double2));
-    if (by_lds < P) P = by_lds;
-    if (P > 4) P = 4;
+    g.static_L = solver_static_length(L) ? L : 0;
+    const int max_lds = 65536;
+    auto pairs_for = [&](int fixed_bytes) {
+        int P = 1024 / g.Tp;
+        const int by_lds = (max_lds - fixed_bytes) / (int)(L * sizeof(double2));
+        if (by_lds < P) P = by_lds;
+        return P > 4 ? 4 : P;
+    };
+    int P;
+    if (g.static_L) {
+        // stage twiddles in LDS when that still leaves two line pairs
+        P = pairs_for((L - 1) * (int)sizeof(double2));
+        g.tw_lds = P >= 2;
+        if (!g.tw_lds) P = pairs_for(0);
+    } else {
+        g.tw_lds = 0;
+        P = pairs_for((L / 4 + 1) * (int)sizeof(double));
+    }
     if (P < 1) {
         set_error("grid too large for one FFT line pair in 64 KiB of LDS");
         return fail(LCODE_E_UNSUPPORTED);
@@ -210,6 +224,25 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     if ((N + g.Tp - 1) / g.Tp > 4) {
         set_error("internal: too many points per thread");
         return fail(LCODE_E_UNSUPPORTED);
+    }
+    if (g.static_L) {
+        // stage s with span sp and radix R: exp(-2 pi i q k / (sp R)) at
+        // sp - 1 + k (R - 1) + q - 1
+        std::vector<double2> tw(L - 1);
+        int sp = 1;
+        for (int st = 0; st < g.fft.nstages; ++st) {
+            const int R = g.fft.radix[st];
+            for (int k = 0; k < sp; ++k)
+                for (int q = 1; q < R; ++q) {
+                    const long double a = 2.0L * 3.14159265358979323846264338327950288L *
+                                          q * k / ((long double)sp * R);
+                    tw[sp - 1 + k * (R - 1) + q - 1] =
+                        make_double2((double)std::cos(a), (double)-std::sin(a));
+                }
+            sp *= R;
+        }
+        if ((rc = upload(ws, &ws->stage_tw, tw))) return fail(rc);
+        g.stage_tw = ws->stage_tw;
     }
 
     std::vector<double> costab(L / 4 + 1);

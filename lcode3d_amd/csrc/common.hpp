@@ -214,10 +214,15 @@ struct SolverGeom {
     int N;
     FftPlan fft;
     const double* costab;     // cos(2 pi r / L), r in [0, L / 4]
+    const double2* stage_tw;  // per-stage twiddles of the static plan
+    int static_L;             // L with a compile-time plan, else 0
+    int tw_lds;               // static plan: twiddles copied to LDS
     int P, Tp;                // pairs of lines per block, threads per pair
 };
 
 int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
                  Profiler* prof);
+bool solver_static_length(int L);
+int solver_lds_bytes(const SolverGeom& g);
 
 }  // namespace lcode

@@ -175,6 +175,93 @@ __device__ __forceinline__ double* load_costab(double2* lds, int P, int L,
     return ct;
 }
 
+// Compile-time plans for the common transform lengths: the radix of each
+// stage and its span are constants, so the index arithmetic folds away,
+// and the twiddles of a stage with span s and radix R,
+// exp(-2 pi i q k / (s R)) for k < s, 1 <= q < R, sit contiguously at
+// tw[s - 1 + k (R - 1) + q - 1] (L - 1 entries for the whole plan).
+__host__ __device__ constexpr int next_radix(int rem) {
+    return rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
+}
+
+__host__ __device__ constexpr int threads_per_pair(int L) {
+    return ((L + 3) / 4 + 63) / 64 * 64;
+}
+
+template <int L, int SPAN>
+__device__ __forceinline__ void fft_static(double2* z,
+                                           const double2* __restrict__ tw,
+                                           int t) {
+    if constexpr (SPAN < L) {
+        constexpr int R = next_radix(L / SPAN);
+        constexpr int T = L / R;
+        constexpr int Tp = threads_per_pair(L);
+        constexpr int ITER = (T + Tp - 1) / Tp;
+        const double2* tws = tw + (SPAN - 1);
+        double2 v[ITER][R];
+#pragma unroll
+        for (int it = 0; it < ITER; ++it) {
+            const int b = t + it * Tp;
+            if (b < T) {
+                const int k = b % SPAN;
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    double2 x = z[b + q * T];
+                    if (SPAN > 1 && q) x = cmul(x, tws[k * (R - 1) + q - 1]);
+                    v[it][q] = x;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < ITER; ++it) {
+            const int b = t + it * Tp;
+            if (b < T) {
+                Dft<R>::run(v[it]);
+                const int k = b % SPAN;
+                const int j = (b - k) * R + k;
+#pragma unroll
+                for (int q = 0; q < R; ++q) z[j + q * SPAN] = v[it][q];
+            }
+        }
+        __syncthreads();
+        fft_static<L, SPAN * R>(z, tw, t);
+    }
+}
+
+// Twiddle source of a block: for L == 0 (runtime plan) the quarter-wave
+// cosine table, copied to LDS; for a static L the stage twiddles, copied to
+// LDS when TW_LDS, else read from global memory.
+struct Tw {
+    const double* ct;
+    const double2* tw;
+};
+
+template <int L, bool TW_LDS>
+__device__ __forceinline__ Tw load_twiddles(double2* lds, const SolverGeom& g) {
+    Tw r{nullptr, nullptr};
+    if constexpr (L == 0) {
+        r.ct = load_costab(lds, g.P, g.fft.L, g.costab);
+    } else if constexpr (TW_LDS) {
+        double2* d = lds + (size_t)g.P * L;
+        for (int k = threadIdx.x; k < L - 1; k += blockDim.x) d[k] = g.stage_tw[k];
+        r.tw = d;
+    } else {
+        r.tw = g.stage_tw;
+    }
+    return r;
+}
+
+template <int L>
+__device__ __forceinline__ void fft_any(double2* z, const SolverGeom& g,
+                                        const Tw& tw, int t) {
+    if constexpr (L == 0) {
+        fft_lds(z, g.fft, tw.ct, t, g.Tp);
+    } else {
+        fft_static<L, 1>(z, tw.tw, t);
+    }
+}
+
 // Extension writers/readers.  Cosine (DCT-I, n points): z[k] and z[L-k]
 // hold input k.  Sine (DST-I, n points): z[k+1] = input k, z[L-1-k] =
 // -input k, z[0] = z[n+1] = 0.  For a pair (a real, b imaginary) the FFT
@@ -279,6 +366,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Passes
 // ---------------------------------------------------------------------------
 
+template <int L_, bool TW_LDS>
 __global__ void __launch_bounds__(1024)
 pass_a_kernel(SolverGeom g, SolveBatch b) {
     extern __shared__ double2 lds[];
@@ -288,7 +376,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     if ((int)blockIdx.x * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     double2* z = lds + (size_t)pair * L;
-    double* ct = load_costab(lds, g.P, L, g.costab);
+    const Tw tw = load_twiddles<L_, TW_LDS>(lds, g);
     const int ra = ((int)blockIdx.x * g.P + pair) * 2, rb = ra + 1;
     const bool va = ra < lines, vb = rb < lines;
     for (int n = t; n < fp.lenA; n += Tp) {
@@ -299,7 +387,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     }
     put_ext_zeros(z, fp.xfA, fp.lenA, t);
     __syncthreads();
-    fft_lds(z, g.fft, ct, t, Tp);
+    fft_any<L_>(z, g, tw, t);
     for (int k = t; k < fp.lenA; k += Tp) {
         double a, c;
         get_ext(z, fp.xfA, k, a, c);
@@ -310,6 +398,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
 
 constexpr int kMaxPerThread = 4;
 
+template <int L_, bool TW_LDS>
 __global__ void __launch_bounds__(1024)
 pass_b_kernel(SolverGeom g, SolveBatch b) {
     extern __shared__ double2 lds[];
@@ -320,7 +409,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     if ((int)blockIdx.x * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     double2* z = lds + (size_t)pair * L;
-    double* ct = load_costab(lds, g.P, L, g.costab);
+    const Tw tw = load_twiddles<L_, TW_LDS>(lds, g);
     const int la = ((int)blockIdx.x * g.P + pair) * 2, lb = la + 1;
     const bool va = la < lines, vb = lb < lines;
     double* rowa = fp.T + (size_t)la * len;
@@ -332,7 +421,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
         put_ext(z, xf, n, len, L, va ? rowa[n] : 0., vb ? rowb[n] : 0.);
     put_ext_zeros(z, xf, len, t);
     __syncthreads();
-    fft_lds(z, g.fft, ct, t, Tp);
+    fft_any<L_>(z, g, tw, t);
 
     double ha[kMaxPerThread], hb[kMaxPerThread];
 #pragma unroll
@@ -354,7 +443,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     }
     put_ext_zeros(z, xf, len, t);
     __syncthreads();
-    fft_lds(z, g.fft, ct, t, Tp);
+    fft_any<L_>(z, g, tw, t);
 
     double sa = 0., sb = 0.;
     for (int k = t; k < len; k += Tp) {
@@ -388,6 +477,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     }
 }
 
+template <int L_, bool TW_LDS>
 __global__ void __launch_bounds__(1024)
 pass_c_kernel(SolverGeom g, SolveBatch b) {
     extern __shared__ double2 lds[];
@@ -397,7 +487,7 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     if ((int)blockIdx.x * g.P * 2 >= N) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     double2* z = lds + (size_t)pair * L;
-    double* ct = load_costab(lds, g.P, L, g.costab);
+    const Tw tw = load_twiddles<L_, TW_LDS>(lds, g);
     const int ra = ((int)blockIdx.x * g.P + pair) * 2, rb = ra + 1;
     const int ca = ra - fp.rowsA0, cb = rb - fp.rowsA0;
     const bool ha = ra < N && ca >= 0 && ca < fp.rowsA;
@@ -409,7 +499,7 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     }
     put_ext_zeros(z, fp.xfA, fp.lenA, t);
     __syncthreads();
-    fft_lds(z, g.fft, ct, t, Tp);
+    fft_any<L_>(z, g, tw, t);
 
     double mean = 0.;
     if (fp.partial) {
@@ -451,12 +541,50 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
 
 }  // namespace
 
+template <int L_, bool TW_LDS>
+static int launch_passes(const SolverGeom& g, const SolveBatch& b,
+                         hipStream_t s, Profiler* prof, size_t lds,
+                         dim3 ga, dim3 gb, dim3 gc) {
+    const dim3 block(g.P * g.Tp);
+    {
+        ProfScope ps(prof, "solve_pass_a", s);
+        pass_a_kernel<L_, TW_LDS><<<ga, block, lds, s>>>(g, b);
+    }
+    LCODE_HIP(hipGetLastError());
+    {
+        ProfScope ps(prof, "solve_pass_b", s);
+        pass_b_kernel<L_, TW_LDS><<<gb, block, lds, s>>>(g, b);
+    }
+    LCODE_HIP(hipGetLastError());
+    {
+        ProfScope ps(prof, "solve_pass_c", s);
+        pass_c_kernel<L_, TW_LDS><<<gc, block, lds, s>>>(g, b);
+    }
+    LCODE_HIP(hipGetLastError());
+    return LCODE_OK;
+}
+
+int solver_lds_bytes(const SolverGeom& g) {
+    const size_t data = (size_t)g.P * g.fft.L * sizeof(double2);
+    if (g.static_L == 0) return (int)(data + ((size_t)g.fft.L / 4 + 1) * sizeof(double));
+    if (g.tw_lds) return (int)(data + (size_t)(g.fft.L - 1) * sizeof(double2));
+    return (int)data;
+}
+
+bool solver_static_length(int L) {
+    switch (L) {
+        case 32: case 64: case 128: case 256: case 512: case 640:
+        case 1024: case 1280: case 2048: case 2560:
+            return true;
+        default:
+            return false;
+    }
+}
+
 int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
                  Profiler* prof) {
     if (b.nfields == 0) return LCODE_OK;
-    const size_t lds = (size_t)g.P * g.fft.L * sizeof(double2) +
-                       ((size_t)g.fft.L / 4 + 1) * sizeof(double);
-    const dim3 block(g.P * g.Tp);
+    const size_t lds = (size_t)solver_lds_bytes(g);
     const int per_block = 2 * g.P;
     int max_a = 0, max_b = 0;
     for (int f = 0; f < b.nfields; ++f) {
@@ -466,22 +594,18 @@ int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
     const dim3 ga((max_a + per_block - 1) / per_block, b.nfields);
     const dim3 gb((max_b + per_block - 1) / per_block, b.nfields);
     const dim3 gc((g.N + per_block - 1) / per_block, b.nfields);
-    {
-        ProfScope ps(prof, "solve_pass_a", s);
-        pass_a_kernel<<<ga, block, lds, s>>>(g, b);
+#define LCODE_CASE(LL)                                                        \
+    case LL:                                                                  \
+        return g.tw_lds ? launch_passes<LL, true>(g, b, s, prof, lds, ga, gb, gc) \
+                        : launch_passes<LL, false>(g, b, s, prof, lds, ga, gb, gc);
+    switch (g.static_L) {
+        LCODE_CASE(32) LCODE_CASE(64) LCODE_CASE(128) LCODE_CASE(256)
+        LCODE_CASE(512) LCODE_CASE(640) LCODE_CASE(1024) LCODE_CASE(1280)
+        LCODE_CASE(2048) LCODE_CASE(2560)
+        default:
+            return launch_passes<0, false>(g, b, s, prof, lds, ga, gb, gc);
     }
-    LCODE_HIP(hipGetLastError());
-    {
-        ProfScope ps(prof, "solve_pass_b", s);
-        pass_b_kernel<<<gb, block, lds, s>>>(g, b);
-    }
-    LCODE_HIP(hipGetLastError());
-    {
-        ProfScope ps(prof, "solve_pass_c", s);
-        pass_c_kernel<<<gc, block, lds, s>>>(g, b);
-    }
-    LCODE_HIP(hipGetLastError());
-    return LCODE_OK;
+#undef LCODE_CASE
 }
 
 }  // namespace lcode
