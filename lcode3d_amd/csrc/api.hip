@@ -100,6 +100,8 @@ int check_cfg(const lcode_config* c) {
     LCODE_REQUIRE(c->nc >= 0 && c->nf >= 0, "negative particle counts");
     LCODE_REQUIRE(c->plasma_coarseness > 0 && c->plasma_fineness > 0,
                   "plasma_coarseness and plasma_fineness must be positive");
+    LCODE_REQUIRE(c->plasma_fineness <= 3,
+                  "plasma_fineness > 3 is not supported (deposition tile tables)");
     LCODE_REQUIRE(c->solver == LCODE_SOLVER_FFT64, "unknown solver");
     return LCODE_OK;
 }
@@ -486,7 +488,9 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.spill = ws->spill;
     a.spill_count = ws->spill_ctr;
     a.spill_done = ws->spill_ctr + 1;
-    a.halo = kDepositHalo;   // <= 16 (LDS index tables of the tiles)
+    a.halo = kDepositHalo;
+    // the tile's coarse rectangle has at most ext / coarseness + 3 rows
+    a.staged = (32 + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= 18;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
     return launch_deposit(a, stream, &ws->prof);
 }

@@ -87,10 +87,15 @@ __device__ __forceinline__ int cell_of(double x, int N, double h) {
 // Cell index and quadratic-B-spline weights of the point (x, y).  A centre
 // cell whose 3x3 stencil would leave the grid is clamped into it and, when
 // fault is not NULL, counted there (the reference has no check: there it
-// is an out-of-bounds access).
+// is an out-of-bounds access).  x_h = x / h + .5 exactly as the reference
+// (lcode.py:404), or x * inv_h + .5 when inv_h is given (used by the
+// deposition; a 1-ulp change of x_h moves no weight measurably: the TSC
+// weights are continuous across cell edges).
 __device__ __forceinline__ void tsc_weights(double x, double y, int N,
-                                            double h, Tsc& t, int* fault) {
-    const double x_h = x / h + .5, y_h = y / h + .5;
+                                            double h, Tsc& t, int* fault,
+                                            double inv_h = 0.) {
+    const double x_h = inv_h != 0. ? x * inv_h + .5 : x / h + .5;
+    const double y_h = inv_h != 0. ? y * inv_h + .5 : y / h + .5;
     const double fx = floor(x_h), fy = floor(y_h);
     int i = (int)fx + N / 2, j = (int)fy + N / 2;
     const double x_loc = x_h - fx - .5, y_loc = y_h - fy - .5;
@@ -159,6 +164,7 @@ struct DepositArgs {
     int* spill_count;              // zero between calls
     int* spill_done;               // zero between calls
     int halo;                      // tile halo in cells
+    int staged;                    // coarse rectangle of a tile fits LDS
 };
 
 int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof);

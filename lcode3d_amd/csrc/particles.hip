@@ -47,7 +47,9 @@ estimate_kernel(EstimateArgs a) {
 // move_smart_kernel (lcode.py:704-793): midpoint position, 6-field TSC
 // gather, two fixed-point iterations of the half-step momentum, offset and
 // momentum update, reflection with strict > / < flipping the momentum.
-__global__ void __launch_bounds__(kBlock)
+constexpr int kPushBlock = 64;
+
+__global__ void __launch_bounds__(kPushBlock)
 push_kernel(PushArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.n) return;
@@ -109,8 +111,23 @@ push_kernel(PushArgs a) {
     a.out_pz[k] = pz;
 }
 
+// Deposition values of a fine particle with mass m, charge q, momentum p
+// (lcode.py:651-655): dro = q / (1 - pz / gamma), dj = p dro / gamma,
+// evaluated with one division as dro = q gamma r, dj = q p r,
+// r = 1 / (gamma - pz) (the same quantities, rounded differently).
+__device__ __forceinline__ void deposit_values(double m, double q, double px,
+                                               double py, double pz,
+                                               double* v) {
+    const double gamma_m = sqrt(m * m + px * px + py * py + pz * pz);
+    const double qr = q / (gamma_m - pz);
+    v[0] = gamma_m * qr;
+    v[1] = px * qr;
+    v[2] = py * qr;
+    v[3] = pz * qr;
+}
+
 // Coarse->fine interpolation of fine particle (fi, fj) (lcode.py:591-621)
-// and its deposition values (lcode.py:651-655).
+// and its deposition values, reading everything from global memory.
 struct FineParticle {
     double x, y, v[4];   // position; dro, djx, djy, djz
 };
@@ -131,25 +148,16 @@ __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
     p.x = a.fg[fi] + LCODE_MIX(a.x_offt);
     p.y = a.fg[fj] + LCODE_MIX(a.y_offt);
     const double s = a.smallness;
-    const double m = s * LCODE_MIX(a.m);
-    const double q = s * LCODE_MIX(a.q);
-    const double px = s * LCODE_MIX(a.px);
-    const double py = s * LCODE_MIX(a.py);
-    const double pz = s * LCODE_MIX(a.pz);
+    deposit_values(s * LCODE_MIX(a.m), s * LCODE_MIX(a.q), s * LCODE_MIX(a.px),
+                   s * LCODE_MIX(a.py), s * LCODE_MIX(a.pz), p.v);
 #undef LCODE_MIX
-    const double gamma_m = sqrt(m * m + px * px + py * py + pz * pz);
-    const double dro = q / (1 - pz / gamma_m);
-    p.v[0] = dro;
-    p.v[1] = px * (dro / gamma_m);
-    p.v[2] = py * (dro / gamma_m);
-    p.v[3] = pz * (dro / gamma_m);
 }
 
 // Tiles of the owner-computes deposition: a block owns kTile x kTile cells
 // and deposits, in LDS, every fine particle whose cell moved at most
 // a.halo cells from its initial cell and whose stencil touches the tile.
 constexpr int kTile = 32;
-constexpr int kDepBlock = 256;
+constexpr int kDepBlock = 512;
 
 // First fine index whose initial cell is >= c (fg is increasing).
 __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
@@ -163,86 +171,140 @@ __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
 }
 
 // deposit_kernel (lcode.py:626-663) as owner-computes tiles: each block
-// accumulates its kTile^2 cells of ro, jx, jy, jz in LDS (FP64 LDS atomics),
-// then stores them once, adding the ion background last (lcode.py:687).
-// Every cell of the grid is written by exactly one block, so the outputs
-// need no zeroing.  Work is split by initial cell: a thread takes the
-// fineness^2 fine particles that start in one cell, sums in registers the
-// contributions of those that land in the same cell as the first one, and
-// adds them to LDS with one set of 36 atomics.  A particle displaced by
-// more than a.halo cells is left to deposit_spill_kernel, appended to the
-// spill list by its home tile (the tile owning its initial cell).
-constexpr int kMaxExt = kTile + 2 * 16 + 3;   // extended rows for halo <= 16
+// accumulates its kTile^2 cells of ro, jx, jy, jz in LDS (FP64 LDS atomics,
+// ds_add_f64), then stores them once, adding the ion background last
+// (lcode.py:687).  Every cell of the grid is written by exactly one block,
+// so the outputs need no zeroing.  A block visits the fine particles whose
+// initial cell lies within halo + 1 cells of its tile; their per-index
+// interpolation data and, when they fit, the coarse particles they
+// interpolate from are staged in LDS first.  A particle displaced by more
+// than a.halo cells from its initial cell is left to deposit_spill_kernel,
+// appended to the spill list by its home tile (the tile owning its initial
+// cell).
+constexpr int kFineMax = 128;    // fine indices per side: fineness (kTile + 2 halo + 2)
+constexpr int kCoarseMax = 18;   // coarse particles per side staged in LDS
 
-// Adds val[c][k] (or v[c] * w[k] when val is NULL) to the stencil points of
-// cell (i, j) that lie in this block's tile.
-__device__ __forceinline__ void lds_deposit(double (*acc)[kTile][kTile],
-                                            int i, int j, int ci0, int cj0,
-                                            const double* v, const double* w,
-                                            const double (*val)[9]) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const int si = i + stencil_di(k) - ci0;
-        const int sj = j + stencil_dj(k) - cj0;
-        if (si >= 0 && si < kTile && sj >= 0 && sj < kTile) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                atomicAdd(&acc[c][si][sj], val ? val[c][k] : v[c] * w[k]);
-        }
-    }
-}
+struct FineTab {                 // per fine index along one axis
+    double ip[kFineMax], in[kFineMax], fg[kFineMax];
+    int P[kFineMax], Nn[kFineMax], cell[kFineMax];
+};
 
-__global__ void __launch_bounds__(kDepBlock)
-deposit_tile_kernel(DepositArgs a) {
+template <bool STAGED>
+__device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     __shared__ double acc[4][kTile][kTile];
-    __shared__ int fsI[kMaxExt + 1], fsJ[kMaxExt + 1];
+    __shared__ FineTab ti, tj;
+    __shared__ double cs[7][kCoarseMax * kCoarseMax];
+    __shared__ int range[4];
     const int N = a.N;
+    const double inv_h = 1. / a.h;
     const int ci0 = blockIdx.y * kTile, cj0 = blockIdx.x * kTile;
-    const int ext = kTile + 2 * a.halo + 2;       // initial cells per side
+    const int ext = kTile + 2 * a.halo + 2;
     const int ci_lo = ci0 - 1 - a.halo, cj_lo = cj0 - 1 - a.halo;
     for (int k = threadIdx.x; k < 4 * kTile * kTile; k += blockDim.x)
         (&acc[0][0][0])[k] = 0.;
-    for (int r = threadIdx.x; r <= ext; r += blockDim.x) {
-        fsI[r] = first_fine_at_or_after(a.fg, a.nf, ci_lo + r, N, a.h);
-        fsJ[r] = first_fine_at_or_after(a.fg, a.nf, cj_lo + r, N, a.h);
+    if (threadIdx.x < 4) {
+        const int c = (threadIdx.x & 1) ? ext : 0;
+        const int lo = (threadIdx.x < 2) ? ci_lo : cj_lo;
+        range[threadIdx.x] = first_fine_at_or_after(a.fg, a.nf, lo + c, N, a.h);
     }
     __syncthreads();
-    for (int task = threadIdx.x; task < ext * ext; task += blockDim.x) {
-        const int r = task / ext, c = task - r * ext;
-        const int ci = ci_lo + r, cj = cj_lo + c;   // initial cell
+    const int fi_a = range[0], ni = range[1] - range[0];
+    const int fj_a = range[2], nj = range[3] - range[2];
+    if (ni > kFineMax || nj > kFineMax) {
+        // denser fine grid than the host-checked fineness allows: flag the
+        // tile as faulted and fill it with NaN rather than drop particles
+        if (threadIdx.x == 0) atomicAdd(a.fault, 1);
+        for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
+            const int i = ci0 + k / kTile, j = cj0 + k % kTile;
+            if (i >= N || j >= N) continue;
+            const size_t idx = (size_t)i * N + j;
+            a.ro[idx] = a.jx[idx] = a.jy[idx] = a.jz[idx] = __builtin_nan("");
+        }
+        return;
+    }
+    for (int k = threadIdx.x; k < ni + nj; k += blockDim.x) {
+        const bool isj = k >= ni;
+        const int f = isj ? fj_a + k - ni : fi_a + k;
+        FineTab& t = isj ? tj : ti;
+        const int s = isj ? k - ni : k;
+        t.ip[s] = a.ip[f];
+        t.in[s] = a.in[f];
+        t.fg[s] = a.fg[f];
+        t.P[s] = clampi((int)a.P[f], a.nc);
+        t.Nn[s] = clampi((int)a.Nn[f], a.nc);
+        t.cell[s] = cell_of(a.fg[f], N, a.h);
+    }
+    __syncthreads();
+    // coarse rectangle referenced by this block's fine particles
+    int cI0 = 0, cJ0 = 0, cW = 0;
+    if constexpr (STAGED) {
+        // indices_prev/next grow with the fine index, so the rectangle is
+        // [P[first], Nn[last]] on each axis; the host checked that it fits
+        cI0 = ni ? ti.P[0] : 0;
+        cJ0 = nj ? tj.P[0] : 0;
+        int cH = ni ? ti.Nn[ni - 1] - cI0 + 1 : 0;
+        cW = nj ? tj.Nn[nj - 1] - cJ0 + 1 : 0;
+        if (cH > kCoarseMax || cW > kCoarseMax || cH < 0 || cW < 0) {
+            if (threadIdx.x == 0) atomicAdd(a.fault, 1);
+            cH = cH < 0 ? 0 : (cH > kCoarseMax ? kCoarseMax : cH);
+            cW = cW < 0 ? 0 : (cW > kCoarseMax ? kCoarseMax : cW);
+        }
+        const double* src[7] = {a.x_offt, a.y_offt, a.m, a.q, a.px, a.py, a.pz};
+        for (int k = threadIdx.x; k < cH * cW; k += blockDim.x) {
+            const size_t g = (size_t)(cI0 + k / cW) * a.nc + cJ0 + k % cW;
+#pragma unroll
+            for (int c = 0; c < 7; ++c) cs[c][k] = src[c][g];
+        }
+        __syncthreads();
+    }
+    const double sm = a.smallness;
+    for (int p = threadIdx.x; p < ni * nj; p += blockDim.x) {
+        const int li = p / nj, lj = p - li * nj;
+        const int ci = ti.cell[li], cj = tj.cell[lj];   // initial cell
         const bool home = ci >= ci0 && ci < ci0 + kTile &&
                           cj >= cj0 && cj < cj0 + kTile;
-        double sum[4][9];
+        const double A = ti.ip[li] * tj.ip[lj], B = ti.ip[li] * tj.in[lj];
+        const double C = ti.in[li] * tj.ip[lj], D = ti.in[li] * tj.in[lj];
+        double x, y, v[4];
+        if constexpr (STAGED) {
+            const int last = kCoarseMax * kCoarseMax - 1;
+            const int pp = min(max((ti.P[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
+            const int pn = min(max((ti.P[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
+            const int np = min(max((ti.Nn[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
+            const int nn = min(max((ti.Nn[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
+#define LCODE_MIX(c) (A * cs[c][pp] + B * cs[c][pn] + C * cs[c][np] + D * cs[c][nn])
+            x = ti.fg[li] + LCODE_MIX(0);
+            y = tj.fg[lj] + LCODE_MIX(1);
+            deposit_values(sm * LCODE_MIX(2), sm * LCODE_MIX(3), sm * LCODE_MIX(4),
+                           sm * LCODE_MIX(5), sm * LCODE_MIX(6), v);
+#undef LCODE_MIX
+        } else {
+            const int nc = a.nc;
+            const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
+            const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
+#define LCODE_MIX(arr) (A * arr[pp] + B * arr[pn] + C * arr[np] + D * arr[nn])
+            x = ti.fg[li] + LCODE_MIX(a.x_offt);
+            y = tj.fg[lj] + LCODE_MIX(a.y_offt);
+            deposit_values(sm * LCODE_MIX(a.m), sm * LCODE_MIX(a.q),
+                           sm * LCODE_MIX(a.px), sm * LCODE_MIX(a.py),
+                           sm * LCODE_MIX(a.pz), v);
+#undef LCODE_MIX
+        }
+        Tsc t;
+        tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
+        if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
+            if (home) a.spill[atomicAdd(a.spill_count, 1)] = (fi_a + li) * a.nf + fj_a + lj;
+            continue;
+        }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int k = 0; k < 9; ++k) {
+            const int si = t.i + stencil_di(k) - ci0;
+            const int sj = t.j + stencil_dj(k) - cj0;
+            if (si >= 0 && si < kTile && sj >= 0 && sj < kTile) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) sum[q][k] = 0.;
-        int i0 = -1, j0 = -1;
-        for (int fi = fsI[r]; fi < fsI[r + 1]; ++fi) {
-            for (int fj = fsJ[c]; fj < fsJ[c + 1]; ++fj) {
-                FineParticle fp;
-                fine_particle(a, fi, fj, fp);
-                Tsc t;
-                tsc_weights(fp.x, fp.y, N, a.h, t, home ? a.fault : nullptr);
-                if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
-                    if (home) a.spill[atomicAdd(a.spill_count, 1)] = fi * a.nf + fj;
-                    continue;
-                }
-                if (i0 < 0) {
-                    i0 = t.i;
-                    j0 = t.j;
-                }
-                if (t.i == i0 && t.j == j0) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-#pragma unroll
-                        for (int k = 0; k < 9; ++k) sum[q][k] += fp.v[q] * t.w[k];
-                } else {
-                    lds_deposit(acc, t.i, t.j, ci0, cj0, fp.v, t.w, nullptr);
-                }
+                for (int c = 0; c < 4; ++c) atomicAdd(&acc[c][si][sj], v[c] * t.w[k]);
             }
         }
-        if (i0 >= 0) lds_deposit(acc, i0, j0, ci0, cj0, nullptr, nullptr, sum);
     }
     __syncthreads();
     for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
@@ -259,6 +321,12 @@ deposit_tile_kernel(DepositArgs a) {
     }
 }
 
+template <bool STAGED>
+__global__ void __launch_bounds__(kDepBlock)
+deposit_tile_kernel(DepositArgs a) {
+    deposit_tile_body<STAGED>(a);
+}
+
 // Atomic deposition of the spilled particles; the last block to finish
 // resets the list for the next call (every block has read the count by
 // then).
@@ -272,7 +340,7 @@ deposit_spill_kernel(DepositArgs a) {
         FineParticle fp;
         fine_particle(a, fk / a.nf, fk % a.nf, fp);
         Tsc t;
-        tsc_weights(fp.x, fp.y, N, a.h, t, nullptr);
+        tsc_weights(fp.x, fp.y, N, a.h, t, nullptr, 1. / a.h);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             const size_t idx = (size_t)(t.i + stencil_di(k)) * N + (t.j + stencil_dj(k));
@@ -309,7 +377,8 @@ int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof) {
 int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof) {
     if (a.n == 0) return LCODE_OK;
     ProfScope ps(prof, "push", s);
-    push_kernel<<<blocks_for(a.n), kBlock, 0, s>>>(a);
+    // one wave per block spreads the (few) coarse particles over all CUs
+    push_kernel<<<(a.n + kPushBlock - 1) / kPushBlock, kPushBlock, 0, s>>>(a);
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
@@ -318,7 +387,10 @@ int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof) {
     const dim3 grid((a.N + kTile - 1) / kTile, (a.N + kTile - 1) / kTile);
     {
         ProfScope ps(prof, "deposit_tile", s);
-        deposit_tile_kernel<<<grid, kDepBlock, 0, s>>>(a);
+        if (a.staged)
+            deposit_tile_kernel<true><<<grid, kDepBlock, 0, s>>>(a);
+        else
+            deposit_tile_kernel<false><<<grid, kDepBlock, 0, s>>>(a);
     }
     LCODE_HIP(hipGetLastError());
     {
