@@ -142,7 +142,7 @@ def main():
         state = lc.step(cfg, const, vp, state, beams[args.warmup + k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    assert torch.isfinite(state.Ez).all().item()
+    finite = bool(torch.isfinite(state.Ez).all().item())
     disp = max(state.x_offt.abs().max().item(), state.y_offt.abs().max().item())
     log(f"max coarse displacement {disp / cfg.grid_step_size:.2f} cells")
     faults = lc.fault_count(cfg, nc)
@@ -216,6 +216,9 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if not finite:
+        log("ERROR: non-finite fields at the end of the timed run")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

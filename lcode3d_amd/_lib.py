@@ -14,7 +14,8 @@ import torch  # noqa: F401  (must load the HIP runtime before the library)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_NAME = "liblcode3d_hip.so"
-LIB_PATH = os.path.join(HERE, LIB_NAME)
+#: LCODE3D_HIP_LIB selects another build of the same ABI (e.g. a debug one)
+LIB_PATH = os.environ.get("LCODE3D_HIP_LIB") or os.path.join(HERE, LIB_NAME)
 
 LCODE_OK = 0
 LCODE_SOLVER_FFT64 = 0
