@@ -157,7 +157,7 @@ __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
 // and deposits, in LDS, every fine particle whose cell moved at most
 // a.halo cells from its initial cell and whose stencil touches the tile.
 constexpr int kTile = 32;
-constexpr int kDepBlock = 512;
+constexpr int kDepBlock = 256;
 
 // First fine index whose initial cell is >= c (fg is increasing).
 __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
@@ -177,17 +177,75 @@ __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
 // so the outputs need no zeroing.  A block visits the fine particles whose
 // initial cell lies within halo + 1 cells of its tile; their per-index
 // interpolation data and, when they fit, the coarse particles they
-// interpolate from are staged in LDS first.  A particle displaced by more
-// than a.halo cells from its initial cell is left to deposit_spill_kernel,
-// appended to the spill list by its home tile (the tile owning its initial
-// cell).
+// interpolate from are staged in LDS first.  Work is split by initial cell:
+// a thread takes the fineness^2 fine particles that start in one cell and
+// sums in registers the contributions of those that land in the same cell
+// as the first, so one set of 36 LDS atomics serves them all.  A particle
+// displaced by more than a.halo cells from its initial cell is appended to
+// the spill list by its home tile (the tile owning its initial cell); the
+// last block to finish deposits the list with global atomics and resets it.
 constexpr int kFineMax = 128;    // fine indices per side: fineness (kTile + 2 halo + 2)
 constexpr int kCoarseMax = 18;   // coarse particles per side staged in LDS
+constexpr int kExtMax = 48;      // initial cells per side: kTile + 2 halo + 2
 
 struct FineTab {                 // per fine index along one axis
     double ip[kFineMax], in[kFineMax], fg[kFineMax];
     int P[kFineMax], Nn[kFineMax], cell[kFineMax];
+    int start[kExtMax + 1];      // first local fine index of each initial cell
 };
+
+// First fine index whose initial cell is >= c, found by one wave (every
+// lane calls it): 64 samples narrow it to a stride, 64 more pin it down.
+__device__ int wave_first_fine(const double* fg, int nf, int c, int N,
+                               double h) {
+    const int lane = threadIdx.x & 63;
+    if (nf > 64 * 64) return first_fine_at_or_after(fg, nf, c, N, h);
+    const int stride = (nf + 63) / 64;
+    const int i1 = lane * stride;
+    const bool below1 = i1 < nf && cell_of(fg[i1], N, h) < c;
+    const int n1 = __popcll(__ballot(below1));
+    if (n1 == 0) return 0;
+    const int base = (n1 - 1) * stride;
+    const int i2 = base + lane;
+    const bool below2 = lane < stride && i2 < nf && cell_of(fg[i2], N, h) < c;
+    return base + __popcll(__ballot(below2));
+}
+
+// Adds v[c] * w[k] (or val[c][k]) to the stencil points of cell (i, j) that
+// lie in this block's tile.
+__device__ __forceinline__ void lds_deposit(double (*acc)[kTile][kTile],
+                                            int i, int j, int ci0, int cj0,
+                                            const double* v, const double* w,
+                                            const double (*val)[9]) {
+    const bool inside = i - 1 >= ci0 && i + 1 < ci0 + kTile &&
+                        j - 1 >= cj0 && j + 1 < cj0 + kTile;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int si = i + stencil_di(k) - ci0;
+        const int sj = j + stencil_dj(k) - cj0;
+        if (inside || (si >= 0 && si < kTile && sj >= 0 && sj < kTile)) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                atomicAdd(&acc[c][si][sj], val ? val[c][k] : v[c] * w[k]);
+        }
+    }
+}
+
+__device__ __forceinline__ void deposit_spill_one(const DepositArgs& a, int fk,
+                                                  int N) {
+    FineParticle fp;
+    fine_particle(a, fk / a.nf, fk % a.nf, fp);
+    Tsc t;
+    tsc_weights(fp.x, fp.y, N, a.h, t, nullptr, 1. / a.h);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const size_t idx = (size_t)(t.i + stencil_di(k)) * N + (t.j + stencil_dj(k));
+        unsafeAtomicAdd(a.ro + idx, fp.v[0] * t.w[k]);
+        unsafeAtomicAdd(a.jx + idx, fp.v[1] * t.w[k]);
+        unsafeAtomicAdd(a.jy + idx, fp.v[2] * t.w[k]);
+        unsafeAtomicAdd(a.jz + idx, fp.v[3] * t.w[k]);
+    }
+}
 
 template <bool STAGED>
 __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
@@ -195,6 +253,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     __shared__ FineTab ti, tj;
     __shared__ double cs[7][kCoarseMax * kCoarseMax];
     __shared__ int range[4];
+    __shared__ int is_last;
     const int N = a.N;
     const double inv_h = 1. / a.h;
     const int ci0 = blockIdx.y * kTile, cj0 = blockIdx.x * kTile;
@@ -202,108 +261,135 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     const int ci_lo = ci0 - 1 - a.halo, cj_lo = cj0 - 1 - a.halo;
     for (int k = threadIdx.x; k < 4 * kTile * kTile; k += blockDim.x)
         (&acc[0][0][0])[k] = 0.;
-    if (threadIdx.x < 4) {
-        const int c = (threadIdx.x & 1) ? ext : 0;
-        const int lo = (threadIdx.x < 2) ? ci_lo : cj_lo;
-        range[threadIdx.x] = first_fine_at_or_after(a.fg, a.nf, lo + c, N, a.h);
+    {
+        const int wave = threadIdx.x >> 6;
+        if (wave < 4) {
+            const int lo = (wave < 2) ? ci_lo : cj_lo;
+            const int r = wave_first_fine(a.fg, a.nf, lo + ((wave & 1) ? ext : 0), N, a.h);
+            if ((threadIdx.x & 63) == 0) range[wave] = r;
+        }
     }
     __syncthreads();
     const int fi_a = range[0], ni = range[1] - range[0];
     const int fj_a = range[2], nj = range[3] - range[2];
-    if (ni > kFineMax || nj > kFineMax) {
+    const bool overfull = ni > kFineMax || nj > kFineMax || ext > kExtMax;
+    if (overfull) {
         // denser fine grid than the host-checked fineness allows: flag the
         // tile as faulted and fill it with NaN rather than drop particles
         if (threadIdx.x == 0) atomicAdd(a.fault, 1);
-        for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
-            const int i = ci0 + k / kTile, j = cj0 + k % kTile;
-            if (i >= N || j >= N) continue;
-            const size_t idx = (size_t)i * N + j;
-            a.ro[idx] = a.jx[idx] = a.jy[idx] = a.jz[idx] = __builtin_nan("");
-        }
-        return;
-    }
-    for (int k = threadIdx.x; k < ni + nj; k += blockDim.x) {
-        const bool isj = k >= ni;
-        const int f = isj ? fj_a + k - ni : fi_a + k;
-        FineTab& t = isj ? tj : ti;
-        const int s = isj ? k - ni : k;
-        t.ip[s] = a.ip[f];
-        t.in[s] = a.in[f];
-        t.fg[s] = a.fg[f];
-        t.P[s] = clampi((int)a.P[f], a.nc);
-        t.Nn[s] = clampi((int)a.Nn[f], a.nc);
-        t.cell[s] = cell_of(a.fg[f], N, a.h);
-    }
-    __syncthreads();
-    // coarse rectangle referenced by this block's fine particles
-    int cI0 = 0, cJ0 = 0, cW = 0;
-    if constexpr (STAGED) {
-        // indices_prev/next grow with the fine index, so the rectangle is
-        // [P[first], Nn[last]] on each axis; the host checked that it fits
-        cI0 = ni ? ti.P[0] : 0;
-        cJ0 = nj ? tj.P[0] : 0;
-        int cH = ni ? ti.Nn[ni - 1] - cI0 + 1 : 0;
-        cW = nj ? tj.Nn[nj - 1] - cJ0 + 1 : 0;
-        if (cH > kCoarseMax || cW > kCoarseMax || cH < 0 || cW < 0) {
-            if (threadIdx.x == 0) atomicAdd(a.fault, 1);
-            cH = cH < 0 ? 0 : (cH > kCoarseMax ? kCoarseMax : cH);
-            cW = cW < 0 ? 0 : (cW > kCoarseMax ? kCoarseMax : cW);
-        }
-        const double* src[7] = {a.x_offt, a.y_offt, a.m, a.q, a.px, a.py, a.pz};
-        for (int k = threadIdx.x; k < cH * cW; k += blockDim.x) {
-            const size_t g = (size_t)(cI0 + k / cW) * a.nc + cJ0 + k % cW;
-#pragma unroll
-            for (int c = 0; c < 7; ++c) cs[c][k] = src[c][g];
+    } else {
+        for (int k = threadIdx.x; k < ni + nj; k += blockDim.x) {
+            const bool isj = k >= ni;
+            const int f = isj ? fj_a + k - ni : fi_a + k;
+            FineTab& t = isj ? tj : ti;
+            const int s = isj ? k - ni : k;
+            t.ip[s] = a.ip[f];
+            t.in[s] = a.in[f];
+            t.fg[s] = a.fg[f];
+            t.P[s] = clampi((int)a.P[f], a.nc);
+            t.Nn[s] = clampi((int)a.Nn[f], a.nc);
+            t.cell[s] = cell_of(a.fg[f], N, a.h);
         }
         __syncthreads();
-    }
-    const double sm = a.smallness;
-    for (int p = threadIdx.x; p < ni * nj; p += blockDim.x) {
-        const int li = p / nj, lj = p - li * nj;
-        const int ci = ti.cell[li], cj = tj.cell[lj];   // initial cell
-        const bool home = ci >= ci0 && ci < ci0 + kTile &&
-                          cj >= cj0 && cj < cj0 + kTile;
-        const double A = ti.ip[li] * tj.ip[lj], B = ti.ip[li] * tj.in[lj];
-        const double C = ti.in[li] * tj.ip[lj], D = ti.in[li] * tj.in[lj];
-        double x, y, v[4];
-        if constexpr (STAGED) {
-            const int last = kCoarseMax * kCoarseMax - 1;
-            const int pp = min(max((ti.P[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
-            const int pn = min(max((ti.P[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
-            const int np = min(max((ti.Nn[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
-            const int nn = min(max((ti.Nn[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
-#define LCODE_MIX(c) (A * cs[c][pp] + B * cs[c][pn] + C * cs[c][np] + D * cs[c][nn])
-            x = ti.fg[li] + LCODE_MIX(0);
-            y = tj.fg[lj] + LCODE_MIX(1);
-            deposit_values(sm * LCODE_MIX(2), sm * LCODE_MIX(3), sm * LCODE_MIX(4),
-                           sm * LCODE_MIX(5), sm * LCODE_MIX(6), v);
-#undef LCODE_MIX
-        } else {
-            const int nc = a.nc;
-            const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
-            const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
-#define LCODE_MIX(arr) (A * arr[pp] + B * arr[pn] + C * arr[np] + D * arr[nn])
-            x = ti.fg[li] + LCODE_MIX(a.x_offt);
-            y = tj.fg[lj] + LCODE_MIX(a.y_offt);
-            deposit_values(sm * LCODE_MIX(a.m), sm * LCODE_MIX(a.q),
-                           sm * LCODE_MIX(a.px), sm * LCODE_MIX(a.py),
-                           sm * LCODE_MIX(a.pz), v);
-#undef LCODE_MIX
-        }
-        Tsc t;
-        tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
-        if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
-            if (home) a.spill[atomicAdd(a.spill_count, 1)] = (fi_a + li) * a.nf + fj_a + lj;
-            continue;
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const int si = t.i + stencil_di(k) - ci0;
-            const int sj = t.j + stencil_dj(k) - cj0;
-            if (si >= 0 && si < kTile && sj >= 0 && sj < kTile) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) atomicAdd(&acc[c][si][sj], v[c] * t.w[k]);
+        // first local fine index of each initial cell (cell[] is sorted)
+        for (int k = threadIdx.x; k < 2 * (ext + 1); k += blockDim.x) {
+            const bool isj = k > ext;
+            FineTab& t = isj ? tj : ti;
+            const int r = isj ? k - ext - 1 : k;
+            const int c = (isj ? cj_lo : ci_lo) + r, n = isj ? nj : ni;
+            int lo = 0, hi = n;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (t.cell[mid] < c) lo = mid + 1; else hi = mid;
             }
+            t.start[r] = lo;
+        }
+        // coarse rectangle referenced by this block's fine particles
+        int cI0 = 0, cJ0 = 0, cW = 0;
+        if constexpr (STAGED) {
+            // indices_prev/next grow with the fine index, so the rectangle
+            // is [P[first], Nn[last]] on each axis; the host checked the fit
+            cI0 = ni ? ti.P[0] : 0;
+            cJ0 = nj ? tj.P[0] : 0;
+            int cH = ni ? ti.Nn[ni - 1] - cI0 + 1 : 0;
+            cW = nj ? tj.Nn[nj - 1] - cJ0 + 1 : 0;
+            if (cH > kCoarseMax || cW > kCoarseMax || cH < 0 || cW < 0) {
+                if (threadIdx.x == 0) atomicAdd(a.fault, 1);
+                cH = cH < 0 ? 0 : (cH > kCoarseMax ? kCoarseMax : cH);
+                cW = cW < 0 ? 0 : (cW > kCoarseMax ? kCoarseMax : cW);
+            }
+            const double* src[7] = {a.x_offt, a.y_offt, a.m, a.q, a.px, a.py, a.pz};
+            for (int k = threadIdx.x; k < cH * cW; k += blockDim.x) {
+                const size_t g = (size_t)(cI0 + k / cW) * a.nc + cJ0 + k % cW;
+#pragma unroll
+                for (int c = 0; c < 7; ++c) cs[c][k] = src[c][g];
+            }
+        }
+        __syncthreads();
+        const double sm = a.smallness;
+        for (int task = threadIdx.x; task < ext * ext; task += blockDim.x) {
+            const int r = task / ext, c = task - r * ext;
+            const int ci = ci_lo + r, cj = cj_lo + c;   // initial cell
+            const bool home = ci >= ci0 && ci < ci0 + kTile &&
+                              cj >= cj0 && cj < cj0 + kTile;
+            double sum[4][9];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int k = 0; k < 9; ++k) sum[q][k] = 0.;
+            int i0 = -1, j0 = -1;
+            for (int li = ti.start[r]; li < ti.start[r + 1]; ++li) {
+                for (int lj = tj.start[c]; lj < tj.start[c + 1]; ++lj) {
+                    const double A = ti.ip[li] * tj.ip[lj], B = ti.ip[li] * tj.in[lj];
+                    const double C = ti.in[li] * tj.ip[lj], D = ti.in[li] * tj.in[lj];
+                    double x, y, v[4];
+                    if constexpr (STAGED) {
+                        const int last = kCoarseMax * kCoarseMax - 1;
+                        const int pp = min(max((ti.P[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
+                        const int pn = min(max((ti.P[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
+                        const int np = min(max((ti.Nn[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
+                        const int nn = min(max((ti.Nn[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
+#define LCODE_MIX(c) (A * cs[c][pp] + B * cs[c][pn] + C * cs[c][np] + D * cs[c][nn])
+                        x = ti.fg[li] + LCODE_MIX(0);
+                        y = tj.fg[lj] + LCODE_MIX(1);
+                        deposit_values(sm * LCODE_MIX(2), sm * LCODE_MIX(3), sm * LCODE_MIX(4),
+                                       sm * LCODE_MIX(5), sm * LCODE_MIX(6), v);
+#undef LCODE_MIX
+                    } else {
+                        const int nc = a.nc;
+                        const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
+                        const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
+#define LCODE_MIX(arr) (A * arr[pp] + B * arr[pn] + C * arr[np] + D * arr[nn])
+                        x = ti.fg[li] + LCODE_MIX(a.x_offt);
+                        y = tj.fg[lj] + LCODE_MIX(a.y_offt);
+                        deposit_values(sm * LCODE_MIX(a.m), sm * LCODE_MIX(a.q),
+                                       sm * LCODE_MIX(a.px), sm * LCODE_MIX(a.py),
+                                       sm * LCODE_MIX(a.pz), v);
+#undef LCODE_MIX
+                    }
+                    Tsc t;
+                    tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
+                    if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
+                        if (home)
+                            a.spill[atomicAdd(a.spill_count, 1)] =
+                                (fi_a + li) * a.nf + fj_a + lj;
+                        continue;
+                    }
+                    if (i0 < 0) {
+                        i0 = t.i;
+                        j0 = t.j;
+                    }
+                    if (t.i == i0 && t.j == j0) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+#pragma unroll
+                            for (int k = 0; k < 9; ++k) sum[q][k] += v[q] * t.w[k];
+                    } else {
+                        lds_deposit(acc, t.i, t.j, ci0, cj0, v, t.w, nullptr);
+                    }
+                }
+            }
+            if (i0 >= 0) lds_deposit(acc, i0, j0, ci0, cj0, nullptr, nullptr, sum);
         }
     }
     __syncthreads();
@@ -312,12 +398,35 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         const int i = ci0 + r, j = cj0 + c;
         if (i >= N || j >= N) continue;
         const size_t idx = (size_t)i * N + j;
+        if (overfull) {
+            a.ro[idx] = a.jx[idx] = a.jy[idx] = a.jz[idx] = __builtin_nan("");
+            continue;
+        }
         double ro = acc[0][r][c];
         if (a.ro_initial) ro += a.ro_initial[idx];
         a.ro[idx] = ro;
         a.jx[idx] = acc[1][r][c];
         a.jy[idx] = acc[2][r][c];
         a.jz[idx] = acc[3][r][c];
+    }
+    // the last block to finish deposits the spilled particles (every other
+    // block's stores and appends are visible after its release fence)
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int total = gridDim.x * gridDim.y;
+        is_last = atomicAdd(a.spill_done, 1) == total - 1;
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    const int count = atomicAdd(a.spill_count, 0);
+    for (int p = threadIdx.x; p < count; p += blockDim.x)
+        deposit_spill_one(a, a.spill[p], N);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicExch(a.spill_count, 0);
+        atomicExch(a.spill_done, 0);
     }
 }
 
@@ -326,41 +435,6 @@ __global__ void __launch_bounds__(kDepBlock)
 deposit_tile_kernel(DepositArgs a) {
     deposit_tile_body<STAGED>(a);
 }
-
-// Atomic deposition of the spilled particles; the last block to finish
-// resets the list for the next call (every block has read the count by
-// then).
-__global__ void __launch_bounds__(kBlock)
-deposit_spill_kernel(DepositArgs a) {
-    const int count = atomicAdd(a.spill_count, 0);
-    const int N = a.N;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count;
-         p += gridDim.x * blockDim.x) {
-        const int fk = a.spill[p];
-        FineParticle fp;
-        fine_particle(a, fk / a.nf, fk % a.nf, fp);
-        Tsc t;
-        tsc_weights(fp.x, fp.y, N, a.h, t, nullptr, 1. / a.h);
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            const size_t idx = (size_t)(t.i + stencil_di(k)) * N + (t.j + stencil_dj(k));
-            unsafeAtomicAdd(a.ro + idx, fp.v[0] * t.w[k]);
-            unsafeAtomicAdd(a.jx + idx, fp.v[1] * t.w[k]);
-            unsafeAtomicAdd(a.jy + idx, fp.v[2] * t.w[k]);
-            unsafeAtomicAdd(a.jz + idx, fp.v[3] * t.w[k]);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(a.spill_done, 1) == (int)gridDim.x - 1) {
-            atomicExch(a.spill_count, 0);
-            atomicExch(a.spill_done, 0);
-        }
-    }
-}
-
-constexpr int kSpillBlocks = 64;
 
 inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -391,11 +465,6 @@ int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof) {
             deposit_tile_kernel<true><<<grid, kDepBlock, 0, s>>>(a);
         else
             deposit_tile_kernel<false><<<grid, kDepBlock, 0, s>>>(a);
-    }
-    LCODE_HIP(hipGetLastError());
-    {
-        ProfScope ps(prof, "deposit_spill", s);
-        deposit_spill_kernel<<<kSpillBlocks, kBlock, 0, s>>>(a);
     }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
