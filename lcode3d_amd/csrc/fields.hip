@@ -362,6 +362,25 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// XCD-aware block order.  Workgroups go to the 8 XCDs round-robin by
+// linear id and each XCD has its own L2; neighbouring line blocks touch the
+// same 128-B lines of the transposed spectra, so give every XCD a
+// contiguous range of (field, line block) pairs.  A bijection on the grid.
+struct BlockPos {
+    int field, block;
+};
+
+__device__ __forceinline__ BlockPos xcd_block() {
+    constexpr int kXcd = 8;
+    const int gx = gridDim.x, total = gx * gridDim.y;
+    const int lin = blockIdx.y * gx + blockIdx.x;
+    const int q = total / kXcd, r = total % kXcd;
+    const int xcd = lin % kXcd, slot = lin / kXcd;
+    const int logical = xcd < r ? xcd * (q + 1) + slot
+                                : r * (q + 1) + (xcd - r) * q + slot;
+    return BlockPos{logical / gx, logical % gx};
+}
+
 // ---------------------------------------------------------------------------
 // Passes
 // ---------------------------------------------------------------------------
@@ -370,14 +389,15 @@ template <int L_, bool TW_LDS>
 __global__ void __launch_bounds__(1024)
 pass_a_kernel(SolverGeom g, SolveBatch b) {
     extern __shared__ double2 lds[];
-    const FieldPlan& fp = b.f[blockIdx.y];
+    const BlockPos bp = xcd_block();
+    const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L, N = g.N;
     const int lines = fp.rowsA;
-    if ((int)blockIdx.x * g.P * 2 >= lines) return;
+    if (bp.block * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     double2* z = lds + (size_t)pair * L;
     const Tw tw = load_twiddles<L_, TW_LDS>(lds, g);
-    const int ra = ((int)blockIdx.x * g.P + pair) * 2, rb = ra + 1;
+    const int ra = (bp.block * g.P + pair) * 2, rb = ra + 1;
     const bool va = ra < lines, vb = rb < lines;
     for (int n = t; n < fp.lenA; n += Tp) {
         const int j = fp.colA0 + n;
@@ -403,14 +423,15 @@ __global__ void __launch_bounds__(1024)
 pass_b_kernel(SolverGeom g, SolveBatch b) {
     extern __shared__ double2 lds[];
     __shared__ double red[2][16];
-    const FieldPlan& fp = b.f[blockIdx.y];
+    const BlockPos bp = xcd_block();
+    const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L;
     const int lines = fp.lenA, len = fp.rowsA, xf = fp.xfB;
-    if ((int)blockIdx.x * g.P * 2 >= lines) return;
+    if (bp.block * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     double2* z = lds + (size_t)pair * L;
     const Tw tw = load_twiddles<L_, TW_LDS>(lds, g);
-    const int la = ((int)blockIdx.x * g.P + pair) * 2, lb = la + 1;
+    const int la = (bp.block * g.P + pair) * 2, lb = la + 1;
     const bool va = la < lines, vb = lb < lines;
     double* rowa = fp.T + (size_t)la * len;
     double* rowb = fp.T + (size_t)lb * len;
@@ -482,13 +503,14 @@ __global__ void __launch_bounds__(1024)
 pass_c_kernel(SolverGeom g, SolveBatch b) {
     extern __shared__ double2 lds[];
     __shared__ double red[16];
-    const FieldPlan& fp = b.f[blockIdx.y];
+    const BlockPos bp = xcd_block();
+    const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L, N = g.N;
-    if ((int)blockIdx.x * g.P * 2 >= N) return;
+    if (bp.block * g.P * 2 >= N) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     double2* z = lds + (size_t)pair * L;
     const Tw tw = load_twiddles<L_, TW_LDS>(lds, g);
-    const int ra = ((int)blockIdx.x * g.P + pair) * 2, rb = ra + 1;
+    const int ra = (bp.block * g.P + pair) * 2, rb = ra + 1;
     const int ca = ra - fp.rowsA0, cb = rb - fp.rowsA0;
     const bool ha = ra < N && ca >= 0 && ca < fp.rowsA;
     const bool hb = rb < N && cb >= 0 && cb < fp.rowsA;

@@ -47,20 +47,26 @@ estimate_kernel(EstimateArgs a) {
 // move_smart_kernel (lcode.py:704-793): midpoint position, 6-field TSC
 // gather, two fixed-point iterations of the half-step momentum, offset and
 // momentum update, reflection with strict > / < flipping the momentum.
+// Four lanes share one particle: each gathers the 6 fields at 2-3 of the 9
+// stencil points and two butterfly shuffles sum them, so the few coarse
+// particles fill four times as many waves and their gathers overlap.
 constexpr int kPushBlock = 64;
+constexpr int kLanesPerParticle = 4;
 
 __global__ void __launch_bounds__(kPushBlock)
 push_kernel(PushArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.n) return;
-    const double m = a.m[k], q = a.q[k];
-    const double opx = a.px[k], opy = a.py[k], opz = a.pz[k];
-    const double xi0 = a.x_init[k], yi0 = a.y_init[k];
-    double x_offt = a.prev_x_offt[k], y_offt = a.prev_y_offt[k];
+    const int lane = threadIdx.x & (kLanesPerParticle - 1);
+    const int k = (blockIdx.x * blockDim.x + threadIdx.x) / kLanesPerParticle;
+    const bool live = k < a.n;
+    const int kk = live ? k : 0;    // dead lanes still join the shuffles
+    const double m = a.m[kk], q = a.q[kk];
+    const double opx = a.px[kk], opy = a.py[kk], opz = a.pz[kk];
+    const double xi0 = a.x_init[kk], yi0 = a.y_init[kk];
+    double x_offt = a.prev_x_offt[kk], y_offt = a.prev_y_offt[kk];
     double ex, ey;
     if (a.est_x) {
-        ex = a.est_x[k];
-        ey = a.est_y[k];
+        ex = a.est_x[kk];
+        ey = a.est_y[kk];
     } else {
         estimate_one(m, xi0, yi0, x_offt, y_offt, opx, opy, opz, a.dxi,
                      a.reflect, ex, ey);
@@ -68,13 +74,25 @@ push_kernel(PushArgs a) {
     const double x_half = xi0 + (x_offt + ex) / 2;
     const double y_half = yi0 + (y_offt + ey) / 2;
     Tsc t;
-    tsc_weights(x_half, y_half, a.N, a.h, t, a.fault);
-    const double Ex = interp9(a.F[0], a.N, t);
-    const double Ey = interp9(a.F[1], a.N, t);
-    const double Ez = interp9(a.F[2], a.N, t);
-    const double Bx = interp9(a.F[3], a.N, t);
-    const double By = interp9(a.F[4], a.N, t);
-    const double Bz = interp9(a.F[5], a.N, t);
+    tsc_weights(x_half, y_half, a.N, a.h, t,
+                (live && lane == 0) ? a.fault : nullptr);
+    double F[6] = {0., 0., 0., 0., 0., 0.};
+    const int N = a.N;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int pt = lane + kLanesPerParticle * s;   // stencil point
+        if (pt < 9) {
+            const size_t idx = (size_t)(t.i + stencil_di(pt)) * N + (t.j + stencil_dj(pt));
+#pragma unroll
+            for (int f = 0; f < 6; ++f) F[f] += a.F[f][idx] * t.w[pt];
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        F[f] += __shfl_xor(F[f], 1, 64);
+        F[f] += __shfl_xor(F[f], 2, 64);
+    }
+    const double Ex = F[0], Ey = F[1], Ez = F[2], Bx = F[3], By = F[4], Bz = F[5];
 
     double px = opx, py = opy, pz = opz;
     double dpx = 0, dpy = 0, dpz = 0;
@@ -104,11 +122,13 @@ push_kernel(PushArgs a) {
     if (y > +r) { y = +2 * r - y; y_offt = y - yi0; py = -py; }
     if (y < -r) { y = -2 * r - y; y_offt = y - yi0; py = -py; }
 
-    a.out_x[k] = x_offt;
-    a.out_y[k] = y_offt;
-    a.out_px[k] = px;
-    a.out_py[k] = py;
-    a.out_pz[k] = pz;
+    if (live && lane == 0) {
+        a.out_x[k] = x_offt;
+        a.out_y[k] = y_offt;
+        a.out_px[k] = px;
+        a.out_py[k] = py;
+        a.out_pz[k] = pz;
+    }
 }
 
 // Deposition values of a fine particle with mass m, charge q, momentum p
@@ -157,7 +177,7 @@ __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
 // and deposits, in LDS, every fine particle whose cell moved at most
 // a.halo cells from its initial cell and whose stencil touches the tile.
 constexpr int kTile = 32;
-constexpr int kDepBlock = 256;
+constexpr int kDepBlock = 512;
 
 // First fine index whose initial cell is >= c (fg is increasing).
 __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
@@ -177,10 +197,8 @@ __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
 // so the outputs need no zeroing.  A block visits the fine particles whose
 // initial cell lies within halo + 1 cells of its tile; their per-index
 // interpolation data and, when they fit, the coarse particles they
-// interpolate from are staged in LDS first.  Work is split by initial cell:
-// a thread takes the fineness^2 fine particles that start in one cell and
-// sums in registers the contributions of those that land in the same cell
-// as the first, so one set of 36 LDS atomics serves them all.  A particle
+// interpolate from are staged in LDS first; then one thread per fine
+// particle adds its 9 x 4 weighted values to the LDS tile.  A particle
 // displaced by more than a.halo cells from its initial cell is appended to
 // the spill list by its home tile (the tile owning its initial cell); the
 // last block to finish deposits the list with global atomics and resets it.
@@ -191,7 +209,6 @@ constexpr int kExtMax = 48;      // initial cells per side: kTile + 2 halo + 2
 struct FineTab {                 // per fine index along one axis
     double ip[kFineMax], in[kFineMax], fg[kFineMax];
     int P[kFineMax], Nn[kFineMax], cell[kFineMax];
-    int start[kExtMax + 1];      // first local fine index of each initial cell
 };
 
 // First fine index whose initial cell is >= c, found by one wave (every
@@ -291,19 +308,6 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             t.cell[s] = cell_of(a.fg[f], N, a.h);
         }
         __syncthreads();
-        // first local fine index of each initial cell (cell[] is sorted)
-        for (int k = threadIdx.x; k < 2 * (ext + 1); k += blockDim.x) {
-            const bool isj = k > ext;
-            FineTab& t = isj ? tj : ti;
-            const int r = isj ? k - ext - 1 : k;
-            const int c = (isj ? cj_lo : ci_lo) + r, n = isj ? nj : ni;
-            int lo = 0, hi = n;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (t.cell[mid] < c) lo = mid + 1; else hi = mid;
-            }
-            t.start[r] = lo;
-        }
         // coarse rectangle referenced by this block's fine particles
         int cI0 = 0, cJ0 = 0, cW = 0;
         if constexpr (STAGED) {
@@ -327,69 +331,46 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         }
         __syncthreads();
         const double sm = a.smallness;
-        for (int task = threadIdx.x; task < ext * ext; task += blockDim.x) {
-            const int r = task / ext, c = task - r * ext;
-            const int ci = ci_lo + r, cj = cj_lo + c;   // initial cell
+        for (int p = threadIdx.x; p < ni * nj; p += blockDim.x) {
+            const int li = p / nj, lj = p - li * nj;
+            const int ci = ti.cell[li], cj = tj.cell[lj];   // initial cell
             const bool home = ci >= ci0 && ci < ci0 + kTile &&
                               cj >= cj0 && cj < cj0 + kTile;
-            double sum[4][9];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-#pragma unroll
-                for (int k = 0; k < 9; ++k) sum[q][k] = 0.;
-            int i0 = -1, j0 = -1;
-            for (int li = ti.start[r]; li < ti.start[r + 1]; ++li) {
-                for (int lj = tj.start[c]; lj < tj.start[c + 1]; ++lj) {
-                    const double A = ti.ip[li] * tj.ip[lj], B = ti.ip[li] * tj.in[lj];
-                    const double C = ti.in[li] * tj.ip[lj], D = ti.in[li] * tj.in[lj];
-                    double x, y, v[4];
-                    if constexpr (STAGED) {
-                        const int last = kCoarseMax * kCoarseMax - 1;
-                        const int pp = min(max((ti.P[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
-                        const int pn = min(max((ti.P[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
-                        const int np = min(max((ti.Nn[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
-                        const int nn = min(max((ti.Nn[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
+            const double A = ti.ip[li] * tj.ip[lj], B = ti.ip[li] * tj.in[lj];
+            const double C = ti.in[li] * tj.ip[lj], D = ti.in[li] * tj.in[lj];
+            double x, y, v[4];
+            if constexpr (STAGED) {
+                const int last = kCoarseMax * kCoarseMax - 1;
+                const int pp = min(max((ti.P[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
+                const int pn = min(max((ti.P[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
+                const int np = min(max((ti.Nn[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
+                const int nn = min(max((ti.Nn[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
 #define LCODE_MIX(c) (A * cs[c][pp] + B * cs[c][pn] + C * cs[c][np] + D * cs[c][nn])
-                        x = ti.fg[li] + LCODE_MIX(0);
-                        y = tj.fg[lj] + LCODE_MIX(1);
-                        deposit_values(sm * LCODE_MIX(2), sm * LCODE_MIX(3), sm * LCODE_MIX(4),
-                                       sm * LCODE_MIX(5), sm * LCODE_MIX(6), v);
+                x = ti.fg[li] + LCODE_MIX(0);
+                y = tj.fg[lj] + LCODE_MIX(1);
+                deposit_values(sm * LCODE_MIX(2), sm * LCODE_MIX(3), sm * LCODE_MIX(4),
+                               sm * LCODE_MIX(5), sm * LCODE_MIX(6), v);
 #undef LCODE_MIX
-                    } else {
-                        const int nc = a.nc;
-                        const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
-                        const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
+            } else {
+                const int nc = a.nc;
+                const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
+                const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
 #define LCODE_MIX(arr) (A * arr[pp] + B * arr[pn] + C * arr[np] + D * arr[nn])
-                        x = ti.fg[li] + LCODE_MIX(a.x_offt);
-                        y = tj.fg[lj] + LCODE_MIX(a.y_offt);
-                        deposit_values(sm * LCODE_MIX(a.m), sm * LCODE_MIX(a.q),
-                                       sm * LCODE_MIX(a.px), sm * LCODE_MIX(a.py),
-                                       sm * LCODE_MIX(a.pz), v);
+                x = ti.fg[li] + LCODE_MIX(a.x_offt);
+                y = tj.fg[lj] + LCODE_MIX(a.y_offt);
+                deposit_values(sm * LCODE_MIX(a.m), sm * LCODE_MIX(a.q),
+                               sm * LCODE_MIX(a.px), sm * LCODE_MIX(a.py),
+                               sm * LCODE_MIX(a.pz), v);
 #undef LCODE_MIX
-                    }
-                    Tsc t;
-                    tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
-                    if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
-                        if (home)
-                            a.spill[atomicAdd(a.spill_count, 1)] =
-                                (fi_a + li) * a.nf + fj_a + lj;
-                        continue;
-                    }
-                    if (i0 < 0) {
-                        i0 = t.i;
-                        j0 = t.j;
-                    }
-                    if (t.i == i0 && t.j == j0) {
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-#pragma unroll
-                            for (int k = 0; k < 9; ++k) sum[q][k] += v[q] * t.w[k];
-                    } else {
-                        lds_deposit(acc, t.i, t.j, ci0, cj0, v, t.w, nullptr);
-                    }
-                }
             }
-            if (i0 >= 0) lds_deposit(acc, i0, j0, ci0, cj0, nullptr, nullptr, sum);
+            Tsc t;
+            tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
+            if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
+                if (home)
+                    a.spill[atomicAdd(a.spill_count, 1)] = (fi_a + li) * a.nf + fj_a + lj;
+                continue;
+            }
+            lds_deposit(acc, t.i, t.j, ci0, cj0, v, t.w, nullptr);
         }
     }
     __syncthreads();
@@ -451,8 +432,8 @@ int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof) {
 int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof) {
     if (a.n == 0) return LCODE_OK;
     ProfScope ps(prof, "push", s);
-    // one wave per block spreads the (few) coarse particles over all CUs
-    push_kernel<<<(a.n + kPushBlock - 1) / kPushBlock, kPushBlock, 0, s>>>(a);
+    const long lanes = (long)a.n * kLanesPerParticle;
+    push_kernel<<<(int)((lanes + kPushBlock - 1) / kPushBlock), kPushBlock, 0, s>>>(a);
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
