@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
+                    help="field-solve transform precision (fp32: mixed)")
     ap.add_argument("--settle", type=int, default=100,
                     help="untimed slices run before warm-up (plasma develops)")
     ap.add_argument("--profile-steps", type=int, default=20,
@@ -122,6 +124,7 @@ def main():
 
     torch.cuda.set_device(0)
     cfg = make_config()
+    cfg.field_solver_precision = args.precision
     xs, ys, const, vp, state = lc.init(cfg)
     nc = const.m.shape[0]
     n = cfg.grid_steps
@@ -194,7 +197,9 @@ def main():
         log(f"cpu baseline {cpu['value']:.3f} xi-steps/s on {cpu['cores']} cores")
 
     out = {
-        "metric": "xi-steps/s at 641^2 grid (plain FP64 HIP step)",
+        "metric": "xi-steps/s at 641^2 grid (plain FP64 HIP step)"
+                  if args.precision == "fp64" else
+                  "xi-steps/s at 641^2 grid (FP32-transform mixed HIP step)",
         "value": steps_per_s,
         "unit": "xi-steps/s",
         "n_gpus": 1,
@@ -204,11 +209,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64+f32 transforms",
         "data": "synthetic cosine-Gaussian beam (amplitude 0.05, sigma 1), "
                 f"slices {args.settle}+ of the bunch",
         "config": {"workload": "lcode3d xi step, 641^2 grid, coarseness 3, "
-                               "fineness 2, plain FP64",
+                               f"fineness 2, {'plain FP64' if args.precision == 'fp64' else 'FP32 transforms'}",
                    "grid_steps": n, "coarse_particles": nc * nc,
                    "fine_particles": int(vp.fine_grid.numel()) ** 2},
         "roofline": roofline,

@@ -50,7 +50,11 @@ enum {
 
 /* Transform back-end of the field solves. */
 enum {
-    LCODE_SOLVER_FFT64 = 0  /* hand-written mixed-radix FFT in LDS, FP64 */
+    LCODE_SOLVER_FFT64 = 0, /* hand-written mixed-radix FFT in LDS, FP64 */
+    LCODE_SOLVER_FFT32 = 1  /* the same in FP32 (mixed precision): the
+                               transforms, spectra and eigenvalue tables;
+                               RHS, Bz mean and zero mode, Variant A and
+                               averages stay FP64 */
 };
 
 /* The config attributes the step reads (config_example.py:1-17), with

@@ -19,6 +19,7 @@ LIB_PATH = os.environ.get("LCODE3D_HIP_LIB") or os.path.join(HERE, LIB_NAME)
 
 LCODE_OK = 0
 LCODE_SOLVER_FFT64 = 0
+LCODE_SOLVER_FFT32 = 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)

@@ -30,12 +30,15 @@ struct lcode_workspace {
     SolverGeom geom{};
     double* costab = nullptr;      // quarter-wave cosine table
     double2* stage_tw = nullptr;   // static-plan stage twiddles
-    double* tab_mixed = nullptr;   // [M][N]   (DST index, DCT index)
-    double* tab_mixed_t = nullptr; // [N][M]
-    double* tab_dir = nullptr;     // [M][M]
-    double* tab_neu = nullptr;     // [N][N]
+    float* costab32 = nullptr;
+    float2* stage_tw32 = nullptr;
+    int fp32 = 0;                  // LCODE_SOLVER_FFT32
+    void* tab_mixed = nullptr;     // [M][N]   (DST index, DCT index)
+    void* tab_mixed_t = nullptr;   // [N][M]
+    void* tab_dir = nullptr;       // [M][M]
+    void* tab_neu = nullptr;       // [N][N]
     double* colsum = nullptr;      // [N]
-    double* T = nullptr;           // 6 * N * N transform scratch
+    void* T = nullptr;             // 6 * N * N transform scratch (S)
     double* partial = nullptr;     // [N]
     double* p1[5] = {};            // predictor particles (nc * nc)
     double* p2[5] = {};            // corrector particles
@@ -67,6 +70,13 @@ int upload(lcode_workspace* ws, T** dst, const std::vector<T>& src) {
     LCODE_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T),
                         hipMemcpyHostToDevice));
     return LCODE_OK;
+}
+
+// Eigenvalue table in the workspace's transform precision.
+int upload_table(lcode_workspace* ws, void** dst, const std::vector<double>& v) {
+    if (!ws->fp32) return upload(ws, reinterpret_cast<double**>(dst), v);
+    std::vector<float> f(v.begin(), v.end());
+    return upload(ws, reinterpret_cast<float**>(dst), f);
 }
 
 bool factor_radices(int L, FftPlan& plan) {
@@ -102,7 +112,8 @@ int check_cfg(const lcode_config* c) {
                   "plasma_coarseness and plasma_fineness must be positive");
     LCODE_REQUIRE(c->plasma_fineness <= 3,
                   "plasma_fineness > 3 is not supported (deposition tile tables)");
-    LCODE_REQUIRE(c->solver == LCODE_SOLVER_FFT64, "unknown solver");
+    LCODE_REQUIRE(c->solver == LCODE_SOLVER_FFT64 || c->solver == LCODE_SOLVER_FFT32,
+                  "unknown solver");
     return LCODE_OK;
 }
 
@@ -113,6 +124,8 @@ int check_ws(const lcode_workspace* ws, const lcode_config* c) {
     LCODE_REQUIRE(ws->N == c->grid_steps && ws->h == c->grid_step_size &&
                       ws->trick_on == (c->subtraction_trick != 0),
                   "workspace was built for another grid/trick");
+    LCODE_REQUIRE(ws->fp32 == (c->solver == LCODE_SOLVER_FFT32),
+                  "workspace was built for another solver precision");
     return LCODE_OK;
 }
 
@@ -122,7 +135,8 @@ FieldPlan plan_for(lcode_workspace* ws, int kind, int slot) {
     const int N = ws->N, M = ws->M;
     FieldPlan p{};
     p.kind = kind;
-    p.T = ws->T + (size_t)slot * N * N;
+    p.T = static_cast<char*>(ws->T) +
+          (size_t)slot * N * N * (ws->fp32 ? sizeof(float) : sizeof(double));
     switch (kind) {
         case F_EY:
         case F_BX:  // DST along x (rows 1..N-2), DCT along y (lcode.py:275-286)
@@ -193,6 +207,9 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     // FFT geometry: one complex FFT of L = 2 (N - 1) points per line pair.
     SolverGeom& g = ws->geom;
     g.N = N;
+    ws->fp32 = cfg->solver == LCODE_SOLVER_FFT32;
+    g.fp32 = ws->fp32;
+    const int cplx = ws->fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
     const int L = 2 * (N - 1);
     if (!factor_radices(L, g.fft)) {
         set_error("2 (grid_steps - 1) = " + std::to_string(L) +
@@ -204,19 +221,19 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     const int max_lds = 65536;
     auto pairs_for = [&](int fixed_bytes) {
         int P = 1024 / g.Tp;
-        const int by_lds = (max_lds - fixed_bytes) / (int)(L * sizeof(double2));
+        const int by_lds = (max_lds - fixed_bytes) / (L * cplx);
         if (by_lds < P) P = by_lds;
         return P > 4 ? 4 : P;
     };
     int P;
     if (g.static_L) {
         // stage twiddles in LDS when that still leaves two line pairs
-        P = pairs_for((L - 1) * (int)sizeof(double2));
+        P = pairs_for((L - 1) * cplx);
         g.tw_lds = P >= 2;
         if (!g.tw_lds) P = pairs_for(0);
     } else {
         g.tw_lds = 0;
-        P = pairs_for((L / 4 + 1) * (int)sizeof(double));
+        P = pairs_for((L / 4 + 1) * cplx / 2);
     }
     if (P < 1) {
         set_error("grid too large for one FFT line pair in 64 KiB of LDS");
@@ -245,6 +262,13 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         }
         if ((rc = upload(ws, &ws->stage_tw, tw))) return fail(rc);
         g.stage_tw = ws->stage_tw;
+        if (ws->fp32) {
+            std::vector<float2> tw32(tw.size());
+            for (size_t k = 0; k < tw.size(); ++k)
+                tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
+            if ((rc = upload(ws, &ws->stage_tw32, tw32))) return fail(rc);
+            g.stage_tw32 = ws->stage_tw32;
+        }
     }
 
     std::vector<double> costab(L / 4 + 1);
@@ -254,6 +278,11 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     }
     if ((rc = upload(ws, &ws->costab, costab))) return fail(rc);
     g.costab = ws->costab;
+    if (ws->fp32) {
+        std::vector<float> c32(costab.begin(), costab.end());
+        if ((rc = upload(ws, &ws->costab32, c32))) return fail(rc);
+        g.costab32 = ws->costab32;
+    }
 
     // Eigenvalue tables, normalisation 1 / (2 (N - 1))^2 folded in
     // (lcode.py:144-157, :215-231, :316-330).
@@ -286,14 +315,15 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         const long double c = (k == 0 || k == N - 1) ? 1 : 2;
         colsum[k] = (double)(c * s);
     }
-    if ((rc = upload(ws, &ws->tab_mixed, mixed))) return fail(rc);
-    if ((rc = upload(ws, &ws->tab_mixed_t, mixed_t))) return fail(rc);
-    if ((rc = upload(ws, &ws->tab_dir, dir))) return fail(rc);
-    if ((rc = upload(ws, &ws->tab_neu, neu))) return fail(rc);
+    if ((rc = upload_table(ws, &ws->tab_mixed, mixed))) return fail(rc);
+    if ((rc = upload_table(ws, &ws->tab_mixed_t, mixed_t))) return fail(rc);
+    if ((rc = upload_table(ws, &ws->tab_dir, dir))) return fail(rc);
+    if ((rc = upload_table(ws, &ws->tab_neu, neu))) return fail(rc);
     if ((rc = upload(ws, &ws->colsum, colsum))) return fail(rc);
 
     const size_t n2 = (size_t)N * N, np = (size_t)cfg->nc * cfg->nc;
-    if ((rc = dev_alloc(ws, (void**)&ws->T, 6 * n2 * sizeof(double)))) return fail(rc);
+    if ((rc = dev_alloc(ws, &ws->T, 6 * n2 * (ws->fp32 ? sizeof(float) : sizeof(double)))))
+        return fail(rc);
     if ((rc = dev_alloc(ws, (void**)&ws->partial, N * sizeof(double)))) return fail(rc);
     for (int k = 0; k < 5; ++k) {
         if ((rc = dev_alloc(ws, (void**)&ws->p1[k], (np ? np : 1) * sizeof(double)))) return fail(rc);

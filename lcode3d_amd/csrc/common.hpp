@@ -200,8 +200,8 @@ struct FieldPlan {
     int rowsA0, rowsA;        // rows i of the RHS transformed along j
     int xfA, lenA, colA0;     // transform along j: type, length, first j
     int xfB;                  // transform along i (length rowsA)
-    const double* table;      // [lenA][rowsA] spectral multiplier
-    double* T;                // [lenA][rowsA] scratch
+    const void* table;        // [lenA][rowsA] spectral multiplier (S)
+    void* T;                  // [lenA][rowsA] scratch spectra (S)
     const double* colsum;     // Bz only: column sums of the last DCT
     double* partial;          // Bz only: [lenA] mean partials
     const double* prevF;      // Variant A / averaging source (or NULL)
@@ -221,6 +221,9 @@ struct SolverGeom {
     FftPlan fft;
     const double* costab;     // cos(2 pi r / L), r in [0, L / 4]
     const double2* stage_tw;  // per-stage twiddles of the static plan
+    const float* costab32;    // the same tables in FP32
+    const float2* stage_tw32;
+    int fp32;                 // transforms, spectra and tables in FP32
     int static_L;             // L with a compile-time plan, else 0
     int tw_lds;               // static plan: twiddles copied to LDS
     int P, Tp;                // pairs of lines per block, threads per pair

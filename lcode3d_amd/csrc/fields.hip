@@ -1,4 +1,4 @@
-// Field solves of the xi step (lcode.py:122-364) on gfx950, FP64.
+// Field solves of the xi step (lcode.py:122-364) on gfx950.
 //
 // Every solve is out = T_x T_y (Mat o (T_y T_x rhs)) with T_x, T_y type-1
 // sine or cosine transforms along x (i) and y (j).  The transforms commute,
@@ -16,6 +16,10 @@
 // two real lines ride in one complex FFT (real and imaginary parts).  All
 // six fields of a step share L = 2 (N - 1).  The FFT is a mixed-radix
 // (4, 2, 3, 5) Stockham transform in LDS.
+//
+// The transforms, the spectra T and the eigenvalue tables are in S = double
+// (plain path) or S = float (mixed-precision path); the RHS assembly, the
+// Bz mean and the epilogue are FP64 in both.
 
 #include "common.hpp"
 
@@ -23,35 +27,44 @@ namespace lcode {
 
 namespace {
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <typename S> struct Vec;
+template <> struct Vec<double> { using T = double2; };
+template <> struct Vec<float> { using T = float2; };
+
+template <typename V>
+__device__ __forceinline__ V mk(decltype(V::x) x, decltype(V::x) y) {
+    V v;
+    v.x = x;
+    v.y = y;
+    return v;
 }
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
-    return make_double2(a.x + b.x, a.y + b.y);
+template <typename V>
+__device__ __forceinline__ V cmul(V a, V b) {
+    return mk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) {
-    return make_double2(a.x - b.x, a.y - b.y);
-}
+template <typename V>
+__device__ __forceinline__ V cadd(V a, V b) { return mk<V>(a.x + b.x, a.y + b.y); }
+template <typename V>
+__device__ __forceinline__ V csub(V a, V b) { return mk<V>(a.x - b.x, a.y - b.y); }
 // multiply by -i
-__device__ __forceinline__ double2 mul_mi(double2 a) {
-    return make_double2(a.y, -a.x);
-}
+template <typename V>
+__device__ __forceinline__ V mul_mi(V a) { return mk<V>(a.y, -a.x); }
 
 // Forward DFT of R points held in registers: y_p = sum_q x_q e^{-2 pi i pq/R}.
-template <int R> struct Dft;
+template <int R, typename V> struct Dft;
 
-template <> struct Dft<2> {
-    static __device__ __forceinline__ void run(double2* v) {
-        const double2 a = v[0], b = v[1];
+template <typename V> struct Dft<2, V> {
+    static __device__ __forceinline__ void run(V* v) {
+        const V a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     }
 };
 
-template <> struct Dft<4> {
-    static __device__ __forceinline__ void run(double2* v) {
-        const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-        const double2 s13 = cadd(v[1], v[3]), d13 = mul_mi(csub(v[1], v[3]));
+template <typename V> struct Dft<4, V> {
+    static __device__ __forceinline__ void run(V* v) {
+        const V s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const V s13 = cadd(v[1], v[3]), d13 = mul_mi(csub(v[1], v[3]));
         v[0] = cadd(s02, s13);
         v[2] = csub(s02, s13);
         v[1] = cadd(d02, d13);
@@ -59,35 +72,33 @@ template <> struct Dft<4> {
     }
 };
 
-template <> struct Dft<3> {
-    static __device__ __forceinline__ void run(double2* v) {
-        constexpr double s = 0.86602540378443864676;  // sin(2 pi / 3)
-        const double2 t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-        const double2 base = make_double2(v[0].x - .5 * t.x, v[0].y - .5 * t.y);
-        const double2 rot = make_double2(s * d.y, -s * d.x);  // -i s d
+template <typename V> struct Dft<3, V> {
+    static __device__ __forceinline__ void run(V* v) {
+        using S = decltype(V::x);
+        constexpr S s = (S)0.86602540378443864676;  // sin(2 pi / 3)
+        const V t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const V base = mk<V>(v[0].x - (S).5 * t.x, v[0].y - (S).5 * t.y);
+        const V rot = mk<V>(s * d.y, -s * d.x);     // -i s d
         v[0] = cadd(v[0], t);
         v[1] = cadd(base, rot);
         v[2] = csub(base, rot);
     }
 };
 
-template <> struct Dft<5> {
-    static __device__ __forceinline__ void run(double2* v) {
-        constexpr double c1 = 0.30901699437494742410;   // cos(2 pi / 5)
-        constexpr double c2 = -0.80901699437494742410;  // cos(4 pi / 5)
-        constexpr double s1 = 0.95105651629515357212;   // sin(2 pi / 5)
-        constexpr double s2 = 0.58778525229247312917;   // sin(4 pi / 5)
-        const double2 x0 = v[0];
-        const double2 t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
-        const double2 t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
-        const double2 a1 = make_double2(x0.x + c1 * t1.x + c2 * t2.x,
-                                        x0.y + c1 * t1.y + c2 * t2.y);
-        const double2 a2 = make_double2(x0.x + c2 * t1.x + c1 * t2.x,
-                                        x0.y + c2 * t1.y + c1 * t2.y);
-        const double2 b1 = make_double2(s1 * d1.x + s2 * d2.x,
-                                        s1 * d1.y + s2 * d2.y);
-        const double2 b2 = make_double2(s2 * d1.x - s1 * d2.x,
-                                        s2 * d1.y - s1 * d2.y);
+template <typename V> struct Dft<5, V> {
+    static __device__ __forceinline__ void run(V* v) {
+        using S = decltype(V::x);
+        constexpr S c1 = (S)0.30901699437494742410;   // cos(2 pi / 5)
+        constexpr S c2 = (S)-0.80901699437494742410;  // cos(4 pi / 5)
+        constexpr S s1 = (S)0.95105651629515357212;   // sin(2 pi / 5)
+        constexpr S s2 = (S)0.58778525229247312917;   // sin(4 pi / 5)
+        const V x0 = v[0];
+        const V t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+        const V t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+        const V a1 = mk<V>(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+        const V a2 = mk<V>(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+        const V b1 = mk<V>(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+        const V b2 = mk<V>(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
         v[0] = cadd(x0, cadd(t1, t2));
         v[1] = cadd(a1, mul_mi(b1));
         v[4] = csub(a1, mul_mi(b1));
@@ -96,32 +107,36 @@ template <> struct Dft<5> {
     }
 };
 
-// Twiddle exp(-2 pi i t / L) from a quarter-wave cosine table in LDS:
-// ct[r] = cos(2 pi r / L), r in [0, L/4] (L is a multiple of 4).
-__device__ __forceinline__ double2 twiddle(const double* ct, int t, int L) {
+// ---------------------------------------------------------------------------
+// Runtime plan (any L = 2^a 3^b 5^c): twiddles from a quarter-wave cosine
+// table ct[r] = cos(2 pi r / L), r in [0, L/4], held in LDS.
+// ---------------------------------------------------------------------------
+
+template <typename V>
+__device__ __forceinline__ V twiddle(const decltype(V::x)* ct, int t, int L) {
+    using S = decltype(V::x);
     const int Q = L >> 2;
     const int quad = t / Q, r = t - quad * Q;
-    const double c = ct[r], s = ct[Q - r];   // cos and sin of 2 pi r / L
-    // cos / sin of 2 pi t / L = quad * pi / 2 + 2 pi r / L
-    double cs, sn;
+    const S c = ct[r], s = ct[Q - r];   // cos and sin of 2 pi r / L
+    S cs, sn;                           // of 2 pi t / L = quad pi / 2 + ...
     switch (quad) {
         case 0: cs = c; sn = s; break;
         case 1: cs = -s; sn = c; break;
         case 2: cs = -c; sn = -s; break;
         default: cs = s; sn = -c; break;
     }
-    return make_double2(cs, -sn);
+    return mk<V>(cs, -sn);
 }
 
 // One radix-R Stockham stage over the L points of this thread's pair.
 // Thread t of Tp handles butterflies t and t + Tp (Tp >= L / 4 makes two
 // enough for every radix >= 2).  span = product of the previous radices.
-template <int R>
-__device__ __forceinline__ void fft_stage(double2* z, const double* ct,
+template <int R, typename V>
+__device__ __forceinline__ void fft_stage(V* z, const decltype(V::x)* ct,
                                           int L, int span, int t, int Tp) {
     const int T = L / R;
     const int tw = L / (span * R);
-    double2 v[2][R];
+    V v[2][R];
 #pragma unroll
     for (int it = 0; it < 2; ++it) {
         const int b = t + it * Tp;
@@ -129,8 +144,8 @@ __device__ __forceinline__ void fft_stage(double2* z, const double* ct,
             const int k = b % span;
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                double2 x = z[b + q * T];
-                if (q && k) x = cmul(x, twiddle(ct, q * k * tw, L));
+                V x = z[b + q * T];
+                if (q && k) x = cmul(x, twiddle<V>(ct, q * k * tw, L));
                 v[it][q] = x;
             }
         }
@@ -140,7 +155,7 @@ __device__ __forceinline__ void fft_stage(double2* z, const double* ct,
     for (int it = 0; it < 2; ++it) {
         const int b = t + it * Tp;
         if (b < T) {
-            Dft<R>::run(v[it]);
+            Dft<R, V>::run(v[it]);
             const int k = b % span;
             const int j = (b - k) * R + k;
 #pragma unroll
@@ -152,7 +167,8 @@ __device__ __forceinline__ void fft_stage(double2* z, const double* ct,
 
 // In-place forward FFT of every pair's buffer (block-wide barriers inside:
 // every thread of the block must call it).
-__device__ void fft_lds(double2* z, const FftPlan& plan, const double* ct,
+template <typename V>
+__device__ void fft_lds(V* z, const FftPlan& plan, const decltype(V::x)* ct,
                         int t, int Tp) {
     int span = 1;
     for (int s = 0; s < plan.nstages; ++s) {
@@ -167,19 +183,14 @@ __device__ void fft_lds(double2* z, const FftPlan& plan, const double* ct,
     }
 }
 
-// Copies the quarter-wave cosine table into LDS (after it: a barrier).
-__device__ __forceinline__ double* load_costab(double2* lds, int P, int L,
-                                               const double* __restrict__ cg) {
-    double* ct = reinterpret_cast<double*>(lds + (size_t)P * L);
-    for (int r = threadIdx.x; r <= (L >> 2); r += blockDim.x) ct[r] = cg[r];
-    return ct;
-}
-
+// ---------------------------------------------------------------------------
 // Compile-time plans for the common transform lengths: the radix of each
 // stage and its span are constants, so the index arithmetic folds away,
 // and the twiddles of a stage with span s and radix R,
 // exp(-2 pi i q k / (s R)) for k < s, 1 <= q < R, sit contiguously at
 // tw[s - 1 + k (R - 1) + q - 1] (L - 1 entries for the whole plan).
+// ---------------------------------------------------------------------------
+
 __host__ __device__ constexpr int next_radix(int rem) {
     return rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
 }
@@ -188,17 +199,16 @@ __host__ __device__ constexpr int threads_per_pair(int L) {
     return ((L + 3) / 4 + 63) / 64 * 64;
 }
 
-template <int L, int SPAN>
-__device__ __forceinline__ void fft_static(double2* z,
-                                           const double2* __restrict__ tw,
+template <int L, int SPAN, typename V>
+__device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
                                            int t) {
     if constexpr (SPAN < L) {
         constexpr int R = next_radix(L / SPAN);
         constexpr int T = L / R;
         constexpr int Tp = threads_per_pair(L);
         constexpr int ITER = (T + Tp - 1) / Tp;
-        const double2* tws = tw + (SPAN - 1);
-        double2 v[ITER][R];
+        const V* tws = tw + (SPAN - 1);
+        V v[ITER][R];
 #pragma unroll
         for (int it = 0; it < ITER; ++it) {
             const int b = t + it * Tp;
@@ -206,7 +216,7 @@ __device__ __forceinline__ void fft_static(double2* z,
                 const int k = b % SPAN;
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
-                    double2 x = z[b + q * T];
+                    V x = z[b + q * T];
                     if (SPAN > 1 && q) x = cmul(x, tws[k * (R - 1) + q - 1]);
                     v[it][q] = x;
                 }
@@ -217,7 +227,7 @@ __device__ __forceinline__ void fft_static(double2* z,
         for (int it = 0; it < ITER; ++it) {
             const int b = t + it * Tp;
             if (b < T) {
-                Dft<R>::run(v[it]);
+                Dft<R, V>::run(v[it]);
                 const int k = b % SPAN;
                 const int j = (b - k) * R + k;
 #pragma unroll
@@ -232,29 +242,47 @@ __device__ __forceinline__ void fft_static(double2* z,
 // Twiddle source of a block: for L == 0 (runtime plan) the quarter-wave
 // cosine table, copied to LDS; for a static L the stage twiddles, copied to
 // LDS when TW_LDS, else read from global memory.
+template <typename S>
 struct Tw {
-    const double* ct;
-    const double2* tw;
+    const S* ct;
+    const typename Vec<S>::T* tw;
 };
 
-template <int L, bool TW_LDS>
-__device__ __forceinline__ Tw load_twiddles(double2* lds, const SolverGeom& g) {
-    Tw r{nullptr, nullptr};
+template <typename S> struct TwSrc;
+template <> struct TwSrc<double> {
+    static __device__ const double* ct(const SolverGeom& g) { return g.costab; }
+    static __device__ const double2* tw(const SolverGeom& g) { return g.stage_tw; }
+};
+template <> struct TwSrc<float> {
+    static __device__ const float* ct(const SolverGeom& g) { return g.costab32; }
+    static __device__ const float2* tw(const SolverGeom& g) { return g.stage_tw32; }
+};
+
+template <int L, bool TW_LDS, typename S>
+__device__ __forceinline__ Tw<S> load_twiddles(typename Vec<S>::T* lds,
+                                               const SolverGeom& g) {
+    using V = typename Vec<S>::T;
+    Tw<S> r{nullptr, nullptr};
     if constexpr (L == 0) {
-        r.ct = load_costab(lds, g.P, g.fft.L, g.costab);
+        S* ct = reinterpret_cast<S*>(lds + (size_t)g.P * g.fft.L);
+        const S* src = TwSrc<S>::ct(g);
+        for (int k = threadIdx.x; k <= (g.fft.L >> 2); k += blockDim.x) ct[k] = src[k];
+        r.ct = ct;
     } else if constexpr (TW_LDS) {
-        double2* d = lds + (size_t)g.P * L;
-        for (int k = threadIdx.x; k < L - 1; k += blockDim.x) d[k] = g.stage_tw[k];
+        V* d = lds + (size_t)g.P * L;
+        const V* src = TwSrc<S>::tw(g);
+        for (int k = threadIdx.x; k < L - 1; k += blockDim.x) d[k] = src[k];
         r.tw = d;
     } else {
-        r.tw = g.stage_tw;
+        r.tw = TwSrc<S>::tw(g);
     }
     return r;
 }
 
-template <int L>
-__device__ __forceinline__ void fft_any(double2* z, const SolverGeom& g,
-                                        const Tw& tw, int t) {
+template <int L, typename S>
+__device__ __forceinline__ void fft_any(typename Vec<S>::T* z,
+                                        const SolverGeom& g, const Tw<S>& tw,
+                                        int t) {
     if constexpr (L == 0) {
         fft_lds(z, g.fft, tw.ct, t, g.Tp);
     } else {
@@ -267,41 +295,44 @@ __device__ __forceinline__ void fft_any(double2* z, const SolverGeom& g,
 // -input k, z[0] = z[n+1] = 0.  For a pair (a real, b imaginary) the FFT
 // gives DCT(a) = Re Z[k], DCT(b) = Im Z[k]; DST(a) = -Im Z[k+1],
 // DST(b) = Re Z[k+1].
-__device__ __forceinline__ void put_ext(double2* z, int xf, int n, int len,
-                                        int L, double a, double b) {
-    const double2 v = make_double2(a, b);
+template <typename V>
+__device__ __forceinline__ void put_ext(V* z, int xf, int n, int len, int L,
+                                        double a, double b) {
+    using S = decltype(V::x);
+    const V v = mk<V>((S)a, (S)b);
     if (xf == XF_DCT) {
         z[n] = v;
         if (n >= 1 && n <= len - 2) z[L - n] = v;
     } else {
         z[n + 1] = v;
-        z[L - 1 - n] = make_double2(-a, -b);
+        z[L - 1 - n] = mk<V>(-v.x, -v.y);
     }
 }
 
-__device__ __forceinline__ void put_ext_zeros(double2* z, int xf, int len,
-                                              int t) {
+template <typename V>
+__device__ __forceinline__ void put_ext_zeros(V* z, int xf, int len, int t) {
     if (xf == XF_DST && t == 0) {
-        z[0] = make_double2(0., 0.);
-        z[len + 1] = make_double2(0., 0.);
+        z[0] = mk<V>(0, 0);
+        z[len + 1] = mk<V>(0, 0);
     }
 }
 
-__device__ __forceinline__ void get_ext(const double2* z, int xf, int k,
-                                        double& a, double& b) {
+template <typename V>
+__device__ __forceinline__ void get_ext(const V* z, int xf, int k, double& a,
+                                        double& b) {
     if (xf == XF_DCT) {
-        const double2 v = z[k];
+        const V v = z[k];
         a = v.x;
         b = v.y;
     } else {
-        const double2 v = z[k + 1];
+        const V v = z[k + 1];
         a = -v.y;
         b = v.x;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Right-hand sides (lcode.py:166-168, :234-268, :339-343)
+// Right-hand sides (lcode.py:166-168, :234-268, :339-343), FP64
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ double rho_src(const RhsInputs& in, size_t idx) {
@@ -353,9 +384,6 @@ __device__ __forceinline__ double rhs_value(int kind, int i, int j, int N,
     }
 }
 
-// Deterministic block sum of one value per thread, grouped by pair: the
-// result for pair p is returned to thread 0 of that pair.  red needs one
-// slot per wave.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -385,18 +413,20 @@ __device__ __forceinline__ BlockPos xcd_block() {
 // Passes
 // ---------------------------------------------------------------------------
 
-template <int L_, bool TW_LDS>
+template <int L_, bool TW_LDS, typename S>
 __global__ void __launch_bounds__(1024)
 pass_a_kernel(SolverGeom g, SolveBatch b) {
-    extern __shared__ double2 lds[];
+    using V = typename Vec<S>::T;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    V* lds = reinterpret_cast<V*>(lds_raw);
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L, N = g.N;
     const int lines = fp.rowsA;
     if (bp.block * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
-    double2* z = lds + (size_t)pair * L;
-    const Tw tw = load_twiddles<L_, TW_LDS>(lds, g);
+    V* z = lds + (size_t)pair * L;
+    const Tw<S> tw = load_twiddles<L_, TW_LDS, S>(lds, g);
     const int ra = (bp.block * g.P + pair) * 2, rb = ra + 1;
     const bool va = ra < lines, vb = rb < lines;
     for (int n = t; n < fp.lenA; n += Tp) {
@@ -407,21 +437,24 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     }
     put_ext_zeros(z, fp.xfA, fp.lenA, t);
     __syncthreads();
-    fft_any<L_>(z, g, tw, t);
+    fft_any<L_, S>(z, g, tw, t);
+    S* T = static_cast<S*>(fp.T);
     for (int k = t; k < fp.lenA; k += Tp) {
         double a, c;
         get_ext(z, fp.xfA, k, a, c);
-        if (va) fp.T[(size_t)k * lines + ra] = a;
-        if (vb) fp.T[(size_t)k * lines + rb] = c;
+        if (va) T[(size_t)k * lines + ra] = (S)a;
+        if (vb) T[(size_t)k * lines + rb] = (S)c;
     }
 }
 
 constexpr int kMaxPerThread = 4;
 
-template <int L_, bool TW_LDS>
+template <int L_, bool TW_LDS, typename S>
 __global__ void __launch_bounds__(1024)
 pass_b_kernel(SolverGeom g, SolveBatch b) {
-    extern __shared__ double2 lds[];
+    using V = typename Vec<S>::T;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    V* lds = reinterpret_cast<V*>(lds_raw);
     __shared__ double red[2][16];
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
@@ -429,49 +462,49 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     const int lines = fp.lenA, len = fp.rowsA, xf = fp.xfB;
     if (bp.block * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
-    double2* z = lds + (size_t)pair * L;
-    const Tw tw = load_twiddles<L_, TW_LDS>(lds, g);
+    V* z = lds + (size_t)pair * L;
+    const Tw<S> tw = load_twiddles<L_, TW_LDS, S>(lds, g);
     const int la = (bp.block * g.P + pair) * 2, lb = la + 1;
     const bool va = la < lines, vb = lb < lines;
-    double* rowa = fp.T + (size_t)la * len;
-    double* rowb = fp.T + (size_t)lb * len;
-    const double* taba = fp.table + (size_t)la * len;
-    const double* tabb = fp.table + (size_t)lb * len;
+    S* rowa = static_cast<S*>(fp.T) + (size_t)la * len;
+    S* rowb = static_cast<S*>(fp.T) + (size_t)lb * len;
+    const S* taba = static_cast<const S*>(fp.table) + (size_t)la * len;
+    const S* tabb = static_cast<const S*>(fp.table) + (size_t)lb * len;
 
     for (int n = t; n < len; n += Tp)
-        put_ext(z, xf, n, len, L, va ? rowa[n] : 0., vb ? rowb[n] : 0.);
+        put_ext(z, xf, n, len, L, va ? (double)rowa[n] : 0., vb ? (double)rowb[n] : 0.);
     put_ext_zeros(z, xf, len, t);
     __syncthreads();
-    fft_any<L_>(z, g, tw, t);
+    fft_any<L_, S>(z, g, tw, t);
 
-    double ha[kMaxPerThread], hb[kMaxPerThread];
+    S ha[kMaxPerThread], hb[kMaxPerThread];
 #pragma unroll
     for (int u = 0; u < kMaxPerThread; ++u) {
         const int k = t + u * Tp;
-        ha[u] = hb[u] = 0.;
+        ha[u] = hb[u] = 0;
         if (k < len) {
             double a, c;
             get_ext(z, xf, k, a, c);
-            if (va) ha[u] = a * taba[k];
-            if (vb) hb[u] = c * tabb[k];
+            if (va) ha[u] = (S)a * taba[k];
+            if (vb) hb[u] = (S)c * tabb[k];
         }
     }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < kMaxPerThread; ++u) {
         const int k = t + u * Tp;
-        if (k < len) put_ext(z, xf, k, len, L, ha[u], hb[u]);
+        if (k < len) put_ext(z, xf, k, len, L, (double)ha[u], (double)hb[u]);
     }
     put_ext_zeros(z, xf, len, t);
     __syncthreads();
-    fft_any<L_>(z, g, tw, t);
+    fft_any<L_, S>(z, g, tw, t);
 
     double sa = 0., sb = 0.;
     for (int k = t; k < len; k += Tp) {
         double a, c;
         get_ext(z, xf, k, a, c);
-        if (va) rowa[k] = a;
-        if (vb) rowb[k] = c;
+        if (va) rowa[k] = (S)a;
+        if (vb) rowb[k] = (S)c;
         sa += a;
         sb += c;
     }
@@ -498,30 +531,34 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     }
 }
 
-template <int L_, bool TW_LDS>
+template <int L_, bool TW_LDS, typename S>
 __global__ void __launch_bounds__(1024)
 pass_c_kernel(SolverGeom g, SolveBatch b) {
-    extern __shared__ double2 lds[];
+    using V = typename Vec<S>::T;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    V* lds = reinterpret_cast<V*>(lds_raw);
     __shared__ double red[16];
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L, N = g.N;
     if (bp.block * g.P * 2 >= N) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
-    double2* z = lds + (size_t)pair * L;
-    const Tw tw = load_twiddles<L_, TW_LDS>(lds, g);
+    V* z = lds + (size_t)pair * L;
+    const Tw<S> tw = load_twiddles<L_, TW_LDS, S>(lds, g);
     const int ra = (bp.block * g.P + pair) * 2, rb = ra + 1;
     const int ca = ra - fp.rowsA0, cb = rb - fp.rowsA0;
     const bool ha = ra < N && ca >= 0 && ca < fp.rowsA;
     const bool hb = rb < N && cb >= 0 && cb < fp.rowsA;
     const int stride = fp.rowsA;
+    const S* T = static_cast<const S*>(fp.T);
     for (int n = t; n < fp.lenA; n += Tp) {
-        const double* col = fp.T + (size_t)n * stride;
-        put_ext(z, fp.xfA, n, fp.lenA, L, ha ? col[ca] : 0., hb ? col[cb] : 0.);
+        const S* col = T + (size_t)n * stride;
+        put_ext(z, fp.xfA, n, fp.lenA, L, ha ? (double)col[ca] : 0.,
+                hb ? (double)col[cb] : 0.);
     }
     put_ext_zeros(z, fp.xfA, fp.lenA, t);
     __syncthreads();
-    fft_any<L_>(z, g, tw, t);
+    fft_any<L_, S>(z, g, tw, t);
 
     double mean = 0.;
     if (fp.partial) {
@@ -563,33 +600,34 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
 
 }  // namespace
 
-template <int L_, bool TW_LDS>
+template <int L_, bool TW_LDS, typename S>
 static int launch_passes(const SolverGeom& g, const SolveBatch& b,
                          hipStream_t s, Profiler* prof, size_t lds,
                          dim3 ga, dim3 gb, dim3 gc) {
     const dim3 block(g.P * g.Tp);
     {
         ProfScope ps(prof, "solve_pass_a", s);
-        pass_a_kernel<L_, TW_LDS><<<ga, block, lds, s>>>(g, b);
+        pass_a_kernel<L_, TW_LDS, S><<<ga, block, lds, s>>>(g, b);
     }
     LCODE_HIP(hipGetLastError());
     {
         ProfScope ps(prof, "solve_pass_b", s);
-        pass_b_kernel<L_, TW_LDS><<<gb, block, lds, s>>>(g, b);
+        pass_b_kernel<L_, TW_LDS, S><<<gb, block, lds, s>>>(g, b);
     }
     LCODE_HIP(hipGetLastError());
     {
         ProfScope ps(prof, "solve_pass_c", s);
-        pass_c_kernel<L_, TW_LDS><<<gc, block, lds, s>>>(g, b);
+        pass_c_kernel<L_, TW_LDS, S><<<gc, block, lds, s>>>(g, b);
     }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
 
 int solver_lds_bytes(const SolverGeom& g) {
-    const size_t data = (size_t)g.P * g.fft.L * sizeof(double2);
-    if (g.static_L == 0) return (int)(data + ((size_t)g.fft.L / 4 + 1) * sizeof(double));
-    if (g.tw_lds) return (int)(data + (size_t)(g.fft.L - 1) * sizeof(double2));
+    const size_t cplx = g.fp32 ? sizeof(float2) : sizeof(double2);
+    const size_t data = (size_t)g.P * g.fft.L * cplx;
+    if (g.static_L == 0) return (int)(data + ((size_t)g.fft.L / 4 + 1) * cplx / 2);
+    if (g.tw_lds) return (int)(data + (size_t)(g.fft.L - 1) * cplx);
     return (int)data;
 }
 
@@ -603,9 +641,9 @@ bool solver_static_length(int L) {
     }
 }
 
-int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
-                 Profiler* prof) {
-    if (b.nfields == 0) return LCODE_OK;
+template <typename S>
+static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
+                          hipStream_t s, Profiler* prof) {
     const size_t lds = (size_t)solver_lds_bytes(g);
     const int per_block = 2 * g.P;
     int max_a = 0, max_b = 0;
@@ -616,18 +654,26 @@ int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
     const dim3 ga((max_a + per_block - 1) / per_block, b.nfields);
     const dim3 gb((max_b + per_block - 1) / per_block, b.nfields);
     const dim3 gc((g.N + per_block - 1) / per_block, b.nfields);
-#define LCODE_CASE(LL)                                                        \
-    case LL:                                                                  \
-        return g.tw_lds ? launch_passes<LL, true>(g, b, s, prof, lds, ga, gb, gc) \
-                        : launch_passes<LL, false>(g, b, s, prof, lds, ga, gb, gc);
+#define LCODE_CASE(LL)                                                          \
+    case LL:                                                                    \
+        return g.tw_lds                                                         \
+                   ? launch_passes<LL, true, S>(g, b, s, prof, lds, ga, gb, gc) \
+                   : launch_passes<LL, false, S>(g, b, s, prof, lds, ga, gb, gc);
     switch (g.static_L) {
         LCODE_CASE(32) LCODE_CASE(64) LCODE_CASE(128) LCODE_CASE(256)
         LCODE_CASE(512) LCODE_CASE(640) LCODE_CASE(1024) LCODE_CASE(1280)
         LCODE_CASE(2048) LCODE_CASE(2560)
         default:
-            return launch_passes<0, false>(g, b, s, prof, lds, ga, gb, gc);
+            return launch_passes<0, false, S>(g, b, s, prof, lds, ga, gb, gc);
     }
 #undef LCODE_CASE
+}
+
+int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
+                 Profiler* prof) {
+    if (b.nfields == 0) return LCODE_OK;
+    return g.fp32 ? launch_solve_t<float>(g, b, s, prof)
+                  : launch_solve_t<double>(g, b, s, prof);
 }
 
 }  // namespace lcode

@@ -47,26 +47,21 @@ estimate_kernel(EstimateArgs a) {
 // move_smart_kernel (lcode.py:704-793): midpoint position, 6-field TSC
 // gather, two fixed-point iterations of the half-step momentum, offset and
 // momentum update, reflection with strict > / < flipping the momentum.
-// Four lanes share one particle: each gathers the 6 fields at 2-3 of the 9
-// stencil points and two butterfly shuffles sum them, so the few coarse
-// particles fill four times as many waves and their gathers overlap.
+// One-wave blocks spread the (few) coarse particles over all CUs.
 constexpr int kPushBlock = 64;
-constexpr int kLanesPerParticle = 4;
 
 __global__ void __launch_bounds__(kPushBlock)
 push_kernel(PushArgs a) {
-    const int lane = threadIdx.x & (kLanesPerParticle - 1);
-    const int k = (blockIdx.x * blockDim.x + threadIdx.x) / kLanesPerParticle;
-    const bool live = k < a.n;
-    const int kk = live ? k : 0;    // dead lanes still join the shuffles
-    const double m = a.m[kk], q = a.q[kk];
-    const double opx = a.px[kk], opy = a.py[kk], opz = a.pz[kk];
-    const double xi0 = a.x_init[kk], yi0 = a.y_init[kk];
-    double x_offt = a.prev_x_offt[kk], y_offt = a.prev_y_offt[kk];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const double m = a.m[k], q = a.q[k];
+    const double opx = a.px[k], opy = a.py[k], opz = a.pz[k];
+    const double xi0 = a.x_init[k], yi0 = a.y_init[k];
+    double x_offt = a.prev_x_offt[k], y_offt = a.prev_y_offt[k];
     double ex, ey;
     if (a.est_x) {
-        ex = a.est_x[kk];
-        ey = a.est_y[kk];
+        ex = a.est_x[k];
+        ey = a.est_y[k];
     } else {
         estimate_one(m, xi0, yi0, x_offt, y_offt, opx, opy, opz, a.dxi,
                      a.reflect, ex, ey);
@@ -74,25 +69,13 @@ push_kernel(PushArgs a) {
     const double x_half = xi0 + (x_offt + ex) / 2;
     const double y_half = yi0 + (y_offt + ey) / 2;
     Tsc t;
-    tsc_weights(x_half, y_half, a.N, a.h, t,
-                (live && lane == 0) ? a.fault : nullptr);
-    double F[6] = {0., 0., 0., 0., 0., 0.};
-    const int N = a.N;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) {
-        const int pt = lane + kLanesPerParticle * s;   // stencil point
-        if (pt < 9) {
-            const size_t idx = (size_t)(t.i + stencil_di(pt)) * N + (t.j + stencil_dj(pt));
-#pragma unroll
-            for (int f = 0; f < 6; ++f) F[f] += a.F[f][idx] * t.w[pt];
-        }
-    }
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        F[f] += __shfl_xor(F[f], 1, 64);
-        F[f] += __shfl_xor(F[f], 2, 64);
-    }
-    const double Ex = F[0], Ey = F[1], Ez = F[2], Bx = F[3], By = F[4], Bz = F[5];
+    tsc_weights(x_half, y_half, a.N, a.h, t, a.fault);
+    const double Ex = interp9(a.F[0], a.N, t);
+    const double Ey = interp9(a.F[1], a.N, t);
+    const double Ez = interp9(a.F[2], a.N, t);
+    const double Bx = interp9(a.F[3], a.N, t);
+    const double By = interp9(a.F[4], a.N, t);
+    const double Bz = interp9(a.F[5], a.N, t);
 
     double px = opx, py = opy, pz = opz;
     double dpx = 0, dpy = 0, dpz = 0;
@@ -122,13 +105,11 @@ push_kernel(PushArgs a) {
     if (y > +r) { y = +2 * r - y; y_offt = y - yi0; py = -py; }
     if (y < -r) { y = -2 * r - y; y_offt = y - yi0; py = -py; }
 
-    if (live && lane == 0) {
-        a.out_x[k] = x_offt;
-        a.out_y[k] = y_offt;
-        a.out_px[k] = px;
-        a.out_py[k] = py;
-        a.out_pz[k] = pz;
-    }
+    a.out_x[k] = x_offt;
+    a.out_y[k] = y_offt;
+    a.out_px[k] = px;
+    a.out_py[k] = py;
+    a.out_pz[k] = pz;
 }
 
 // Deposition values of a fine particle with mass m, charge q, momentum p
@@ -177,7 +158,7 @@ __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
 // and deposits, in LDS, every fine particle whose cell moved at most
 // a.halo cells from its initial cell and whose stencil touches the tile.
 constexpr int kTile = 32;
-constexpr int kDepBlock = 512;
+constexpr int kDepBlock = 256;
 
 // First fine index whose initial cell is >= c (fg is increasing).
 __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
@@ -390,17 +371,19 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         a.jy[idx] = acc[2][r][c];
         a.jz[idx] = acc[3][r][c];
     }
-    // the last block to finish deposits the spilled particles (every other
-    // block's stores and appends are visible after its release fence)
-    __threadfence();
+    // the last block to finish deposits the spilled particles: the barrier
+    // orders this block's stores before thread 0's device-scope release
+    // fence, which makes them visible before the block counts itself done
     __syncthreads();
     if (threadIdx.x == 0) {
+        __threadfence();
         const int total = gridDim.x * gridDim.y;
         is_last = atomicAdd(a.spill_done, 1) == total - 1;
     }
     __syncthreads();
     if (!is_last) return;
-    __threadfence();
+    if (threadIdx.x == 0) __threadfence();   // acquire the other blocks' stores
+    __syncthreads();
     const int count = atomicAdd(a.spill_count, 0);
     for (int p = threadIdx.x; p < count; p += blockDim.x)
         deposit_spill_one(a, a.spill[p], N);
@@ -432,8 +415,7 @@ int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof) {
 int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof) {
     if (a.n == 0) return LCODE_OK;
     ProfScope ps(prof, "push", s);
-    const long lanes = (long)a.n * kLanesPerParticle;
-    push_kernel<<<(int)((lanes + kPushBlock - 1) / kPushBlock), kPushBlock, 0, s>>>(a);
+    push_kernel<<<(a.n + kPushBlock - 1) / kPushBlock, kPushBlock, 0, s>>>(a);
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }

@@ -112,8 +112,21 @@ def _config_struct(config, nc=0, nf=0):
     c.plasma_fineness = int(config.plasma_fineness)
     c.nc = int(nc)
     c.nf = int(nf)
-    c.solver = _lib.LCODE_SOLVER_FFT64
+    c.solver = _solver_of(config)
     return c
+
+
+def _solver_of(config):
+    """config.field_solver_precision (an extension of the reference's config
+    surface): 'fp64' (default, the plain path) or 'fp32' (transforms,
+    spectra and eigenvalue tables in FP32; RHS, Bz mean, Variant A and
+    averages in FP64)."""
+    p = getattr(config, "field_solver_precision", "fp64")
+    if p == "fp64":
+        return _lib.LCODE_SOLVER_FFT64
+    if p == "fp32":
+        return _lib.LCODE_SOLVER_FFT32
+    raise ValueError(f"field_solver_precision must be 'fp64' or 'fp32', not {p!r}")
 
 
 def _stream():
