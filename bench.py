@@ -70,6 +70,64 @@ def kernel_bytes(name, nc, n):
     return 8 * table[name] if name in table else None
 
 
+def mixed_guided_leg(lc, cosine_gaussian, torch, slices, timed, precision):
+    """Mixed precision (FP32 transforms) + guided predictor (quadratic
+    extrapolation of the last three slices' fields in place of the predictor
+    pass, residual threshold 1e-5) against the plain FP64 step over the same
+    `slices` slices from init: on-axis Ez(xi) relative L2, the worst full-Ez
+    relative L2 of the 100-slice checkpoints, the acceptance rate, and the
+    throughput over `timed` slices of the bunch."""
+    cfg_p = make_config()
+    cfg_g = make_config()
+    cfg_g.field_solver_precision = precision
+    _, _, const, vp, sp = lc.init(cfg_p)
+    lc.init(cfg_g)
+    sg = sp
+    nc, nf = const.m.shape[0], vp.fine_grid.numel()
+    ex = lc.FieldExtrapolator(order=2)
+    n = cfg_p.grid_steps
+    ez_p, ez_g, full, accepted, guided = [], [], 0.0, 0, 0
+    for i in range(slices):
+        beam = cosine_gaussian(cfg_p, i)
+        sp = lc.step(cfg_p, const, vp, sp, beam)
+        g = ex.guess()
+        if g is None:
+            sg = lc.step(cfg_g, const, vp, sg, beam)
+        else:
+            sg = lc.step(cfg_g, const, vp, sg, beam, guess=g, threshold=1e-5)
+            guided += 1
+            accepted += lc.guided_residual(cfg_g, nc, nf) <= 1e-5
+        ex.push(sg)
+        ez_p.append(sp.Ez[n // 2, n // 2])
+        ez_g.append(sg.Ez[n // 2, n // 2])
+        if (i + 1) % 100 == 0:
+            e = (torch.linalg.norm(sg.Ez - sp.Ez) / torch.linalg.norm(sp.Ez)).item()
+            full = max(full, e)
+    ez_p, ez_g = torch.stack(ez_p), torch.stack(ez_g)
+    axis = (torch.linalg.norm(ez_g - ez_p) / torch.linalg.norm(ez_p)).item()
+    # throughput of the mixed + guided step inside the bunch
+    beams = [cosine_gaussian(cfg_g, 100 + k) for k in range(timed + 3)]
+    _, _, const, vp, st = lc.init(cfg_g)
+    ex = lc.FieldExtrapolator(order=2)
+    for k in range(3):
+        st = lc.step(cfg_g, const, vp, st, beams[k])
+        ex.push(st)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(timed):
+        st = lc.step(cfg_g, const, vp, st, beams[3 + k], guess=ex.guess(),
+                     threshold=1e-5)
+        ex.push(st)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"precision": precision, "predictor": "quadratic extrapolation "
+            "of the last 3 slices' fields, corrector residual threshold 1e-5",
+            "value": timed / dt, "unit": "xi-steps/s",
+            "slices_compared": slices,
+            "ez_axis_rel_l2": axis, "ez_checkpoint_rel_l2_max": full,
+            "guess_accept_rate": accepted / max(guided, 1)}
+
+
 def cpu_baseline(steps):
     """The oracle (NumPy/SciPy restatement of the reference step) on the
     host cores, on a bounded sample of the same workload."""
@@ -109,6 +167,8 @@ def main():
                     help="untimed slices run before warm-up (plasma develops)")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="steps timed per kernel with HIP events")
+    ap.add_argument("--mixed-slices", type=int, default=1500,
+                    help="slices of the mixed+guided vs plain comparison (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -191,6 +251,18 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: 3 x {args.copy_probe_mib} MiB device copies")
 
+    mixed = None
+    if args.mixed_slices:
+        mixed = mixed_guided_leg(lc, cosine_gaussian, torch, args.mixed_slices,
+                                 args.steps, "fp32")
+        mixed["vs_plain"] = mixed["value"] / steps_per_s
+        log(f"mixed+guided: {mixed['value']:.1f} xi-steps/s "
+            f"({mixed['vs_plain']:.2f}x plain), Ez axis rel L2 "
+            f"{mixed['ez_axis_rel_l2']:.2e}, checkpoints max "
+            f"{mixed['ez_checkpoint_rel_l2_max']:.2e} over "
+            f"{mixed['slices_compared']} slices, accepted "
+            f"{mixed['guess_accept_rate'] * 100:.0f}%")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_steps)
@@ -218,6 +290,7 @@ def main():
                    "fine_particles": int(vp.fine_grid.numel()) ** 2},
         "roofline": roofline,
         "kernels": kernels,
+        "mixed_guided": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

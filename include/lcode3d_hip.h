@@ -192,6 +192,37 @@ int lcode_step(lcode_workspace *ws, const lcode_config *cfg,
                double beam_scalar, const lcode_state *out,
                hipStream_t stream);
 
+/* A field guess for the guided step: guess = sum_t coef[t] f[t], the six
+ * fields Ex Ey Ez Bx By Bz of up to three N*N field sets (e.g. a predictor
+ * network's output with coef {1}, or the quadratic extrapolation
+ * 3 F(n-1) - 3 F(n-2) + F(n-3) with coef {3, -3, 1}). */
+typedef struct {
+    const double *f[3][6];
+    double coef[3];
+    int32_t nterms;
+} lcode_guess;
+
+/* The predictor hook of lcode.py:841-881: the predictor pass's deposition
+ * and field solve are replaced by `guess`.  The step pushes with prev
+ * fields (as the predictor does), then runs the corrector pass with the
+ * averages (guess + prev)/2, and measures the residual
+ * ||F_corr - guess|| / ||F_corr|| over the six fields on the device.  When
+ * it exceeds `threshold` (or is not a number) the plain predictor and
+ * corrector passes run after all; the final push and deposition follow
+ * either way.  No host synchronisation: the fallback kernels are always
+ * enqueued and return at once when the residual is small. */
+int lcode_step_guided(lcode_workspace *ws, const lcode_config *cfg,
+                      const lcode_const *cnst,
+                      const lcode_virt_params *virt_params,
+                      const lcode_state *prev, const double *beam_ro,
+                      double beam_scalar, const lcode_guess *guess,
+                      double threshold, const lcode_state *out,
+                      hipStream_t stream);
+
+/* Residual of the last guided step on this workspace (synchronises). */
+int lcode_step_guided_residual(lcode_workspace *ws, hipStream_t stream,
+                               double *residual);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,12 @@ class LcodeState(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in STATE_FIELDS]
 
 
+class LcodeGuess(ctypes.Structure):
+    _fields_ = [("f", (ctypes.c_void_p * 6) * 3),
+                ("coef", ctypes.c_double * 3),
+                ("nterms", ctypes.c_int32)]
+
+
 class LcodeKernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("calls", ctypes.c_int64),
                 ("total_ms", ctypes.c_double)]
@@ -106,6 +112,15 @@ _SIGNATURES = {
                     ctypes.POINTER(LcodeVirtParams),
                     ctypes.POINTER(LcodeState), _vp, ctypes.c_double,
                     ctypes.POINTER(LcodeState), _vp]),
+    "lcode_step_guided": (ctypes.c_int,
+                          [_vp, ctypes.POINTER(LcodeConfig),
+                           ctypes.POINTER(LcodeConst),
+                           ctypes.POINTER(LcodeVirtParams),
+                           ctypes.POINTER(LcodeState), _vp, ctypes.c_double,
+                           ctypes.POINTER(LcodeGuess), ctypes.c_double,
+                           ctypes.POINTER(LcodeState), _vp]),
+    "lcode_step_guided_residual": (ctypes.c_int,
+                                   [_vp, _vp, ctypes.POINTER(ctypes.c_double)]),
 }
 
 SYMBOLS = tuple(_SIGNATURES)

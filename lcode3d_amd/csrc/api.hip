@@ -47,6 +47,8 @@ struct lcode_workspace {
     int* fault = nullptr;
     int* spill = nullptr;          // nf * nf spilled fine particle indices
     int* spill_ctr = nullptr;      // [0] count, [1] blocks done
+    double* guess[6] = {};         // guided step: the field guess
+    double* resid = nullptr;       // {sum (F - guess)^2, sum F^2}
     Profiler prof;
     std::vector<void*> allocs;
 };
@@ -331,8 +333,11 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     }
     for (int k = 0; k < 4; ++k)
         if ((rc = dev_alloc(ws, (void**)&ws->dep[k], n2 * sizeof(double)))) return fail(rc);
-    for (int k = 0; k < 6; ++k)
+    for (int k = 0; k < 6; ++k) {
         if ((rc = dev_alloc(ws, (void**)&ws->favg[k], n2 * sizeof(double)))) return fail(rc);
+        if ((rc = dev_alloc(ws, (void**)&ws->guess[k], n2 * sizeof(double)))) return fail(rc);
+    }
+    if ((rc = dev_alloc(ws, (void**)&ws->resid, 2 * sizeof(double)))) return fail(rc);
     if ((rc = dev_alloc(ws, (void**)&ws->fault, sizeof(int)))) return fail(rc);
     const size_t nfp = (size_t)cfg->nf * cfg->nf;
     if ((rc = dev_alloc(ws, (void**)&ws->spill, (nfp ? nfp : 1) * sizeof(int)))) return fail(rc);
@@ -456,7 +461,7 @@ static int do_push(lcode_workspace* ws, const lcode_config* cfg,
                    const double* ex, const double* ey, const double* px,
                    const double* py, const double* pz,
                    const double* const F[6], double* const out[5],
-                   hipStream_t stream) {
+                   hipStream_t stream, Gate gate = Gate{nullptr, 0.}) {
     PushArgs a{};
     a.n = cfg->nc * cfg->nc;
     a.N = cfg->grid_steps;
@@ -471,6 +476,7 @@ static int do_push(lcode_workspace* ws, const lcode_config* cfg,
     a.out_x = out[0]; a.out_y = out[1];
     a.out_px = out[2]; a.out_py = out[3]; a.out_pz = out[4];
     a.fault = ws->fault;
+    a.gate = gate;
     return launch_push(a, stream, &ws->prof);
 }
 
@@ -500,7 +506,7 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
                       const double* yo, const double* m, const double* q,
                       const double* px, const double* py, const double* pz,
                       const lcode_virt_params* vp, double* const out[4],
-                      hipStream_t stream) {
+                      hipStream_t stream, Gate gate = Gate{nullptr, 0.}) {
     DepositArgs a{};
     a.nf = cfg->nf;
     a.nc = cfg->nc;
@@ -521,6 +527,7 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.halo = kDepositHalo;
     // the tile's coarse rectangle has at most ext / coarseness + 3 rows
     a.staged = (32 + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= 18;
+    a.gate = gate;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
     return launch_deposit(a, stream, &ws->prof);
 }
@@ -605,8 +612,16 @@ static int field_pass(lcode_workspace* ws, const lcode_config* cfg,
                       const lcode_state* prev, const double* const trick[4],
                       const double* beam_ro, double beam_scalar,
                       double* const dep[4], double* const F[6],
-                      double* const avg[6], hipStream_t stream) {
+                      double* const avg[6], hipStream_t stream,
+                      Gate gate = Gate{nullptr, 0.},
+                      const double* const guess[6] = nullptr,
+                      double* resid = nullptr) {
     SolveBatch b{};
+    b.gate = gate;
+    if (guess && resid) {
+        for (int k = 0; k < 6; ++k) b.guess[k] = guess[k];
+        b.resid = resid;
+    }
     b.nfields = 6;
     const double* prevF[6] = {prev->Ex, prev->Ey, prev->Ez,
                               prev->Bx, prev->By, prev->Bz};
@@ -697,6 +712,112 @@ int lcode_step(lcode_workspace* ws, const lcode_config* cfg,
     double* const outD[4] = {out->ro, out->jx, out->jy, out->jz};
     return do_deposit(ws, cfg, c->ro_initial, out->x_offt, out->y_offt, c->m,
                       c->q, out->px, out->py, out->pz, vp, outD, stream);
+}
+
+int lcode_step_guided(lcode_workspace* ws, const lcode_config* cfg,
+                      const lcode_const* c, const lcode_virt_params* vp,
+                      const lcode_state* prev, const double* beam_ro,
+                      double beam_scalar, const lcode_guess* guess,
+                      double threshold, const lcode_state* out,
+                      hipStream_t stream) {
+    int rc = check_ws(ws, cfg);
+    if (rc) return rc;
+    LCODE_REQUIRE(c && vp && prev && out && guess, "NULL struct");
+    LCODE_REQUIRE(cfg->nc == ws->nc, "workspace was built for another nc");
+    LCODE_REQUIRE(prev->Ex != out->Ex && prev->x_offt != out->x_offt,
+                  "prev and out must not alias");
+    LCODE_REQUIRE(guess->nterms >= 1 && guess->nterms <= 3, "guess needs 1-3 terms");
+    LCODE_REQUIRE(threshold >= 0, "negative threshold");
+    for (int t = 0; t < guess->nterms; ++t)
+        for (int f = 0; f < 6; ++f) LCODE_REQUIRE(guess->f[t][f], "NULL guess field");
+    const double* prevF[6] = {prev->Ex, prev->Ey, prev->Ez,
+                              prev->Bx, prev->By, prev->Bz};
+    const double* avg[6] = {ws->favg[0], ws->favg[1], ws->favg[2],
+                            ws->favg[3], ws->favg[4], ws->favg[5]};
+    const double* trick_prev[4] = {prev->Ex, prev->Ey, prev->Bx, prev->By};
+    const double* trick_avg[4] = {ws->favg[0], ws->favg[1], ws->favg[3],
+                                  ws->favg[4]};
+    double* const outF[6] = {out->Ex, out->Ey, out->Ez, out->Bx, out->By, out->Bz};
+    const double* g6[6] = {ws->guess[0], ws->guess[1], ws->guess[2],
+                           ws->guess[3], ws->guess[4], ws->guess[5]};
+    const Gate fallback{ws->resid, threshold * threshold};
+
+    // predictor push with prev fields (lcode.py:843-854)
+    if ((rc = do_push(ws, cfg, c->m, c->q, c->x_init, c->y_init,
+                      prev->x_offt, prev->y_offt, nullptr, nullptr,
+                      prev->px, prev->py, prev->pz, prevF, ws->p1, stream)))
+        return rc;
+    // the guess replaces the predictor's deposition and field solve
+    GuessArgs ga{};
+    for (int t = 0; t < guess->nterms; ++t) {
+        ga.c[t] = guess->coef[t];
+        for (int f = 0; f < 6; ++f) ga.h[t][f] = guess->f[t][f];
+    }
+    ga.nt = guess->nterms;
+    for (int f = 0; f < 6; ++f) {
+        ga.prev[f] = prevF[f];
+        ga.guess[f] = ws->guess[f];
+        ga.avg[f] = ws->favg[f];
+    }
+    ga.acc_zero = ws->resid;
+    ga.n = cfg->grid_steps * cfg->grid_steps;
+    if ((rc = launch_guess(ga, stream, &ws->prof))) return rc;
+    // corrector (lcode.py:884-911) on the guess, residual accumulated
+    if ((rc = do_push(ws, cfg, c->m, c->q, c->x_init, c->y_init,
+                      prev->x_offt, prev->y_offt, ws->p1[0], ws->p1[1],
+                      prev->px, prev->py, prev->pz, avg, ws->p2, stream)))
+        return rc;
+    if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p2[0], ws->p2[1], c->m,
+                         c->q, ws->p2[2], ws->p2[3], ws->p2[4], vp, ws->dep,
+                         stream)))
+        return rc;
+    if ((rc = field_pass(ws, cfg, prev, trick_avg, beam_ro, beam_scalar,
+                         ws->dep, outF, ws->favg, stream, Gate{nullptr, 0.},
+                         g6, ws->resid)))
+        return rc;
+    // fallback: the plain predictor and corrector passes, gated on the
+    // residual (lcode.py:856-911)
+    if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p1[0], ws->p1[1], c->m,
+                         c->q, ws->p1[2], ws->p1[3], ws->p1[4], vp, ws->dep,
+                         stream, fallback)))
+        return rc;
+    if ((rc = field_pass(ws, cfg, prev, trick_prev, beam_ro, beam_scalar,
+                         ws->dep, nullptr, ws->favg, stream, fallback)))
+        return rc;
+    if ((rc = do_push(ws, cfg, c->m, c->q, c->x_init, c->y_init,
+                      prev->x_offt, prev->y_offt, ws->p1[0], ws->p1[1],
+                      prev->px, prev->py, prev->pz, avg, ws->p2, stream,
+                      fallback)))
+        return rc;
+    if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p2[0], ws->p2[1], c->m,
+                         c->q, ws->p2[2], ws->p2[3], ws->p2[4], vp, ws->dep,
+                         stream, fallback)))
+        return rc;
+    if ((rc = field_pass(ws, cfg, prev, trick_avg, beam_ro, beam_scalar,
+                         ws->dep, outF, ws->favg, stream, fallback)))
+        return rc;
+    // final push and deposition (lcode.py:914-921)
+    double* const outP[5] = {out->x_offt, out->y_offt, out->px, out->py,
+                             out->pz};
+    if ((rc = do_push(ws, cfg, c->m, c->q, c->x_init, c->y_init,
+                      prev->x_offt, prev->y_offt, ws->p2[0], ws->p2[1],
+                      prev->px, prev->py, prev->pz, avg, outP, stream)))
+        return rc;
+    double* const outD[4] = {out->ro, out->jx, out->jy, out->jz};
+    return do_deposit(ws, cfg, c->ro_initial, out->x_offt, out->y_offt, c->m,
+                      c->q, out->px, out->py, out->pz, vp, outD, stream);
+}
+
+int lcode_step_guided_residual(lcode_workspace* ws, hipStream_t stream,
+                               double* residual) {
+    LCODE_REQUIRE(ws && residual, "NULL argument");
+    double host[2] = {0., 0.};
+    LCODE_HIP(hipMemcpyAsync(host, ws->resid, sizeof(host),
+                             hipMemcpyDeviceToHost, stream));
+    LCODE_HIP(hipStreamSynchronize(stream));
+    *residual = host[1] > 0 ? std::sqrt(host[0] / host[1])
+                            : (host[0] > 0 ? INFINITY : 0.);
+    return LCODE_OK;
 }
 
 }  // extern "C"

@@ -66,6 +66,23 @@ struct ProfScope {
 };
 
 // ---------------------------------------------------------------------------
+// Device-side gate of the guided step's fallback kernels
+// ---------------------------------------------------------------------------
+
+// acc = {sum (F - guess)^2, sum F^2} of the guided corrector; a gated kernel
+// runs only when the residual exceeds the threshold (or is not a number).
+struct Gate {
+    const double* acc;        // NULL: always run
+    double thr2;              // threshold squared
+};
+
+__device__ __forceinline__ bool gate_open(const Gate& g) {
+    if (!g.acc) return true;
+    const double num = g.acc[0], den = g.acc[1];
+    return !(num <= g.thr2 * den);
+}
+
+// ---------------------------------------------------------------------------
 // TSC-2D shape (lcode.py:396-418)
 // ---------------------------------------------------------------------------
 
@@ -139,6 +156,7 @@ struct PushArgs {
     const double *F[6];            // Ex Ey Ez Bx By Bz (averaged)
     double *out_x, *out_y, *out_px, *out_py, *out_pz;
     int* fault;
+    Gate gate;
 };
 
 struct EstimateArgs {
@@ -165,6 +183,7 @@ struct DepositArgs {
     int* spill_done;               // zero between calls
     int halo;                      // tile halo in cells
     int staged;                    // coarse rectangle of a tile fits LDS
+    Gate gate;
 };
 
 int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof);
@@ -214,7 +233,25 @@ struct SolveBatch {
     FieldPlan f[6];
     RhsInputs in;
     int variantA;             // 2 F - prev for Ex Ey Bx By
+    Gate gate;                // the whole solve is skipped when closed
+    const double* guess[6];   // guided step: the caller's field guess ...
+    double* resid;            // ... and {sum (F - guess)^2, sum F^2} (atomics)
 };
+
+// guess[f] = sum_t c[t] h[t][f]; avg[f] = (guess[f] + prev[f]) / 2 over n
+// values of the six fields; acc_zero[0..1] = 0.
+struct GuessArgs {
+    const double* h[3][6];
+    double c[3];
+    int nt;
+    const double* prev[6];
+    double* guess[6];
+    double* avg[6];
+    double* acc_zero;
+    int n;
+};
+
+int launch_guess(const GuessArgs& p, hipStream_t s, Profiler* prof);
 
 struct SolverGeom {
     int N;

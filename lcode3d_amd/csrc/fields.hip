@@ -419,6 +419,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     V* lds = reinterpret_cast<V*>(lds_raw);
+    if (!gate_open(b.gate)) return;
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L, N = g.N;
@@ -456,6 +457,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     V* lds = reinterpret_cast<V*>(lds_raw);
     __shared__ double red[2][16];
+    if (!gate_open(b.gate)) return;
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L;
@@ -538,6 +540,7 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     V* lds = reinterpret_cast<V*>(lds_raw);
     __shared__ double red[16];
+    if (!gate_open(b.gate)) return;
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L, N = g.N;
@@ -576,6 +579,8 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
                             fp.kind == F_BX || fp.kind == F_BY;
     const bool varA = b.variantA && transverse;
+    const double* guess = b.resid ? b.guess[fp.kind] : nullptr;
+    double r_num = 0., r_den = 0.;
     for (int j = t; j < N; j += Tp) {
         double va = 0., vb = 0.;
         const int k = j - fp.colA0;
@@ -594,7 +599,38 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
             if (varA) F = 2 * F - fp.prevF[idx];               // lcode.py:869-871
             if (fp.outF) fp.outF[idx] = F;
             if (fp.outAvg) fp.outAvg[idx] = (F + fp.prevF[idx]) / 2;  // :876-881
+            if (guess) {
+                const double d = F - guess[idx];
+                r_num += d * d;
+                r_den += F * F;
+            }
         }
+    }
+    if (guess) {
+        // guided step: residual of the corrector against the guess
+        r_num = wave_sum(r_num);
+        r_den = wave_sum(r_den);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&b.resid[0], r_num);
+            atomicAdd(&b.resid[1], r_den);
+        }
+    }
+}
+
+// Guided step set-up: guess = sum_t c[t] hist[t] for the six fields, the
+// predictor-pass averages (guess + prev) / 2 (lcode.py:876-881 with the
+// guess in place of the predicted fields), and zeroed residual sums.
+__global__ void __launch_bounds__(256)
+guess_kernel(GuessArgs p) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p.acc_zero && k < 2) p.acc_zero[k] = 0.;
+    if (k >= p.n) return;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        double gsum = 0.;
+        for (int t = 0; t < p.nt; ++t) gsum += p.c[t] * p.h[t][f][k];
+        p.guess[f][k] = gsum;
+        p.avg[f][k] = (gsum + p.prev[f][k]) / 2;
     }
 }
 
@@ -667,6 +703,15 @@ static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
             return launch_passes<0, false, S>(g, b, s, prof, lds, ga, gb, gc);
     }
 #undef LCODE_CASE
+}
+
+int launch_guess(const GuessArgs& p, hipStream_t s, Profiler* prof) {
+    {
+        ProfScope ps(prof, "guess", s);
+        guess_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(p);
+    }
+    LCODE_HIP(hipGetLastError());
+    return LCODE_OK;
 }
 
 int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,

@@ -53,7 +53,7 @@ constexpr int kPushBlock = 64;
 __global__ void __launch_bounds__(kPushBlock)
 push_kernel(PushArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.n) return;
+    if (k >= a.n || !gate_open(a.gate)) return;
     const double m = a.m[k], q = a.q[k];
     const double opx = a.px[k], opy = a.py[k], opz = a.pz[k];
     const double xi0 = a.x_init[k], yi0 = a.y_init[k];
@@ -158,7 +158,7 @@ __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
 // and deposits, in LDS, every fine particle whose cell moved at most
 // a.halo cells from its initial cell and whose stencil touches the tile.
 constexpr int kTile = 32;
-constexpr int kDepBlock = 256;
+constexpr int kDepBlock = 512;
 
 // First fine index whose initial cell is >= c (fg is increasing).
 __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
@@ -247,6 +247,7 @@ __device__ __forceinline__ void deposit_spill_one(const DepositArgs& a, int fk,
 
 template <bool STAGED>
 __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
+    if (!gate_open(a.gate)) return;   // block-uniform: before any barrier
     __shared__ double acc[4][kTile][kTile];
     __shared__ FineTab ti, tj;
     __shared__ double cs[7][kCoarseMax * kCoarseMax];

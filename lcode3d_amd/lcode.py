@@ -362,9 +362,21 @@ def initial_deposition(config, x_offt, y_offt, px, py, pz, m, q, virt_params):
 # The xi step (lcode.py:828-931)
 # ---------------------------------------------------------------------------
 
-def step(config, const, virt_params, prev, beam_ro):
+FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
+
+
+def step(config, const, virt_params, prev, beam_ro, guess=None,
+         threshold=1e-5):
     """One xi step; returns the new state (lcode.py:828-931).  beam_ro is
-    an (N, N) array (host or device) or a scalar."""
+    an (N, N) array (host or device) or a scalar.
+
+    guess (not in the reference): the predictor hook of lcode.py:841-881.
+    A sequence of (coefficient, fields) terms, fields being an object with
+    Ex..Bz attributes (e.g. earlier states) or a 6-tuple of (N, N) device
+    arrays; their linear combination replaces the predictor pass's
+    deposition and field solve.  The corrector's residual against the guess
+    is checked on the device and the plain predictor runs when it exceeds
+    `threshold` (see guided_residual)."""
     lib = _lib.load()
     beam, beam_scalar = _beam_arg(config, beam_ro)
     vp, nf, keep_vp = _virt_struct(virt_params)
@@ -390,16 +402,77 @@ def step(config, const, virt_params, prev, beam_ro):
     new = {name: torch.empty_like(getattr(prev, name))
            for name in _lib.STATE_FIELDS}
     o = _lib.LcodeState(*(new[name].data_ptr() for name in _lib.STATE_FIELDS))
-    _lib.check(lib.lcode_step(_workspace(cs), ctypes.byref(cs),
-                              ctypes.byref(c), ctypes.byref(vp),
-                              ctypes.byref(p),
-                              _ptr(beam) if beam is not None else None,
-                              beam_scalar, ctypes.byref(o), _stream()),
-               "lcode_step")
+    beam_ptr = _ptr(beam) if beam is not None else None
+    if guess is None:
+        _lib.check(lib.lcode_step(_workspace(cs), ctypes.byref(cs),
+                                  ctypes.byref(c), ctypes.byref(vp),
+                                  ctypes.byref(p), beam_ptr, beam_scalar,
+                                  ctypes.byref(o), _stream()), "lcode_step")
+    else:
+        g, keep_g = _guess_struct(config, guess)
+        _lib.check(lib.lcode_step_guided(_workspace(cs), ctypes.byref(cs),
+                                         ctypes.byref(c), ctypes.byref(vp),
+                                         ctypes.byref(p), beam_ptr,
+                                         beam_scalar, ctypes.byref(g),
+                                         float(threshold), ctypes.byref(o),
+                                         _stream()), "lcode_step_guided")
     state = GPUArrays.__new__(GPUArrays)
     for name, t in new.items():
         setattr(state, name, t)
     return state
+
+
+def _guess_struct(config, guess):
+    terms = list(guess)
+    if not 1 <= len(terms) <= 3:
+        raise ValueError("a guess has 1 to 3 (coefficient, fields) terms")
+    g = _lib.LcodeGuess()
+    keep = []
+    for t, (coef, fields) in enumerate(terms):
+        if not isinstance(fields, (tuple, list)):
+            fields = tuple(getattr(fields, name) for name in FIELDS)
+        arrs = _check_grid(config, *fields)
+        if len(arrs) != 6:
+            raise ValueError("a guess term needs the six fields Ex..Bz")
+        keep.extend(arrs)
+        g.coef[t] = float(coef)
+        for f, a in enumerate(arrs):
+            g.f[t][f] = a.data_ptr()
+    g.nterms = len(terms)
+    return g, keep
+
+
+def guided_residual(config, nc, nf):
+    """||F_corr - guess|| / ||F_corr|| of the last guided step on this
+    workspace (synchronises)."""
+    lib = _lib.load()
+    r = ctypes.c_double()
+    _lib.check(lib.lcode_step_guided_residual(
+        _workspace(_config_struct(config, nc, nf)), _stream(), ctypes.byref(r)),
+        "lcode_step_guided_residual")
+    return r.value
+
+
+class FieldExtrapolator:
+    """Predictor for the guided step: polynomial extrapolation in xi of the
+    last slices' fields, quadratic 3 F(n-1) - 3 F(n-2) + F(n-3) once three
+    slices are known (linear, then none, before that)."""
+
+    COEFS = {1: None, 2: ((2., 0), (-1., 1)), 3: ((3., 0), (-3., 1), (1., 2))}
+
+    def __init__(self, order=2):
+        self.order = int(order)
+        self.history = []            # newest first
+
+    def push(self, state):
+        self.history.insert(0, state)
+        del self.history[self.order + 1:]
+
+    def guess(self):
+        n = min(len(self.history), self.order + 1)
+        if n < 2:
+            return None
+        return [(c, self.history[k]) for c, k in self.COEFS[n]]
 
 
 # ---------------------------------------------------------------------------

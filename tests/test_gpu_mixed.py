@@ -1,7 +1,8 @@
 """The FP32-transform (mixed precision) field solves against the FP64
 oracle.  Transforms, spectra and eigenvalue tables are FP32; the RHS, the
 Bz zero mode and mean, Variant A and the averages stay FP64.  Tolerances
-are stated per check: a single solve keeps relative L2 <= 1e-5; the step
+are stated per check: a single solve keeps relative L2 <= 1e-5 (the Bz
+mean is zero to 1e-7 of max |Bz|); the step
 keeps Ez(xi, 0, 0) and every field within 1e-4 relative L2 over 200 slices
 of the 129^2 workload."""
 
@@ -43,7 +44,9 @@ def test_fp32_solves(lc, oracle, N):
                   oracle.calculate_Ez(cfg, jx, jy)) < 1e-5
     bz = host(lc.calculate_Bz(cfg, dev(jx), dev(jy)))
     assert rel_l2(bz, oracle.calculate_Bz(cfg, jx, jy)) < 1e-5
-    assert abs(bz.mean()) < 1e-12 * np.abs(bz).max()   # FP64 mean removal
+    # the mean is removed in FP64 from the analytic column sums of the FP32
+    # spectra: what is left is the FP32 rounding of the last transform
+    assert abs(bz.mean()) < 1e-7 * np.abs(bz).max()
     args = [smooth_field(rng, N) for _ in range(4)]
     dens = [smooth_field(rng, N) for _ in range(6)]
     got = lc.calculate_Ex_Ey_Bx_By(cfg, *[dev(a) for a in args], 0,
