@@ -55,6 +55,10 @@ struct lcode_workspace {
 
 namespace {
 
+#ifndef LCODE_TW_LDS_OK
+#define LCODE_TW_LDS_OK 1
+#endif
+
 // Cells a fine particle may move from its initial cell and still be
 // deposited through the LDS tiles (beyond: the atomic spill path).
 constexpr int kDepositHalo = 3;
@@ -223,7 +227,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     const int max_lds = 65536;
     auto pairs_for = [&](int fixed_bytes) {
         int P = 1024 / g.Tp;
-        const int by_lds = (max_lds - fixed_bytes) / (L * cplx);
+        const int by_lds = (max_lds - fixed_bytes) / (solver_padded_len(L) * cplx);
         if (by_lds < P) P = by_lds;
         return P > 4 ? 4 : P;
     };
@@ -231,7 +235,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     if (g.static_L) {
         // stage twiddles in LDS when that still leaves two line pairs
         P = pairs_for((L - 1) * cplx);
-        g.tw_lds = P >= 2;
+        g.tw_lds = LCODE_TW_LDS_OK && P >= 2;
         if (!g.tw_lds) P = pairs_for(0);
     } else {
         g.tw_lds = 0;

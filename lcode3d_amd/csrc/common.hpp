@@ -269,6 +269,7 @@ struct SolverGeom {
 int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
                  Profiler* prof);
 bool solver_static_length(int L);
+int solver_padded_len(int L);
 int solver_lds_bytes(const SolverGeom& g);
 
 }  // namespace lcode

@@ -50,6 +50,16 @@ __device__ __forceinline__ V csub(V a, V b) { return mk<V>(a.x - b.x, a.y - b.y)
 template <typename V>
 __device__ __forceinline__ V mul_mi(V a) { return mk<V>(a.y, -a.x); }
 
+// LDS layout of a pair's L complex points: with padding, one spare slot
+// after every 8 points spreads the strided Stockham stores over all banks.
+#ifndef LCODE_FFT_PAD
+#define LCODE_FFT_PAD 0
+#endif
+__host__ __device__ constexpr int zi(int e) {
+    return LCODE_FFT_PAD ? e + (e >> 3) : e;
+}
+__host__ __device__ constexpr int padded_len(int L) { return zi(L - 1) + 1; }
+
 // Forward DFT of R points held in registers: y_p = sum_q x_q e^{-2 pi i pq/R}.
 template <int R, typename V> struct Dft;
 
@@ -144,7 +154,7 @@ __device__ __forceinline__ void fft_stage(V* z, const decltype(V::x)* ct,
             const int k = b % span;
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                V x = z[b + q * T];
+                V x = z[zi(b + q * T)];
                 if (q && k) x = cmul(x, twiddle<V>(ct, q * k * tw, L));
                 v[it][q] = x;
             }
@@ -159,7 +169,7 @@ __device__ __forceinline__ void fft_stage(V* z, const decltype(V::x)* ct,
             const int k = b % span;
             const int j = (b - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) z[j + q * span] = v[it][q];
+            for (int q = 0; q < R; ++q) z[zi(j + q * span)] = v[it][q];
         }
     }
     __syncthreads();
@@ -216,7 +226,7 @@ __device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
                 const int k = b % SPAN;
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
-                    V x = z[b + q * T];
+                    V x = z[zi(b + q * T)];
                     if (SPAN > 1 && q) x = cmul(x, tws[k * (R - 1) + q - 1]);
                     v[it][q] = x;
                 }
@@ -231,7 +241,7 @@ __device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
                 const int k = b % SPAN;
                 const int j = (b - k) * R + k;
 #pragma unroll
-                for (int q = 0; q < R; ++q) z[j + q * SPAN] = v[it][q];
+                for (int q = 0; q < R; ++q) z[zi(j + q * SPAN)] = v[it][q];
             }
         }
         __syncthreads();
@@ -264,12 +274,12 @@ __device__ __forceinline__ Tw<S> load_twiddles(typename Vec<S>::T* lds,
     using V = typename Vec<S>::T;
     Tw<S> r{nullptr, nullptr};
     if constexpr (L == 0) {
-        S* ct = reinterpret_cast<S*>(lds + (size_t)g.P * g.fft.L);
+        S* ct = reinterpret_cast<S*>(lds + (size_t)g.P * padded_len(g.fft.L));
         const S* src = TwSrc<S>::ct(g);
         for (int k = threadIdx.x; k <= (g.fft.L >> 2); k += blockDim.x) ct[k] = src[k];
         r.ct = ct;
     } else if constexpr (TW_LDS) {
-        V* d = lds + (size_t)g.P * L;
+        V* d = lds + (size_t)g.P * padded_len(L);
         const V* src = TwSrc<S>::tw(g);
         for (int k = threadIdx.x; k < L - 1; k += blockDim.x) d[k] = src[k];
         r.tw = d;
@@ -301,19 +311,19 @@ __device__ __forceinline__ void put_ext(V* z, int xf, int n, int len, int L,
     using S = decltype(V::x);
     const V v = mk<V>((S)a, (S)b);
     if (xf == XF_DCT) {
-        z[n] = v;
-        if (n >= 1 && n <= len - 2) z[L - n] = v;
+        z[zi(n)] = v;
+        if (n >= 1 && n <= len - 2) z[zi(L - n)] = v;
     } else {
-        z[n + 1] = v;
-        z[L - 1 - n] = mk<V>(-v.x, -v.y);
+        z[zi(n + 1)] = v;
+        z[zi(L - 1 - n)] = mk<V>(-v.x, -v.y);
     }
 }
 
 template <typename V>
 __device__ __forceinline__ void put_ext_zeros(V* z, int xf, int len, int t) {
     if (xf == XF_DST && t == 0) {
-        z[0] = mk<V>(0, 0);
-        z[len + 1] = mk<V>(0, 0);
+        z[zi(0)] = mk<V>(0, 0);
+        z[zi(len + 1)] = mk<V>(0, 0);
     }
 }
 
@@ -321,11 +331,11 @@ template <typename V>
 __device__ __forceinline__ void get_ext(const V* z, int xf, int k, double& a,
                                         double& b) {
     if (xf == XF_DCT) {
-        const V v = z[k];
+        const V v = z[zi(k)];
         a = v.x;
         b = v.y;
     } else {
-        const V v = z[k + 1];
+        const V v = z[zi(k + 1)];
         a = -v.y;
         b = v.x;
     }
@@ -426,7 +436,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     const int lines = fp.rowsA;
     if (bp.block * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
-    V* z = lds + (size_t)pair * L;
+    V* z = lds + (size_t)pair * padded_len(L);
     const Tw<S> tw = load_twiddles<L_, TW_LDS, S>(lds, g);
     const int ra = (bp.block * g.P + pair) * 2, rb = ra + 1;
     const bool va = ra < lines, vb = rb < lines;
@@ -464,7 +474,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     const int lines = fp.lenA, len = fp.rowsA, xf = fp.xfB;
     if (bp.block * g.P * 2 >= lines) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
-    V* z = lds + (size_t)pair * L;
+    V* z = lds + (size_t)pair * padded_len(L);
     const Tw<S> tw = load_twiddles<L_, TW_LDS, S>(lds, g);
     const int la = (bp.block * g.P + pair) * 2, lb = la + 1;
     const bool va = la < lines, vb = lb < lines;
@@ -546,7 +556,7 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     const int Tp = g.Tp, L = g.fft.L, N = g.N;
     if (bp.block * g.P * 2 >= N) return;
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
-    V* z = lds + (size_t)pair * L;
+    V* z = lds + (size_t)pair * padded_len(L);
     const Tw<S> tw = load_twiddles<L_, TW_LDS, S>(lds, g);
     const int ra = (bp.block * g.P + pair) * 2, rb = ra + 1;
     const int ca = ra - fp.rowsA0, cb = rb - fp.rowsA0;
@@ -661,11 +671,13 @@ static int launch_passes(const SolverGeom& g, const SolveBatch& b,
 
 int solver_lds_bytes(const SolverGeom& g) {
     const size_t cplx = g.fp32 ? sizeof(float2) : sizeof(double2);
-    const size_t data = (size_t)g.P * g.fft.L * cplx;
+    const size_t data = (size_t)g.P * padded_len(g.fft.L) * cplx;
     if (g.static_L == 0) return (int)(data + ((size_t)g.fft.L / 4 + 1) * cplx / 2);
     if (g.tw_lds) return (int)(data + (size_t)(g.fft.L - 1) * cplx);
     return (int)data;
 }
+
+int solver_padded_len(int L) { return padded_len(L); }
 
 bool solver_static_length(int L) {
     switch (L) {

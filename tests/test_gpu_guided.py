@@ -7,8 +7,17 @@ predictor runs when it is too large.
   the plain step to relative L2 1e-12 (same kernels, atomic order aside).
 * Threshold 0 always falls back: same bar.
 * At 129^2 with xi_step_size 0.01 the quadratic extrapolation of the last
-  three slices is accepted on most slices and keeps Ez(xi, 0, 0) and all
-  fields within 1e-4 relative L2 of the plain FP64 step over 150 slices.
+  three slices is accepted on most slices at threshold 5e-3 and keeps
+  Ez(xi, 0, 0) and all fields within 1e-4 relative L2 of the plain FP64
+  step over 150 slices.
+
+The threshold: with the reference's single corrector pass the fields are
+one fixed-point iteration from the predictor, not converged, so even the
+plain step's own result, used as the guess, leaves a residual of about
+0.1 xi_step_size (1.1e-3 at 0.01, 5.5e-4 at 0.005, measured with the
+oracle); a 1e-5 threshold rejects every guess.  The acceptance threshold is
+therefore set above that floor and accuracy is judged against the plain
+step directly.
 """
 
 import types
@@ -86,8 +95,8 @@ def test_extrapolated_guess_accepted_and_accurate(lc):
             st_guided = lc.step(cfg, const, vp, st_guided, beam)
         else:
             st_guided = lc.step(cfg, const, vp, st_guided, beam, guess=g,
-                                threshold=1e-5)
-            accepted += lc.guided_residual(cfg, nc, nf) <= 1e-5
+                                threshold=5e-3)
+            accepted += lc.guided_residual(cfg, nc, nf) <= 5e-3
         ex.push(st_guided)
         ez_p.append(st_plain.Ez[64, 64].item())
         ez_g.append(st_guided.Ez[64, 64].item())
