@@ -70,6 +70,64 @@ def kernel_bytes(name, nc, n):
     return 8 * table[name] if name in table else None
 
 
+def dense_flops_per_solve(n):
+    """Algorithmic flops of one six-field solve in the dense-matrix form
+    (forward and inverse transform along both axes, 2 flops per
+    multiply-add): 8 N (N-1)(N-2) for each of the four transverse fields
+    (an (N-2) x N grid, DST-I along one axis, DCT-I along the other),
+    8 (N-2)^3 for Ez (DST-I both ways) and 8 N^3 for Bz (DCT-I both ways)."""
+    return 32 * n * (n - 1) * (n - 2) + 8 * (n - 2) ** 3 + 8 * n ** 3
+
+
+SOLVE_KERNELS = ("solve_pass_a", "solve_pass_b", "solve_pass_c", "dense_rhs",
+                 "dense_gemm_1", "dense_gemm_2", "dense_gemm_3", "dense_gemm_4")
+MFMA_PEAK_TFS = {"fp64": 78.6, "bf16x3": 2500.0}
+
+
+def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps):
+    """The same step with each field-solver back-end from the same developed
+    state: whole-step throughput over `steps` slices and the six-field solve
+    time from HIP events over `prof_steps` slices."""
+    import copy
+    legs = []
+    for transform, precision in (("fft", "fp64"), ("fft", "fp32"),
+                                 ("dense", "fp64"), ("dense", "bf16x3")):
+        c = copy.copy(cfg)
+        c.field_solver_transform = transform
+        c.field_solver_precision = precision
+        st = state
+        for k in range(3):
+            st = lc.step(c, const, vp, st, beams[k])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            st = lc.step(c, const, vp, st, beams[k % len(beams)])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        lc.profile_begin(64 * prof_steps)
+        for k in range(prof_steps):
+            st = lc.step(c, const, vp, st, beams[k % len(beams)])
+        prof = lc.profile_end()
+        solve_ms = sum(v[1] for k, v in prof.items() if k in SOLVE_KERNELS)
+        solve_us = solve_ms / (2 * prof_steps) * 1e3
+        leg = {"transform": transform, "precision": precision,
+               "xi_steps_per_s": steps / dt, "solve_us": solve_us,
+               "kernels_us": {k: v[1] / v[0] * 1e3 for k, v in prof.items()
+                              if k in SOLVE_KERNELS}}
+        if transform == "dense":
+            tf = dense_flops_per_solve(cfg.grid_steps) / (solve_us * 1e-6) / 1e12
+            leg["algorithmic_TFLOPs"] = tf
+            leg["mfma_peak_TFLOPs"] = MFMA_PEAK_TFS[precision]
+            leg["frac"] = tf / MFMA_PEAK_TFS[precision]
+        legs.append(leg)
+        log(f"  solver {transform:5s} {precision:6s}: {steps / dt:7.1f} xi-steps/s, "
+            f"six-field solve {solve_us:8.1f} us"
+            + (f", {leg['algorithmic_TFLOPs']:.1f} TFLOP/s algorithmic "
+               f"({leg['frac'] * 100:.1f}% of {precision} MFMA peak)"
+               if transform == "dense" else ""))
+    return legs
+
+
 GUIDED_THRESHOLD = 2e-3   # above the one-pass corrector's own residual
                           # (about 0.1 xi_step_size = 5.5e-4, p90 1.1e-3)
 
@@ -169,6 +227,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
                     help="field-solve transform precision (fp32: mixed)")
+    ap.add_argument("--solver-steps", type=int, default=50,
+                    help="slices per field-solver back-end comparison leg (0: skip)")
     ap.add_argument("--settle", type=int, default=100,
                     help="untimed slices run before warm-up (plasma develops)")
     ap.add_argument("--profile-steps", type=int, default=20,
@@ -257,6 +317,11 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: 3 x {args.copy_probe_mib} MiB device copies")
 
+    solvers = None
+    if args.solver_steps:
+        solvers = solver_legs(lc, torch, cfg, const, vp, state, beams,
+                              args.solver_steps, min(args.profile_steps or 10, 10))
+
     mixed = None
     if args.mixed_slices:
         mixed = mixed_guided_leg(lc, cosine_gaussian, torch, args.mixed_slices,
@@ -296,6 +361,7 @@ def main():
                    "fine_particles": int(vp.fine_grid.numel()) ** 2},
         "roofline": roofline,
         "kernels": kernels,
+        "solver_backends": solvers,
         "mixed_guided": mixed,
         "cpu_baseline": cpu,
     }

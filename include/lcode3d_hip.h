@@ -51,10 +51,17 @@ enum {
 /* Transform back-end of the field solves. */
 enum {
     LCODE_SOLVER_FFT64 = 0, /* hand-written mixed-radix FFT in LDS, FP64 */
-    LCODE_SOLVER_FFT32 = 1  /* the same in FP32 (mixed precision): the
+    LCODE_SOLVER_FFT32 = 1, /* the same in FP32 (mixed precision): the
                                transforms, spectra and eigenvalue tables;
                                RHS, Bz mean and zero mode, Variant A and
                                averages stay FP64 */
+    LCODE_SOLVER_DENSE64 = 2,      /* dense N x N sine/cosine matrices,
+                                      FP64 MFMA (v_mfma_f64_16x16x4) */
+    LCODE_SOLVER_DENSE_BF16X3 = 3  /* the same matrices on BF16 MFMA with
+                                      hi/lo splits of both operands, three
+                                      products, FP32 accumulation; RHS,
+                                      tables, Bz mean, Variant A and
+                                      averages stay FP64 */
 };
 
 /* The config attributes the step reads (config_example.py:1-17), with

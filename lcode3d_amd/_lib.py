@@ -20,6 +20,8 @@ LIB_PATH = os.environ.get("LCODE3D_HIP_LIB") or os.path.join(HERE, LIB_NAME)
 LCODE_OK = 0
 LCODE_SOLVER_FFT64 = 0
 LCODE_SOLVER_FFT32 = 1
+LCODE_SOLVER_DENSE64 = 2
+LCODE_SOLVER_DENSE_BF16X3 = 3
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)

@@ -33,6 +33,8 @@ struct lcode_workspace {
     float* costab32 = nullptr;
     float2* stage_tw32 = nullptr;
     int fp32 = 0;                  // LCODE_SOLVER_FFT32
+    int solver = 0;                // LCODE_SOLVER_*
+    DenseGeom dense{};             // LCODE_SOLVER_DENSE*: matrices, scratch
     void* tab_mixed = nullptr;     // [M][N]   (DST index, DCT index)
     void* tab_mixed_t = nullptr;   // [N][M]
     void* tab_dir = nullptr;       // [M][M]
@@ -118,7 +120,9 @@ int check_cfg(const lcode_config* c) {
                   "plasma_coarseness and plasma_fineness must be positive");
     LCODE_REQUIRE(c->plasma_fineness <= 3,
                   "plasma_fineness > 3 is not supported (deposition tile tables)");
-    LCODE_REQUIRE(c->solver == LCODE_SOLVER_FFT64 || c->solver == LCODE_SOLVER_FFT32,
+    LCODE_REQUIRE(c->solver == LCODE_SOLVER_FFT64 || c->solver == LCODE_SOLVER_FFT32 ||
+                      c->solver == LCODE_SOLVER_DENSE64 ||
+                      c->solver == LCODE_SOLVER_DENSE_BF16X3,
                   "unknown solver");
     return LCODE_OK;
 }
@@ -130,8 +134,8 @@ int check_ws(const lcode_workspace* ws, const lcode_config* c) {
     LCODE_REQUIRE(ws->N == c->grid_steps && ws->h == c->grid_step_size &&
                       ws->trick_on == (c->subtraction_trick != 0),
                   "workspace was built for another grid/trick");
-    LCODE_REQUIRE(ws->fp32 == (c->solver == LCODE_SOLVER_FFT32),
-                  "workspace was built for another solver precision");
+    LCODE_REQUIRE(ws->solver == c->solver,
+                  "workspace was built for another solver");
     return LCODE_OK;
 }
 
@@ -170,6 +174,12 @@ FieldPlan plan_for(lcode_workspace* ws, int kind, int slot) {
             break;
     }
     return p;
+}
+
+int run_solve(lcode_workspace* ws, const SolveBatch& b, hipStream_t stream) {
+    if (ws->solver == LCODE_SOLVER_DENSE64 || ws->solver == LCODE_SOLVER_DENSE_BF16X3)
+        return launch_solve_dense(ws->N, ws->dense, b, stream, &ws->prof);
+    return launch_solve(ws->geom, b, stream, &ws->prof);
 }
 
 RhsInputs rhs_base(const lcode_config* c) {
@@ -213,6 +223,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     // FFT geometry: one complex FFT of L = 2 (N - 1) points per line pair.
     SolverGeom& g = ws->geom;
     g.N = N;
+    ws->solver = cfg->solver;
     ws->fp32 = cfg->solver == LCODE_SOLVER_FFT32;
     g.fp32 = ws->fp32;
     const int cplx = ws->fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
@@ -326,6 +337,40 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     if ((rc = upload_table(ws, &ws->tab_dir, dir))) return fail(rc);
     if ((rc = upload_table(ws, &ws->tab_neu, neu))) return fail(rc);
     if ((rc = upload(ws, &ws->colsum, colsum))) return fail(rc);
+
+    if (ws->solver == LCODE_SOLVER_DENSE64 || ws->solver == LCODE_SOLVER_DENSE_BF16X3) {
+        // Dense type-1 transform matrices, unnormalised like the FFT form:
+        // DST-I Sd[k][n] = 2 sin(pi (k+1)(n+1) / (N-1)), DCT-I
+        // Cd[k][n] = w_n cos(pi k n / (N-1)) with w = 1 at both ends, else 2.
+        const long double pi = 3.14159265358979323846264338327950288L;
+        std::vector<double> Sd((size_t)M * M), Cd((size_t)N * N), CdT((size_t)N * N);
+        for (int k = 0; k < M; ++k)
+            for (int n = 0; n < M; ++n)
+                Sd[(size_t)k * M + n] = (double)(2 * std::sin(pi * ((k + 1) * (long long)(n + 1) %
+                                                                 (2 * (N - 1))) / (N - 1)));
+        for (int k = 0; k < N; ++k)
+            for (int n = 0; n < N; ++n) {
+                const long double w = (n == 0 || n == N - 1) ? 1 : 2;
+                const double v = (double)(w * std::cos(pi * ((long long)k * n % (2 * (N - 1))) /
+                                                       (N - 1)));
+                Cd[(size_t)k * N + n] = v;
+                CdT[(size_t)n * N + k] = v;
+            }
+        DenseGeom& d = ws->dense;
+        d.prec = ws->solver == LCODE_SOLVER_DENSE64 ? DENSE_F64 : DENSE_BF16X3;
+        double *pS, *pC, *pCT, *pm, *pmt;
+        if ((rc = upload(ws, &pS, Sd))) return fail(rc);
+        if ((rc = upload(ws, &pC, Cd))) return fail(rc);
+        if ((rc = upload(ws, &pCT, CdT))) return fail(rc);
+        if ((rc = upload(ws, &pm, mixed))) return fail(rc);
+        if ((rc = upload(ws, &pmt, mixed_t))) return fail(rc);
+        d.Sd = pS; d.Cd = pC; d.CdT = pCT; d.mixed = pm; d.mixed_t = pmt;
+        d.colsum = ws->colsum;
+        const size_t n2d = (size_t)N * N;
+        if ((rc = dev_alloc(ws, (void**)&d.X, 6 * n2d * sizeof(double)))) return fail(rc);
+        if ((rc = dev_alloc(ws, (void**)&d.Y, 6 * n2d * sizeof(double)))) return fail(rc);
+        if ((rc = dev_alloc(ws, (void**)&d.mean, sizeof(double)))) return fail(rc);
+    }
 
     const size_t n2 = (size_t)N * N, np = (size_t)cfg->nc * cfg->nc;
     if ((rc = dev_alloc(ws, &ws->T, 6 * n2 * (ws->fp32 ? sizeof(float) : sizeof(double)))))
@@ -579,7 +624,7 @@ int lcode_calculate_Ex_Ey_Bx_By(lcode_workspace* ws, const lcode_config* cfg,
     b.in.beam = beam_ro;
     b.in.beam_scalar = beam_scalar;
     b.variantA = 0;
-    return launch_solve(ws->geom, b, stream, &ws->prof);
+    return run_solve(ws, b, stream);
 }
 
 static int solve_single(lcode_workspace* ws, const lcode_config* cfg,
@@ -595,7 +640,7 @@ static int solve_single(lcode_workspace* ws, const lcode_config* cfg,
     b.in = rhs_base(cfg);
     b.in.jx = jx;
     b.in.jy = jy;
-    return launch_solve(ws->geom, b, stream, &ws->prof);
+    return run_solve(ws, b, stream);
 }
 
 int lcode_calculate_Ez(lcode_workspace* ws, const lcode_config* cfg,
@@ -647,7 +692,7 @@ static int field_pass(lcode_workspace* ws, const lcode_config* cfg,
     b.in.beam = beam_ro;
     b.in.beam_scalar = beam_scalar;
     b.variantA = cfg->variant_A ? 1 : 0;
-    return launch_solve(ws->geom, b, stream, &ws->prof);
+    return run_solve(ws, b, stream);
 }
 
 int lcode_step(lcode_workspace* ws, const lcode_config* cfg,

@@ -214,6 +214,57 @@ struct RhsInputs {
     double trick, h, dxi;
 };
 
+// Right-hand sides (lcode.py:166-168, :234-268, :339-343), FP64.
+
+__device__ __forceinline__ double rho_src(const RhsInputs& in, size_t idx) {
+    double r = in.ro[idx];
+    if (in.ro_prev) r = (r + in.ro_prev[idx]) / 2;     // lcode.py:862
+    return r + (in.beam ? in.beam[idx] : in.beam_scalar);
+}
+
+__device__ __forceinline__ double jz_src(const RhsInputs& in, size_t idx) {
+    double r = in.jz[idx];
+    if (in.jz_prev) r = (r + in.jz_prev[idx]) / 2;     // lcode.py:863
+    return r + (in.beam ? in.beam[idx] : in.beam_scalar);
+}
+
+__device__ __forceinline__ double rhs_value_dev(int kind, int i, int j, int N,
+                                            const RhsInputs& in) {
+    const size_t idx = (size_t)i * N + j;
+    const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
+    const double h2 = in.h * 2;
+    switch (kind) {
+        case F_EX: {
+            const double d = inner ? (rho_src(in, idx + N) - rho_src(in, idx - N)) / h2 : 0.;
+            const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
+            return -((d - dj) - in.Ftrick[0][idx] * in.trick);
+        }
+        case F_EY: {
+            const double d = inner ? (rho_src(in, idx + 1) - rho_src(in, idx - 1)) / h2 : 0.;
+            const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
+            return -((d - dj) - in.Ftrick[1][idx] * in.trick);
+        }
+        case F_BX: {
+            const double d = inner ? (jz_src(in, idx + 1) - jz_src(in, idx - 1)) / h2 : 0.;
+            const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
+            return +((d - dj) + in.Ftrick[2][idx] * in.trick);
+        }
+        case F_BY: {
+            const double d = inner ? (jz_src(in, idx + N) - jz_src(in, idx - N)) / h2 : 0.;
+            const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
+            return -((d - dj) - in.Ftrick[3][idx] * in.trick);
+        }
+        case F_EZ:  // only interior (i, j) reach here
+            return -((in.jx[idx + N] - in.jx[idx - N]) +
+                     (in.jy[idx + 1] - in.jy[idx - 1])) / h2;
+        default: {  // F_BZ
+            const double djx_dy = inner ? in.jx[idx + 1] - in.jx[idx - 1] : 0.;
+            const double djy_dx = inner ? in.jy[idx + N] - in.jy[idx - N] : 0.;
+            return -(djx_dy - djy_dx) / h2;
+        }
+    }
+}
+
 struct FieldPlan {
     int kind;                 // FieldKind
     int rowsA0, rowsA;        // rows i of the RHS transformed along j
@@ -252,6 +303,61 @@ struct GuessArgs {
 };
 
 int launch_guess(const GuessArgs& p, hipStream_t s, Profiler* prof);
+
+// Dense (MFMA) form of the field solves -------------------------------------
+
+enum DensePrec { DENSE_F64 = 0, DENSE_BF16X3 = 1 };
+
+
+struct GemmOp {                // C = A B over an M x N x K iteration space
+    const double* A; int lda, a_off, a_rows;   // A(r, k) = A[(r - a_off) lda + k]
+    const double* B; int ldb, b_off, b_cols;   // B(k, c) = B[k ldb + c - b_off]
+    double* C; int ldc;
+    int M, N, K;
+    int mode;                  // 0 store, 1 x table (+ Bz mean sum), 2 final
+    const double* tab; int ldt;
+    const double* colsum;      // mode 1, Bz: column sums of the DCT-I
+    double* mean_acc;
+    const double* mean_in;     // mode 2, Bz: the sum to remove (/ N^2)
+    int variantA;
+    const double* prevF;
+    double *outF, *outAvg;
+    const double* guess;       // mode 2, guided step: residual against it
+};
+
+struct GemmBatch {
+    int n;
+    GemmOp op[6];
+    Gate gate;
+    double* resid;             // {sum (F - guess)^2, sum F^2}
+};
+
+struct DenseRhsArgs {
+    FieldPlan f[6];
+    RhsInputs in;
+    int N;
+    double* R[6];
+    double* mean_zero;
+    Gate gate;
+};
+
+struct DenseGeom {
+    int prec;                  // DensePrec
+    const double* Sd;          // DST-I, (N-2) x (N-2), symmetric
+    const double* Cd;          // DCT-I, N x N: Cd[k][n] = w_n cos(pi k n / (N-1))
+    const double* CdT;         // its transpose
+    const double* colsum;      // column sums of Cd
+    const double *mixed, *mixed_t;   // eigenvalue tables [M][N] and [N][M]
+    double *X, *Y;             // 6 N^2 scratch each
+    double* mean;              // Bz sum
+};
+
+int launch_gemm(const GemmBatch& b, int prec, hipStream_t s, Profiler* prof,
+                const char* name);
+int launch_dense_rhs(const DenseRhsArgs& a, int nfields, int max_total,
+                     hipStream_t s, Profiler* prof);
+int launch_solve_dense(int N, const DenseGeom& d, const SolveBatch& b,
+                       hipStream_t s, Profiler* prof);
 
 struct SolverGeom {
     int N;

@@ -341,57 +341,9 @@ __device__ __forceinline__ void get_ext(const V* z, int xf, int k, double& a,
     }
 }
 
-// ---------------------------------------------------------------------------
-// Right-hand sides (lcode.py:166-168, :234-268, :339-343), FP64
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ double rho_src(const RhsInputs& in, size_t idx) {
-    double r = in.ro[idx];
-    if (in.ro_prev) r = (r + in.ro_prev[idx]) / 2;     // lcode.py:862
-    return r + (in.beam ? in.beam[idx] : in.beam_scalar);
-}
-
-__device__ __forceinline__ double jz_src(const RhsInputs& in, size_t idx) {
-    double r = in.jz[idx];
-    if (in.jz_prev) r = (r + in.jz_prev[idx]) / 2;     // lcode.py:863
-    return r + (in.beam ? in.beam[idx] : in.beam_scalar);
-}
-
 __device__ __forceinline__ double rhs_value(int kind, int i, int j, int N,
                                             const RhsInputs& in) {
-    const size_t idx = (size_t)i * N + j;
-    const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
-    const double h2 = in.h * 2;
-    switch (kind) {
-        case F_EX: {
-            const double d = inner ? (rho_src(in, idx + N) - rho_src(in, idx - N)) / h2 : 0.;
-            const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
-            return -((d - dj) - in.Ftrick[0][idx] * in.trick);
-        }
-        case F_EY: {
-            const double d = inner ? (rho_src(in, idx + 1) - rho_src(in, idx - 1)) / h2 : 0.;
-            const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
-            return -((d - dj) - in.Ftrick[1][idx] * in.trick);
-        }
-        case F_BX: {
-            const double d = inner ? (jz_src(in, idx + 1) - jz_src(in, idx - 1)) / h2 : 0.;
-            const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
-            return +((d - dj) + in.Ftrick[2][idx] * in.trick);
-        }
-        case F_BY: {
-            const double d = inner ? (jz_src(in, idx + N) - jz_src(in, idx - N)) / h2 : 0.;
-            const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
-            return -((d - dj) - in.Ftrick[3][idx] * in.trick);
-        }
-        case F_EZ:  // only interior (i, j) reach here
-            return -((in.jx[idx + N] - in.jx[idx - N]) +
-                     (in.jy[idx + 1] - in.jy[idx - 1])) / h2;
-        default: {  // F_BZ
-            const double djx_dy = inner ? in.jx[idx + 1] - in.jx[idx - 1] : 0.;
-            const double djy_dx = inner ? in.jy[idx + N] - in.jy[idx - N] : 0.;
-            return -(djx_dy - djy_dx) / h2;
-        }
-    }
+    return rhs_value_dev(kind, i, j, N, in);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -116,17 +116,32 @@ def _config_struct(config, nc=0, nf=0):
     return c
 
 
+_SOLVERS = {
+    ("fft", "fp64"): _lib.LCODE_SOLVER_FFT64,
+    ("fft", "fp32"): _lib.LCODE_SOLVER_FFT32,
+    ("dense", "fp64"): _lib.LCODE_SOLVER_DENSE64,
+    ("dense", "bf16x3"): _lib.LCODE_SOLVER_DENSE_BF16X3,
+}
+
+
 def _solver_of(config):
-    """config.field_solver_precision (an extension of the reference's config
-    surface): 'fp64' (default, the plain path) or 'fp32' (transforms,
-    spectra and eigenvalue tables in FP32; RHS, Bz mean, Variant A and
-    averages in FP64)."""
+    """Field-solver back-end, from two attributes that extend the
+    reference's config surface:
+    config.field_solver_transform: 'fft' (default; mixed-radix FFT in LDS)
+    or 'dense' (N x N sine/cosine matrices on MFMA);
+    config.field_solver_precision: 'fp64' (default, the plain path), 'fp32'
+    (FFT only: transforms, spectra and eigenvalue tables in FP32) or
+    'bf16x3' (dense only: hi/lo BF16 splits, three MFMA products, FP32
+    accumulation).  RHS, Bz mean, Variant A and averages are FP64 in every
+    mode."""
     p = getattr(config, "field_solver_precision", "fp64")
-    if p == "fp64":
-        return _lib.LCODE_SOLVER_FFT64
-    if p == "fp32":
-        return _lib.LCODE_SOLVER_FFT32
-    raise ValueError(f"field_solver_precision must be 'fp64' or 'fp32', not {p!r}")
+    t = getattr(config, "field_solver_transform", "dense" if p == "bf16x3" else "fft")
+    try:
+        return _SOLVERS[(t, p)]
+    except KeyError:
+        raise ValueError(
+            f"unsupported field solver: transform {t!r} with precision {p!r} "
+            f"(supported: {sorted(_SOLVERS)})") from None
 
 
 def _stream():
