@@ -230,6 +230,16 @@ int lcode_step_guided(lcode_workspace *ws, const lcode_config *cfg,
 int lcode_step_guided_residual(lcode_workspace *ws, hipStream_t stream,
                                double *residual);
 
+/* Plasma-noise diagnostic of one density grid (lcode.py:980-989,
+ * diags_ro_zn before its running maximum): zn = mean |ro - G ro| /
+ * 4.23045376e-04, where G is the Gaussian blur of sigma =
+ * 0.25 / grid_step_size cells with scipy.ndimage.gaussian_filter's
+ * conventions (taps truncated at 4 sigma, half-sample 'reflect'
+ * boundaries, axis 0 then axis 1).  `ro` is an N*N device grid; the value
+ * is written to the device double *zn_out; no host synchronisation. */
+int lcode_diag_zn(lcode_workspace *ws, const lcode_config *cfg,
+                  const double *ro, double *zn_out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

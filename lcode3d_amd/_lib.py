@@ -123,6 +123,8 @@ _SIGNATURES = {
                            ctypes.POINTER(LcodeState), _vp]),
     "lcode_step_guided_residual": (ctypes.c_int,
                                    [_vp, _vp, ctypes.POINTER(ctypes.c_double)]),
+    "lcode_diag_zn": (ctypes.c_int,
+                      [_vp, ctypes.POINTER(LcodeConfig), _vp, _vp, _vp]),
 }
 
 SYMBOLS = tuple(_SIGNATURES)

@@ -51,6 +51,10 @@ struct lcode_workspace {
     int* spill_ctr = nullptr;      // [0] count, [1] blocks done
     double* guess[6] = {};         // guided step: the field guess
     double* resid = nullptr;       // {sum (F - guess)^2, sum F^2}
+    double* diag_w = nullptr;      // zn diagnostic: Gaussian taps [2 r + 1]
+    int diag_r = 0;
+    double* diag_tmp = nullptr;    // [N * N]
+    double* diag_sum = nullptr;
     Profiler prof;
     std::vector<void*> allocs;
 };
@@ -64,6 +68,27 @@ namespace {
 // Cells a fine particle may move from its initial cell and still be
 // deposited through the LDS tiles (beyond: the atomic spill path).
 constexpr int kDepositHalo = 3;
+
+// Round-to-nearest-even BF16 bits of a float (finite inputs).
+uint16_t bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+float bf16_value(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// hi + lo BF16 split of v: hi = bf16(v), lo = bf16(v - hi).
+void split_bf16(double v, uint16_t& hi, uint16_t& lo) {
+    hi = bf16_bits((float)v);
+    lo = bf16_bits((float)(v - (double)bf16_value(hi)));
+}
 
 int dev_alloc(lcode_workspace* ws, void** p, size_t bytes) {
     LCODE_HIP(hipMalloc(p, bytes));
@@ -338,6 +363,21 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     if ((rc = upload_table(ws, &ws->tab_neu, neu))) return fail(rc);
     if ((rc = upload(ws, &ws->colsum, colsum))) return fail(rc);
 
+    {
+        // zn diagnostic taps (scipy.ndimage.gaussian_filter1d: truncate 4)
+        const double sigma = 0.25 / ws->h;
+        const int r = (int)(4.0 * sigma + 0.5);
+        std::vector<double> w(2 * r + 1);
+        double tot = 0.;
+        for (int k = -r; k <= r; ++k) tot += (w[k + r] = std::exp(-0.5 / (sigma * sigma) * k * k));
+        for (double& v : w) v /= tot;
+        ws->diag_r = r;
+        if ((rc = upload(ws, &ws->diag_w, w))) return fail(rc);
+        if ((rc = dev_alloc(ws, (void**)&ws->diag_tmp, (size_t)N * N * sizeof(double))))
+            return fail(rc);
+        if ((rc = dev_alloc(ws, (void**)&ws->diag_sum, sizeof(double)))) return fail(rc);
+    }
+
     if (ws->solver == LCODE_SOLVER_DENSE64 || ws->solver == LCODE_SOLVER_DENSE_BF16X3) {
         // Dense type-1 transform matrices, unnormalised like the FFT form:
         // DST-I Sd[k][n] = 2 sin(pi (k+1)(n+1) / (N-1)), DCT-I
@@ -366,10 +406,34 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         if ((rc = upload(ws, &pmt, mixed_t))) return fail(rc);
         d.Sd = pS; d.Cd = pC; d.CdT = pCT; d.mixed = pm; d.mixed_t = pmt;
         d.colsum = ws->colsum;
-        const size_t n2d = (size_t)N * N;
-        if ((rc = dev_alloc(ws, (void**)&d.X, 6 * n2d * sizeof(double)))) return fail(rc);
-        if ((rc = dev_alloc(ws, (void**)&d.Y, 6 * n2d * sizeof(double)))) return fail(rc);
         if ((rc = dev_alloc(ws, (void**)&d.mean, sizeof(double)))) return fail(rc);
+        if (d.prec == DENSE_F64) {
+            const size_t n2d = (size_t)N * N;
+            if ((rc = dev_alloc(ws, (void**)&d.X, 6 * n2d * sizeof(double)))) return fail(rc);
+            if ((rc = dev_alloc(ws, (void**)&d.Y, 6 * n2d * sizeof(double)))) return fail(rc);
+        } else {
+            // hi/lo BF16 planes of the two matrices, zero padded to Np x Np
+            const int Np = (N + 63) / 64 * 64;
+            d.Np = Np;
+            const size_t np2 = (size_t)Np * Np;
+            std::vector<uint16_t> sh(np2, 0), sl(np2, 0), ch(np2, 0), cl(np2, 0);
+            for (int k = 0; k < M; ++k)
+                for (int n = 0; n < M; ++n)
+                    split_bf16(Sd[(size_t)k * M + n], sh[(size_t)k * Np + n], sl[(size_t)k * Np + n]);
+            for (int k = 0; k < N; ++k)
+                for (int n = 0; n < N; ++n)
+                    split_bf16(Cd[(size_t)k * N + n], ch[(size_t)k * Np + n], cl[(size_t)k * Np + n]);
+            uint16_t *p0, *p1, *p2, *p3;
+            if ((rc = upload(ws, &p0, sh)) || (rc = upload(ws, &p1, sl)) ||
+                (rc = upload(ws, &p2, ch)) || (rc = upload(ws, &p3, cl)))
+                return fail(rc);
+            d.Sh = p0; d.Sl = p1; d.Ch = p2; d.Cl = p3;
+            uint16_t** planes[4] = {&d.Xh, &d.Xl, &d.Ph, &d.Pl};
+            for (uint16_t** pl : planes) {
+                if ((rc = dev_alloc(ws, (void**)pl, 6 * np2 * sizeof(uint16_t)))) return fail(rc);
+                LCODE_HIP(hipMemset(*pl, 0, 6 * np2 * sizeof(uint16_t)));
+            }
+        }
     }
 
     const size_t n2 = (size_t)N * N, np = (size_t)cfg->nc * cfg->nc;
@@ -466,6 +530,15 @@ int lcode_profile_end(lcode_workspace* ws, hipStream_t stream,
     *n_out = n;
     profiler_free(p);
     return LCODE_OK;
+}
+
+int lcode_diag_zn(lcode_workspace* ws, const lcode_config* cfg, const double* ro,
+                  double* zn_out, hipStream_t stream) {
+    int rc = check_ws(ws, cfg);
+    if (rc) return rc;
+    LCODE_REQUIRE(ro && zn_out, "NULL array");
+    return launch_diag_zn(ro, ws->diag_w, ws->diag_r, ws->N, ws->diag_tmp,
+                          ws->diag_sum, zn_out, stream, &ws->prof);
 }
 
 int lcode_workspace_fault_count(lcode_workspace* ws, hipStream_t stream,

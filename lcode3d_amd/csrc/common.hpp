@@ -350,6 +350,12 @@ struct DenseGeom {
     const double *mixed, *mixed_t;   // eigenvalue tables [M][N] and [N][M]
     double *X, *Y;             // 6 N^2 scratch each
     double* mean;              // Bz sum
+    // DENSE_BF16X3: BF16 hi/lo planes, [Np][Np] each, zero padded
+    int Np;                    // N rounded up to a multiple of 64
+    const uint16_t *Sh, *Sl;   // DST-I matrix
+    const uint16_t *Ch, *Cl;   // DCT-I matrix
+    uint16_t *Xh, *Xl;         // 6 plane pairs: RHS / P2, stored [column][row]
+    uint16_t *Ph, *Pl;         // 6 plane pairs: P1 / P3, stored [row][column]
 };
 
 int launch_gemm(const GemmBatch& b, int prec, hipStream_t s, Profiler* prof,
@@ -358,6 +364,10 @@ int launch_dense_rhs(const DenseRhsArgs& a, int nfields, int max_total,
                      hipStream_t s, Profiler* prof);
 int launch_solve_dense(int N, const DenseGeom& d, const SolveBatch& b,
                        hipStream_t s, Profiler* prof);
+
+// zn diagnostic: Gaussian taps w[0..2r], scratch grid tmp, sum scratch.
+int launch_diag_zn(const double* ro, const double* w, int r, int N, double* tmp,
+                   double* sum, double* out, hipStream_t s, Profiler* prof);
 
 struct SolverGeom {
     int N;

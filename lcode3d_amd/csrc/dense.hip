@@ -218,6 +218,376 @@ dense_rhs_kernel(DenseRhsArgs a) {
     a.R[f][e] = rhs_value_dev(fp.kind, fp.rowsA0 + r, fp.colA0 + c, a.N, a.in);
 }
 
+
+// ---------------------------------------------------------------------------
+// DENSE_BF16X3 on pre-split planes.  Every operand lives in HBM as a pair of
+// BF16 planes (hi, lo) of a zero-padded Np x Np array (Np = N rounded up to
+// 64), in the layout its consumer reads with 16-byte loads: an A operand
+// row-major (k contiguous), a B operand column by column (k contiguous).  The
+// producing kernel writes that layout (and zeroes the padding), so the main
+// loop has no bounds checks and no conversions:
+//   RHS kernel  -> X = R^T planes        G1: P  = T_B R          (rows)
+//   G2: X = ((P T_A^T) o table)^T        G3: P  = T_B P2         (rows)
+//   G4: field = P T_A^T, FP64 epilogue on the N x N grid.
+// Block tile 64 x 64 (four waves of 32 x 32), K step 32, LDS double buffer.
+// ---------------------------------------------------------------------------
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+enum BfMode { BF_ROWS = 0, BF_TAB_COLS = 1, BF_FINAL = 2 };
+
+struct BfOp {
+    const uint16_t *Ah, *Al;   // A(r, k) = A[r ld + k]
+    const uint16_t *Bh, *Bl;   // B(k, c) = B[c ld + k]
+    uint16_t *Ch, *Cl;         // BF_ROWS: C[r ld + c]; BF_TAB_COLS: C[c ld + r]
+    int ld;                    // Np
+    int M, N;                  // valid output extent
+    int Mp, Npad;              // padded output extent (multiples of 64)
+    int K;                     // padded reduction length (multiple of 32)
+    int mode;
+    const double* tab; int ldt;    // BF_TAB_COLS: eigenvalue table [M][N]
+    const double* colsum;          // ... and, for Bz, the mean sum
+    double* mean_acc;
+    int row0, col0, Ngrid;         // BF_FINAL: placement on the grid
+    const double* mean_in;
+    int variantA;
+    const double* prevF;
+    double *outF, *outAvg;
+    const double* guess;
+};
+
+struct BfBatch {
+    int n;
+    BfOp op[6];
+    Gate gate;
+    double* resid;
+};
+
+__device__ __forceinline__ uint16_t bf_bits(__bf16 v) {
+    return __builtin_bit_cast(uint16_t, v);
+}
+
+// hi = RNE bf16 of v, lo = RNE bf16 of the remainder: |v - hi - lo| <= 2^-18 |v|
+__device__ __forceinline__ void bf_split(double v, uint16_t& h, uint16_t& l) {
+    const __bf16 hi = (__bf16)(float)v;
+    const __bf16 lo = (__bf16)(float)(v - (double)(float)hi);
+    h = bf_bits(hi);
+    l = bf_bits(lo);
+}
+
+// Finished field value F at grid cell idx (the last pass of the FFT
+// solver does the same): Bz mean, Variant A, outputs, guided residual.
+__device__ __forceinline__ void final_cell(const BfOp& o, size_t idx, double F,
+                                           EpiAcc& acc) {
+    if (o.mean_in) F -= *o.mean_in / ((double)o.Ngrid * o.Ngrid);   // lcode.py:362
+    if (o.variantA) F = 2 * F - o.prevF[idx];                       // lcode.py:869-871
+    if (o.outF) o.outF[idx] = F;
+    if (o.outAvg) o.outAvg[idx] = (F + o.prevF[idx]) / 2;           // lcode.py:876-881
+    if (o.guess) {
+        const double d = F - o.guess[idx];
+        acc.rn += d * d;
+        acc.rd += F * F;
+    }
+}
+
+constexpr int kBfLds = 40;   // LDS row stride in bf16 elements (32 + 8)
+
+__global__ void __launch_bounds__(256)
+bf_gemm_kernel(BfBatch batch) {
+    if (!gate_open(batch.gate)) return;
+    const BfOp& o = batch.op[blockIdx.z];
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    if (m0 >= o.Mp || n0 >= o.Npad) return;
+    __shared__ __align__(16) uint16_t sA[2][2][64][kBfLds];   // [stage][hi/lo]
+    __shared__ __align__(16) uint16_t sB[2][2][64][kBfLds];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    // global -> LDS: 64 rows x 32 k per plane, one 16-byte piece per thread
+    const int lr = tid >> 2, lc = (tid & 3) * 8;
+    const size_t offA = (size_t)(m0 + lr) * o.ld + lc;
+    const size_t offB = (size_t)(n0 + lr) * o.ld + lc;
+    uint4 rAh = *reinterpret_cast<const uint4*>(o.Ah + offA);
+    uint4 rAl = *reinterpret_cast<const uint4*>(o.Al + offA);
+    uint4 rBh = *reinterpret_cast<const uint4*>(o.Bh + offB);
+    uint4 rBl = *reinterpret_cast<const uint4*>(o.Bl + offB);
+    *reinterpret_cast<uint4*>(&sA[0][0][lr][lc]) = rAh;
+    *reinterpret_cast<uint4*>(&sA[0][1][lr][lc]) = rAl;
+    *reinterpret_cast<uint4*>(&sB[0][0][lr][lc]) = rBh;
+    *reinterpret_cast<uint4*>(&sB[0][1][lr][lc]) = rBl;
+    __syncthreads();
+    v4f acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    const int nks = o.K / 32;
+    for (int ks = 0; ks < nks; ++ks) {
+        const int cur = ks & 1;
+        const bool more = ks + 1 < nks;
+        if (more) {
+            const int k1 = (ks + 1) * 32;
+            rAh = *reinterpret_cast<const uint4*>(o.Ah + offA + k1);
+            rAl = *reinterpret_cast<const uint4*>(o.Al + offA + k1);
+            rBh = *reinterpret_cast<const uint4*>(o.Bh + offB + k1);
+            rBl = *reinterpret_cast<const uint4*>(o.Bl + offB + k1);
+        }
+        bf8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            ah[t] = *reinterpret_cast<const bf8*>(&sA[cur][0][wm + t * 16 + li][8 * lk]);
+            al[t] = *reinterpret_cast<const bf8*>(&sA[cur][1][wm + t * 16 + li][8 * lk]);
+            bh[t] = *reinterpret_cast<const bf8*>(&sB[cur][0][wn + t * 16 + li][8 * lk]);
+            bl[t] = *reinterpret_cast<const bf8*>(&sB[cur][1][wn + t * 16 + li][8 * lk]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+            }
+        if (more) {
+            const int nxt = cur ^ 1;
+            *reinterpret_cast<uint4*>(&sA[nxt][0][lr][lc]) = rAh;
+            *reinterpret_cast<uint4*>(&sA[nxt][1][lr][lc]) = rAl;
+            *reinterpret_cast<uint4*>(&sB[nxt][0][lr][lc]) = rBh;
+            *reinterpret_cast<uint4*>(&sB[nxt][1][lr][lc]) = rBl;
+        }
+        __syncthreads();
+    }
+
+    EpiAcc ea{0., 0., 0.};
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int r0 = m0 + wm + a * 16 + 4 * lk;   // rows r0 .. r0 + 3
+            const int c = n0 + wn + b * 16 + li;
+            if (o.mode == BF_ROWS) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = r0 + q;
+                    uint16_t h = 0, l = 0;
+                    if (r < o.M && c < o.N) bf_split((double)acc[a][b][q], h, l);
+                    o.Ch[(size_t)r * o.ld + c] = h;
+                    o.Cl[(size_t)r * o.ld + c] = l;
+                }
+            } else if (o.mode == BF_TAB_COLS) {
+                uint16_t h[4], l[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = r0 + q;
+                    h[q] = l[q] = 0;
+                    if (r < o.M && c < o.N) {
+                        const double v = (double)acc[a][b][q] * o.tab[(size_t)r * o.ldt + c];
+                        if (o.mean_acc) ea.mean += o.colsum[r] * v * o.colsum[c];
+                        bf_split(v, h[q], l[q]);
+                    }
+                }
+                const size_t at = (size_t)c * o.ld + r0;
+                *reinterpret_cast<ushort4*>(o.Ch + at) = make_ushort4(h[0], h[1], h[2], h[3]);
+                *reinterpret_cast<ushort4*>(o.Cl + at) = make_ushort4(l[0], l[1], l[2], l[3]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = r0 + q;
+                    if (r < o.M && c < o.N)
+                        final_cell(o, (size_t)(o.row0 + r) * o.Ngrid + o.col0 + c,
+                                   (double)acc[a][b][q], ea);
+                }
+            }
+        }
+    const int wlane = tid & 63;
+    if (o.mode == BF_TAB_COLS && o.mean_acc) {
+        const double m = wave_sum64(ea.mean);
+        if (wlane == 0) atomicAdd(o.mean_acc, m);
+    }
+    if (o.mode == BF_FINAL && o.guess && batch.resid) {
+        const double rn = wave_sum64(ea.rn), rd = wave_sum64(ea.rd);
+        if (wlane == 0) {
+            atomicAdd(&batch.resid[0], rn);
+            atomicAdd(&batch.resid[1], rd);
+        }
+    }
+}
+
+// RHS of each field, split and stored transposed (X[c][r] for the RHS
+// value at grid (rowsA0 + r, colA0 + c)) through a 32 x 32 LDS tile, zero
+// in the padding; then the grid cells outside the field's data region (the
+// zero rim of Ez, Ex, Ey, Bx, By) get the final-pass treatment of F = 0.
+struct BfRhsArgs {
+    FieldPlan f[6];
+    RhsInputs in;
+    int N, Np;
+    uint16_t *Xh[6], *Xl[6];
+    double* mean_zero;
+    Gate gate;
+    int variantA;
+    const double* guess[6];
+    double* resid;
+};
+
+__global__ void __launch_bounds__(256)
+bf_rhs_kernel(BfRhsArgs a) {
+    if (!gate_open(a.gate)) return;
+    __shared__ double tile[32][33];
+    const int f = blockIdx.y;
+    const FieldPlan& fp = a.f[f];
+    const int tiles_side = a.Np / 32;
+    const int ntiles = tiles_side * tiles_side;
+    const int tid = threadIdx.x;
+    if (f == 0 && blockIdx.x == 0 && tid == 0 && a.mean_zero) *a.mean_zero = 0.;
+    if ((int)blockIdx.x < ntiles) {
+        const int r0 = (blockIdx.x / tiles_side) * 32, c0 = (blockIdx.x % tiles_side) * 32;
+        const int tx = tid & 31, ty = tid >> 5;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int r = r0 + ty + 8 * m, c = c0 + tx;
+            tile[ty + 8 * m][tx] = (r < fp.rowsA && c < fp.lenA)
+                ? rhs_value_dev(fp.kind, fp.rowsA0 + r, fp.colA0 + c, a.N, a.in) : 0.;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int c = c0 + ty + 8 * m, r = r0 + tx;
+            uint16_t h, l;
+            bf_split(tile[tx][ty + 8 * m], h, l);
+            a.Xh[f][(size_t)c * a.Np + r] = h;
+            a.Xl[f][(size_t)c * a.Np + r] = l;
+        }
+        return;
+    }
+    // rim: cells of the four grid edges outside the data region
+    const int N = a.N;
+    const int e = ((int)blockIdx.x - ntiles) * 256 + tid;
+    double rn = 0., rd = 0.;
+    if (e < 4 * N) {
+        const int side = e / N, t = e - side * N;
+        int i, j;
+        bool ok = true;
+        if (side < 2) { i = side ? N - 1 : 0; j = t; }
+        else { i = t; j = side == 3 ? N - 1 : 0; ok = t > 0 && t < N - 1; }
+        const bool inside = i >= fp.rowsA0 && i < fp.rowsA0 + fp.rowsA &&
+                            j >= fp.colA0 && j < fp.colA0 + fp.lenA;
+        if (ok && !inside) {
+            const size_t idx = (size_t)i * N + j;
+            const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
+                                    fp.kind == F_BX || fp.kind == F_BY;
+            double F = 0.;
+            if (a.variantA && transverse) F = -fp.prevF[idx];   // 2 * 0 - prev
+            if (fp.outF) fp.outF[idx] = F;
+            if (fp.outAvg) fp.outAvg[idx] = (F + fp.prevF[idx]) / 2;
+            const double* g = a.resid ? a.guess[fp.kind] : nullptr;
+            if (g) {
+                const double d = F - g[idx];
+                rn = d * d;
+                rd = F * F;
+            }
+        }
+    }
+    if (a.resid) {
+        rn = wave_sum64(rn);
+        rd = wave_sum64(rd);
+        if ((tid & 63) == 0 && (rn != 0. || rd != 0.)) {
+            atomicAdd(&a.resid[0], rn);
+            atomicAdd(&a.resid[1], rd);
+        }
+    }
+}
+
+int launch_solve_bf16x3(int N, const DenseGeom& d, const SolveBatch& b,
+                        hipStream_t s, Profiler* prof) {
+    const int Np = d.Np;
+    const size_t np2 = (size_t)Np * Np;
+    auto r64 = [](int x) { return (x + 63) / 64 * 64; };
+    BfRhsArgs ra{};
+    ra.in = b.in;
+    ra.N = N;
+    ra.Np = Np;
+    ra.mean_zero = d.mean;
+    ra.gate = b.gate;
+    ra.variantA = b.variantA;
+    ra.resid = b.resid;
+    for (int k = 0; k < 6; ++k) ra.guess[k] = b.guess[k];
+    BfBatch g1{}, g2{}, g3{}, g4{};
+    g1.n = g2.n = g3.n = g4.n = b.nfields;
+    g1.gate = g2.gate = g3.gate = g4.gate = b.gate;
+    g4.resid = b.resid;
+    int maxM = 0, maxN = 0;
+    for (int f = 0; f < b.nfields; ++f) {
+        const FieldPlan& fp = b.f[f];
+        const int R = fp.rowsA, C = fp.lenA, Rp = r64(R), Cp = r64(C);
+        maxM = maxM > Rp ? maxM : Rp;
+        maxN = maxN > Cp ? maxN : Cp;
+        ra.f[f] = fp;
+        ra.Xh[f] = d.Xh + f * np2;
+        ra.Xl[f] = d.Xl + f * np2;
+        const bool dstB = fp.xfB == XF_DST, dstA = fp.xfA == XF_DST;
+        const uint16_t* TBh = dstB ? d.Sh : d.Ch;
+        const uint16_t* TBl = dstB ? d.Sl : d.Cl;
+        const uint16_t* TAh = dstA ? d.Sh : d.Ch;
+        const uint16_t* TAl = dstA ? d.Sl : d.Cl;
+        uint16_t *Xh = d.Xh + f * np2, *Xl = d.Xl + f * np2;
+        uint16_t *Ph = d.Ph + f * np2, *Pl = d.Pl + f * np2;
+        BfOp o{};
+        o.ld = Np; o.M = R; o.N = C; o.Mp = Rp; o.Npad = Cp;
+        // G1: P = T_B R
+        o.Ah = TBh; o.Al = TBl; o.Bh = Xh; o.Bl = Xl; o.K = Rp;
+        o.mode = BF_ROWS; o.Ch = Ph; o.Cl = Pl;
+        g1.op[f] = o;
+        // G2: X = ((P T_A^T) o table)^T
+        BfOp o2 = o;
+        o2.Ah = Ph; o2.Al = Pl; o2.Bh = TAh; o2.Bl = TAl; o2.K = Cp;
+        o2.mode = BF_TAB_COLS; o2.Ch = Xh; o2.Cl = Xl;
+        o2.tab = fp.kind == F_EY || fp.kind == F_BX ? d.mixed
+               : fp.kind == F_EX || fp.kind == F_BY ? d.mixed_t
+               : static_cast<const double*>(fp.table);
+        o2.ldt = C;
+        if (fp.kind == F_BZ) { o2.colsum = d.colsum; o2.mean_acc = d.mean; }
+        g2.op[f] = o2;
+        // G3: P = T_B X
+        BfOp o3 = o;
+        o3.Ah = TBh; o3.Al = TBl; o3.Bh = Xh; o3.Bl = Xl; o3.K = Rp;
+        o3.mode = BF_ROWS; o3.Ch = Ph; o3.Cl = Pl;
+        g3.op[f] = o3;
+        // G4: field = P T_A^T on the grid
+        BfOp o4 = o;
+        o4.Ah = Ph; o4.Al = Pl; o4.Bh = TAh; o4.Bl = TAl; o4.K = Cp;
+        o4.mode = BF_FINAL;
+        o4.row0 = fp.rowsA0; o4.col0 = fp.colA0; o4.Ngrid = N;
+        if (fp.kind == F_BZ) o4.mean_in = d.mean;
+        const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
+                                fp.kind == F_BX || fp.kind == F_BY;
+        o4.variantA = b.variantA && transverse;
+        o4.prevF = fp.prevF;
+        o4.outF = fp.outF;
+        o4.outAvg = fp.outAvg;
+        o4.guess = b.resid ? b.guess[fp.kind] : nullptr;
+        g4.op[f] = o4;
+    }
+    const int tiles = (Np / 32) * (Np / 32);
+    const dim3 grid_rhs(tiles + (4 * N + 255) / 256, b.nfields);
+    const dim3 grid_mm(maxN / 64, maxM / 64, b.nfields);
+    {
+        ProfScope ps(prof, "dense_rhs", s);
+        bf_rhs_kernel<<<grid_rhs, 256, 0, s>>>(ra);
+    }
+    LCODE_HIP(hipGetLastError());
+    const BfBatch* gs[4] = {&g1, &g2, &g3, &g4};
+    const char* names[4] = {"dense_gemm_1", "dense_gemm_2", "dense_gemm_3", "dense_gemm_4"};
+    for (int k = 0; k < 4; ++k) {
+        {
+            ProfScope ps(prof, names[k], s);
+            bf_gemm_kernel<<<grid_mm, 256, 0, s>>>(*gs[k]);
+        }
+        LCODE_HIP(hipGetLastError());
+    }
+    return LCODE_OK;
+}
+
 }  // namespace
 
 int launch_gemm(const GemmBatch& b, int prec, hipStream_t s, Profiler* prof,
@@ -253,6 +623,7 @@ int launch_dense_rhs(const DenseRhsArgs& a, int nfields, int max_total,
 int launch_solve_dense(int N, const DenseGeom& d, const SolveBatch& b,
                        hipStream_t s, Profiler* prof) {
     if (b.nfields == 0) return LCODE_OK;
+    if (d.prec == DENSE_BF16X3) return launch_solve_bf16x3(N, d, b, s, prof);
     const size_t n2 = (size_t)N * N;
     DenseRhsArgs ra{};
     ra.in = b.in;
