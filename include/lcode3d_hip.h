@@ -23,6 +23,10 @@
  *   lcode_calculate_Ez              - calculate_Ez (lcode.py:160)
  *   lcode_calculate_Bz              - calculate_Bz (lcode.py:333)
  *   lcode_step                      - step (lcode.py:828)
+ *   lcode_step_guided(_residual)    - step with its predictor pass
+ *                                      (lcode.py:841-881) replaced by a
+ *                                      supplied field guess
+ *   lcode_diag_zn                   - diags_ro_zn (lcode.py:981)
  *
  * Grids are (N, N) with [i, j] <-> (x, y) = ((i - N/2) h, (j - N/2) h)
  * (lcode.py:951-953).  Coarse particle arrays hold nc*nc values, the

@@ -48,7 +48,8 @@ struct lcode_workspace {
     double* favg[6] = {};          // field averages of the current pass
     int* fault = nullptr;
     int* spill = nullptr;          // nf * nf spilled fine particle indices
-    int* spill_ctr = nullptr;      // [0] count, [1] blocks done
+    int* spill_ctr = nullptr;      // [0] count, [1] blocks done, [2] pushes'
+                                   // displacement bound in cells
     double* guess[6] = {};         // guided step: the field guess
     double* resid = nullptr;       // {sum (F - guess)^2, sum F^2}
     double* diag_w = nullptr;      // zn diagnostic: Gaussian taps [2 r + 1]
@@ -454,9 +455,9 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     if ((rc = dev_alloc(ws, (void**)&ws->fault, sizeof(int)))) return fail(rc);
     const size_t nfp = (size_t)cfg->nf * cfg->nf;
     if ((rc = dev_alloc(ws, (void**)&ws->spill, (nfp ? nfp : 1) * sizeof(int)))) return fail(rc);
-    if ((rc = dev_alloc(ws, (void**)&ws->spill_ctr, 2 * sizeof(int)))) return fail(rc);
+    if ((rc = dev_alloc(ws, (void**)&ws->spill_ctr, 3 * sizeof(int)))) return fail(rc);
     if (hipMemset(ws->fault, 0, sizeof(int)) != hipSuccess ||
-        hipMemset(ws->spill_ctr, 0, 2 * sizeof(int)) != hipSuccess ||
+        hipMemset(ws->spill_ctr, 0, 3 * sizeof(int)) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
         set_error("hipMemset of the workspace counters failed");
         return fail(LCODE_E_HIP);
@@ -598,6 +599,7 @@ static int do_push(lcode_workspace* ws, const lcode_config* cfg,
     a.out_x = out[0]; a.out_y = out[1];
     a.out_px = out[2]; a.out_py = out[3]; a.out_pz = out[4];
     a.fault = ws->fault;
+    a.max_cells = ws->spill_ctr + 2;
     a.gate = gate;
     return launch_push(a, stream, &ws->prof);
 }
@@ -628,7 +630,8 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
                       const double* yo, const double* m, const double* q,
                       const double* px, const double* py, const double* pz,
                       const lcode_virt_params* vp, double* const out[4],
-                      hipStream_t stream, Gate gate = Gate{nullptr, 0.}) {
+                      hipStream_t stream, Gate gate = Gate{nullptr, 0.},
+                      bool adaptive_halo = false) {
     DepositArgs a{};
     a.nf = cfg->nf;
     a.nc = cfg->nc;
@@ -647,6 +650,8 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.spill_count = ws->spill_ctr;
     a.spill_done = ws->spill_ctr + 1;
     a.halo = kDepositHalo;
+    // after a push of this workspace: a halo of the pushes' displacement
+    a.halo_cells = adaptive_halo ? ws->spill_ctr + 2 : nullptr;
     // the tile's coarse rectangle has at most ext / coarseness + 3 rows
     a.staged = (32 + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= 18;
     a.gate = gate;
@@ -791,7 +796,7 @@ int lcode_step(lcode_workspace* ws, const lcode_config* cfg,
     // 3: deposit (lcode.py:856-859)
     if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p1[0], ws->p1[1], c->m,
                          c->q, ws->p1[2], ws->p1[3], ws->p1[4], vp, ws->dep,
-                         stream)))
+                         stream, Gate{nullptr, 0.}, true)))
         return rc;
     // 4: fields with prev fields in the trick term; only the averages are
     // used afterwards (lcode.py:862-881)
@@ -810,7 +815,7 @@ int lcode_step(lcode_workspace* ws, const lcode_config* cfg,
     // 6: deposit (lcode.py:890-891)
     if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p2[0], ws->p2[1], c->m,
                          c->q, ws->p2[2], ws->p2[3], ws->p2[4], vp, ws->dep,
-                         stream)))
+                         stream, Gate{nullptr, 0.}, true)))
         return rc;
     // 7: fields with the pass-1 averages in the trick term; the solved
     // fields are the step's output (lcode.py:893-911, :927-928).  The
@@ -833,7 +838,8 @@ int lcode_step(lcode_workspace* ws, const lcode_config* cfg,
     // 9: final deposit into the output (lcode.py:920-921)
     double* const outD[4] = {out->ro, out->jx, out->jy, out->jz};
     return do_deposit(ws, cfg, c->ro_initial, out->x_offt, out->y_offt, c->m,
-                      c->q, out->px, out->py, out->pz, vp, outD, stream);
+                      c->q, out->px, out->py, out->pz, vp, outD, stream,
+                      Gate{nullptr, 0.}, true);
 }
 
 int lcode_step_guided(lcode_workspace* ws, const lcode_config* cfg,
@@ -891,14 +897,16 @@ int lcode_step_guided(lcode_workspace* ws, const lcode_config* cfg,
         return rc;
     if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p2[0], ws->p2[1], c->m,
                          c->q, ws->p2[2], ws->p2[3], ws->p2[4], vp, ws->dep,
-                         stream)))
+                         stream, Gate{nullptr, 0.}, true)))
         return rc;
     if ((rc = field_pass(ws, cfg, prev, trick_avg, beam_ro, beam_scalar,
                          ws->dep, outF, ws->favg, stream, Gate{nullptr, 0.},
                          g6, ws->resid)))
         return rc;
     // fallback: the plain predictor and corrector passes, gated on the
-    // residual (lcode.py:856-911)
+    // residual (lcode.py:856-911).  The predictor particles' displacement
+    // bound was consumed by the corrector deposition above, so their
+    // deposition uses the full static halo.
     if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p1[0], ws->p1[1], c->m,
                          c->q, ws->p1[2], ws->p1[3], ws->p1[4], vp, ws->dep,
                          stream, fallback)))
@@ -913,7 +921,7 @@ int lcode_step_guided(lcode_workspace* ws, const lcode_config* cfg,
         return rc;
     if ((rc = do_deposit(ws, cfg, c->ro_initial, ws->p2[0], ws->p2[1], c->m,
                          c->q, ws->p2[2], ws->p2[3], ws->p2[4], vp, ws->dep,
-                         stream, fallback)))
+                         stream, fallback, true)))
         return rc;
     if ((rc = field_pass(ws, cfg, prev, trick_avg, beam_ro, beam_scalar,
                          ws->dep, outF, ws->favg, stream, fallback)))
@@ -927,7 +935,8 @@ int lcode_step_guided(lcode_workspace* ws, const lcode_config* cfg,
         return rc;
     double* const outD[4] = {out->ro, out->jx, out->jy, out->jz};
     return do_deposit(ws, cfg, c->ro_initial, out->x_offt, out->y_offt, c->m,
-                      c->q, out->px, out->py, out->pz, vp, outD, stream);
+                      c->q, out->px, out->py, out->pz, vp, outD, stream,
+                      Gate{nullptr, 0.}, true);
 }
 
 int lcode_step_guided_residual(lcode_workspace* ws, hipStream_t stream,

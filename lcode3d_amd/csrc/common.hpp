@@ -156,6 +156,7 @@ struct PushArgs {
     const double *F[6];            // Ex Ey Ez Bx By Bz (averaged)
     double *out_x, *out_y, *out_px, *out_py, *out_pz;
     int* fault;
+    int* max_cells;                // atomic max of ceil(|offset| / h), or NULL
     Gate gate;
 };
 
@@ -181,7 +182,9 @@ struct DepositArgs {
     int* spill;                    // nf * nf capacity
     int* spill_count;              // zero between calls
     int* spill_done;               // zero between calls
-    int halo;                      // tile halo in cells
+    int halo;                      // tile halo in cells (maximum)
+    int* halo_cells;               // NULL, or the pushes' displacement bound
+                                   // (read, then reset to 0 by the last block)
     int staged;                    // coarse rectangle of a tile fits LDS
     Gate gate;
 };

@@ -50,10 +50,8 @@ estimate_kernel(EstimateArgs a) {
 // One-wave blocks spread the (few) coarse particles over all CUs.
 constexpr int kPushBlock = 64;
 
-__global__ void __launch_bounds__(kPushBlock)
-push_kernel(PushArgs a) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.n || !gate_open(a.gate)) return;
+// One coarse particle's push; returns max(|x_offt|, |y_offt|) after it.
+__device__ __forceinline__ double push_one(const PushArgs& a, int k) {
     const double m = a.m[k], q = a.q[k];
     const double opx = a.px[k], opy = a.py[k], opz = a.pz[k];
     const double xi0 = a.x_init[k], yi0 = a.y_init[k];
@@ -110,6 +108,23 @@ push_kernel(PushArgs a) {
     a.out_px[k] = px;
     a.out_py[k] = py;
     a.out_pz[k] = pz;
+    return fmax(fabs(x_offt), fabs(y_offt));
+}
+
+__global__ void __launch_bounds__(kPushBlock)
+push_kernel(PushArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.;
+    if (k < a.n && gate_open(a.gate)) d = push_one(a, k);
+    if (a.max_cells) {
+        // cells the particles moved from their initial positions, at most
+        // (the deposition sizes its tile halo by it); NaN counts as far
+        const double c = ceil(d / a.h);
+        int cells = c < 1e6 ? (int)c : (1 << 20);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) cells = max(cells, __shfl_xor(cells, off, 64));
+        if ((threadIdx.x & 63) == 0) atomicMax(a.max_cells, cells);
+    }
 }
 
 // Deposition values of a fine particle with mass m, charge q, momentum p
@@ -256,8 +271,11 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     const int N = a.N;
     const double inv_h = 1. / a.h;
     const int ci0 = blockIdx.y * kTile, cj0 = blockIdx.x * kTile;
-    const int ext = kTile + 2 * a.halo + 2;
-    const int ci_lo = ci0 - 1 - a.halo, cj_lo = cj0 - 1 - a.halo;
+    // halo: the displacement bound of the last push when one is given
+    // (particles beyond it take the exact spill path either way)
+    const int halo = a.halo_cells ? min(max(*a.halo_cells, 0), a.halo) : a.halo;
+    const int ext = kTile + 2 * halo + 2;
+    const int ci_lo = ci0 - 1 - halo, cj_lo = cj0 - 1 - halo;
     for (int k = threadIdx.x; k < 4 * kTile * kTile; k += blockDim.x)
         (&acc[0][0][0])[k] = 0.;
     {
@@ -347,7 +365,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             }
             Tsc t;
             tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
-            if (abs(t.i - ci) > a.halo || abs(t.j - cj) > a.halo) {
+            if (abs(t.i - ci) > halo || abs(t.j - cj) > halo) {
                 if (home)
                     a.spill[atomicAdd(a.spill_count, 1)] = (fi_a + li) * a.nf + fj_a + lj;
                 continue;
@@ -392,6 +410,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     if (threadIdx.x == 0) {
         atomicExch(a.spill_count, 0);
         atomicExch(a.spill_done, 0);
+        if (a.halo_cells) atomicExch(a.halo_cells, 0);   // for the next push
     }
 }
 
