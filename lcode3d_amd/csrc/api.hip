@@ -414,7 +414,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
             if ((rc = dev_alloc(ws, (void**)&d.Y, 6 * n2d * sizeof(double)))) return fail(rc);
         } else {
             // hi/lo BF16 planes of the two matrices, zero padded to Np x Np
-            const int Np = (N + 63) / 64 * 64;
+            const int Np = (N + 127) / 128 * 128;
             d.Np = Np;
             const size_t np2 = (size_t)Np * Np;
             std::vector<uint16_t> sh(np2, 0), sl(np2, 0), ch(np2, 0), cl(np2, 0);

@@ -82,6 +82,16 @@ __device__ __forceinline__ bool gate_open(const Gate& g) {
     return !(num <= g.thr2 * den);
 }
 
+// XCD-aware block order.  Workgroups go to the 8 XCDs round-robin by linear
+// id and each XCD has its own L2: map linear id `lin` of `total` to a
+// logical id such that every XCD gets one contiguous range of logical ids.
+__device__ __forceinline__ int xcd_logical(int lin, int total) {
+    constexpr int kXcd = 8;
+    const int q = total / kXcd, r = total % kXcd;
+    const int xcd = lin % kXcd, slot = lin / kXcd;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
 // ---------------------------------------------------------------------------
 // TSC-2D shape (lcode.py:396-418)
 // ---------------------------------------------------------------------------
@@ -354,14 +364,14 @@ struct DenseGeom {
     double *X, *Y;             // 6 N^2 scratch each
     double* mean;              // Bz sum
     // DENSE_BF16X3: BF16 hi/lo planes, [Np][Np] each, zero padded
-    int Np;                    // N rounded up to a multiple of 64
+    int Np;                    // N rounded up to a multiple of 128
     const uint16_t *Sh, *Sl;   // DST-I matrix
     const uint16_t *Ch, *Cl;   // DCT-I matrix
-    uint16_t *Xh, *Xl;         // 6 plane pairs: RHS / P2, stored [column][row]
-    uint16_t *Ph, *Pl;         // 6 plane pairs: P1 / P3, stored [row][column]
+    uint16_t *Xh, *Xl;         // 6 plane pairs (one per field kind): RHS, then P, by rows
+    uint16_t *Ph, *Pl;         // 6 plane pairs: Q, by columns
 };
 
-int launch_gemm(const GemmBatch& b, int prec, hipStream_t s, Profiler* prof,
+int launch_gemm(const GemmBatch& b, hipStream_t s, Profiler* prof,
                 const char* name);
 int launch_dense_rhs(const DenseRhsArgs& a, int nfields, int max_total,
                      hipStream_t s, Profiler* prof);

@@ -5,9 +5,9 @@
 // along x and y,
 //   P1 = T_x R,  P2 = (P1 T_y^T) o Mat,  P3 = T_x P2,  out = P3 T_y^T,
 // batched over the six fields.  Two arithmetic forms:
-//   DENSE_F64    v_mfma_f64_16x16x4_f64 on FP64 data (the plain path);
+//   DENSE_F64    v_mfma_f64_16x16x4_f64 on FP64 data;
 //   DENSE_BF16X3 v_mfma_f32_16x16x32_bf16 on hi/lo bf16 splits of both
-//                operands, three products (hi hi + hi lo + lo hi) accumulated
+//                operands, three products (lo hi + hi lo + hi hi) accumulated
 //                in FP32; the RHS, Bz zero mode and mean, Variant A and the
 //                averages stay FP64.
 // Tiles: 64 x 64 outputs per 256-thread block, four waves of 32 x 32
@@ -21,7 +21,6 @@ namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr int kBM = 64, kBN = 64;
 
@@ -76,7 +75,7 @@ __device__ __forceinline__ double wave_sum64(double v) {
     return v;
 }
 
-template <int PREC>
+// DENSE_F64: FP64 operands with offsets and zero fill, staged through LDS.
 __global__ void __launch_bounds__(256)
 gemm_kernel(GemmBatch batch) {
     if (!gate_open(batch.gate)) return;
@@ -87,7 +86,7 @@ gemm_kernel(GemmBatch batch) {
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
     EpiAcc ea{0., 0., 0.};
-    if constexpr (PREC == DENSE_F64) {
+    {
         constexpr int BK = 16;
         __shared__ double As[kBM][BK + 1];
         __shared__ double Bs[BK][kBN + 1];
@@ -123,71 +122,16 @@ gemm_kernel(GemmBatch batch) {
             }
             __syncthreads();
         }
+        // v_mfma_f64_16x16x4f64 returns D[i][j] in lane j + 16 (i % 4),
+        // item i / 4 (measured on gfx950)
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    epilogue(o, m0 + wm + a * 16 + 4 * lk + q,
+                    epilogue(o, m0 + wm + a * 16 + 4 * q + lk,
                              n0 + wn + b * 16 + li, acc[a][b][q], ea);
-    } else {
-        constexpr int BK = 32;
-        // hi/lo bf16 splits; B stored transposed so fragments are contiguous
-        __shared__ __bf16 Ah[kBM][BK + 8], Al[kBM][BK + 8];
-        __shared__ __bf16 Bh[kBN][BK + 8], Bl[kBN][BK + 8];
-        v4f acc[2][2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < o.K; k0 += BK) {
-            for (int e = threadIdx.x; e < kBM * BK; e += 256) {
-                const int r = e / BK, k = e % BK;
-                const double v = load_a(o, m0 + r, k0 + k);
-                const __bf16 h = (__bf16)(float)v;
-                Ah[r][k] = h;
-                Al[r][k] = (__bf16)(float)(v - (double)(float)h);
-            }
-            for (int e = threadIdx.x; e < BK * kBN; e += 256) {
-                const int k = e / kBN, c = e % kBN;
-                const double v = load_b(o, k0 + k, n0 + c);
-                const __bf16 h = (__bf16)(float)v;
-                Bh[c][k] = h;
-                Bl[c][k] = (__bf16)(float)(v - (double)(float)h);
-            }
-            __syncthreads();
-            v8bf ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    ah[t][q] = Ah[wm + t * 16 + li][8 * lk + q];
-                    al[t][q] = Al[wm + t * 16 + li][8 * lk + q];
-                    bh[t][q] = Bh[wn + t * 16 + li][8 * lk + q];
-                    bl[t][q] = Bl[wn + t * 16 + li][8 * lk + q];
-                }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        al[a], bh[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        ah[a], bl[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        ah[a], bh[b], acc[a][b], 0, 0, 0);
-                }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    epilogue(o, m0 + wm + a * 16 + 4 * lk + q,
-                             n0 + wn + b * 16 + li, (double)acc[a][b][q], ea);
     }
     const int wlane = threadIdx.x & 63;
     if (o.mode == 1 && o.mean_acc) {
@@ -222,30 +166,39 @@ dense_rhs_kernel(DenseRhsArgs a) {
 // ---------------------------------------------------------------------------
 // DENSE_BF16X3 on pre-split planes.  Every operand lives in HBM as a pair of
 // BF16 planes (hi, lo) of a zero-padded Np x Np array (Np = N rounded up to
-// 64), in the layout its consumer reads with 16-byte loads: an A operand
-// row-major (k contiguous), a B operand column by column (k contiguous).  The
-// producing kernel writes that layout (and zeroes the padding), so the main
-// loop has no bounds checks and no conversions:
-//   RHS kernel  -> X = R^T planes        G1: P  = T_B R          (rows)
-//   G2: X = ((P T_A^T) o table)^T        G3: P  = T_B P2         (rows)
-//   G4: field = P T_A^T, FP64 epilogue on the N x N grid.
-// Block tile 64 x 64 (four waves of 32 x 32), K step 32, LDS double buffer.
+// 128), in the layout its consumer reads with 16-byte loads: an A operand
+// row-major (k contiguous), a B operand column by column (k contiguous).
+// The products run in the order that lets the RHS be written row-major:
+//   RHS kernel -> X (rows)              G1: Q = X T_A^T            (columns)
+//   G2: P = (T_B Q) o table  (rows)     G3: Q = P T_A^T            (columns)
+//   G4: field = T_B Q, FP64 epilogue on the N x N grid.
+// Plane slots are per field kind, so each slot always holds one shape and
+// every value outside it stays zero: the main loop has no bounds checks and
+// no conversions.  Block tile (32 TM) x (32 TN), four waves of (16 TM) x
+// (16 TN), K step 32, LDS double buffer.
 // ---------------------------------------------------------------------------
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-enum BfMode { BF_ROWS = 0, BF_TAB_COLS = 1, BF_FINAL = 2 };
+enum BfMode { BF_COLS = 0, BF_TAB_ROWS = 1, BF_FINAL = 2 };
+
+#ifndef LCODE_BF_TM
+#define LCODE_BF_TM 2
+#endif
+#ifndef LCODE_BF_TN
+#define LCODE_BF_TN 2
+#endif
 
 struct BfOp {
     const uint16_t *Ah, *Al;   // A(r, k) = A[r ld + k]
     const uint16_t *Bh, *Bl;   // B(k, c) = B[c ld + k]
-    uint16_t *Ch, *Cl;         // BF_ROWS: C[r ld + c]; BF_TAB_COLS: C[c ld + r]
+    uint16_t *Ch, *Cl;         // BF_COLS: C[c ld + r]; BF_TAB_ROWS: C[r ld + c]
     int ld;                    // Np
     int M, N;                  // valid output extent
-    int Mp, Npad;              // padded output extent (multiples of 64)
+    int Mp, Npad;              // padded output extent (multiples of the tile)
     int K;                     // padded reduction length (multiple of 32)
     int mode;
-    const double* tab; int ldt;    // BF_TAB_COLS: eigenvalue table [M][N]
+    const double* tab; int ldt;    // BF_TAB_ROWS: eigenvalue table [M][N]
     const double* colsum;          // ... and, for Bz, the mean sum
     double* mean_acc;
     int row0, col0, Ngrid;         // BF_FINAL: placement on the grid
@@ -261,6 +214,7 @@ struct BfBatch {
     BfOp op[6];
     Gate gate;
     double* resid;
+    int tiles_m, tiles_n;      // block tiles per field (the largest field)
 };
 
 __device__ __forceinline__ uint16_t bf_bits(__bf16 v) {
@@ -292,104 +246,129 @@ __device__ __forceinline__ void final_cell(const BfOp& o, size_t idx, double F,
 
 constexpr int kBfLds = 40;   // LDS row stride in bf16 elements (32 + 8)
 
+template <int TM, int TN>
 __global__ void __launch_bounds__(256)
 bf_gemm_kernel(BfBatch batch) {
+    constexpr int BM = 32 * TM, BN = 32 * TN;
+    constexpr int PA = BM * 4 / 256, PB = BN * 4 / 256;   // 16-byte pieces per thread
     if (!gate_open(batch.gate)) return;
-    const BfOp& o = batch.op[blockIdx.z];
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    // 1-D grid of n x tiles_m x tiles_n blocks in XCD-aware order: each XCD
+    // takes a contiguous run of one field's tiles, row of tiles by row of
+    // tiles, so the A rows and B columns it reads stay in its L2
+    const int tiles_n = batch.tiles_n, per_field = batch.tiles_m * tiles_n;
+    const int logical = xcd_logical(blockIdx.x, gridDim.x);
+    const int z = logical / per_field, t = logical - z * per_field;
+    const BfOp& o = batch.op[z];
+    const int m0 = (t / tiles_n) * BM, n0 = (t % tiles_n) * BN;
     if (m0 >= o.Mp || n0 >= o.Npad) return;
-    __shared__ __align__(16) uint16_t sA[2][2][64][kBfLds];   // [stage][hi/lo]
-    __shared__ __align__(16) uint16_t sB[2][2][64][kBfLds];
+    __shared__ __align__(16) uint16_t sA[2][2][BM][kBfLds];   // [stage][hi/lo]
+    __shared__ __align__(16) uint16_t sB[2][2][BN][kBfLds];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
     const int li = lane & 15, lk = lane >> 4;
-    // global -> LDS: 64 rows x 32 k per plane, one 16-byte piece per thread
-    const int lr = tid >> 2, lc = (tid & 3) * 8;
-    const size_t offA = (size_t)(m0 + lr) * o.ld + lc;
-    const size_t offB = (size_t)(n0 + lr) * o.ld + lc;
-    uint4 rAh = *reinterpret_cast<const uint4*>(o.Ah + offA);
-    uint4 rAl = *reinterpret_cast<const uint4*>(o.Al + offA);
-    uint4 rBh = *reinterpret_cast<const uint4*>(o.Bh + offB);
-    uint4 rBl = *reinterpret_cast<const uint4*>(o.Bl + offB);
-    *reinterpret_cast<uint4*>(&sA[0][0][lr][lc]) = rAh;
-    *reinterpret_cast<uint4*>(&sA[0][1][lr][lc]) = rAl;
-    *reinterpret_cast<uint4*>(&sB[0][0][lr][lc]) = rBh;
-    *reinterpret_cast<uint4*>(&sB[0][1][lr][lc]) = rBl;
+    size_t offA[PA], offB[PB];
+    int rowA[PA], rowB[PB], colP = (tid & 3) * 8;
+#pragma unroll
+    for (int p = 0; p < PA; ++p) {
+        rowA[p] = (tid >> 2) + 64 * p;
+        offA[p] = (size_t)(m0 + rowA[p]) * o.ld + colP;
+    }
+#pragma unroll
+    for (int p = 0; p < PB; ++p) {
+        rowB[p] = (tid >> 2) + 64 * p;
+        offB[p] = (size_t)(n0 + rowB[p]) * o.ld + colP;
+    }
+    uint4 rAh[PA], rAl[PA], rBh[PB], rBl[PB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < PA; ++p) {
+            rAh[p] = *reinterpret_cast<const uint4*>(o.Ah + offA[p] + k0);
+            rAl[p] = *reinterpret_cast<const uint4*>(o.Al + offA[p] + k0);
+        }
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+            rBh[p] = *reinterpret_cast<const uint4*>(o.Bh + offB[p] + k0);
+            rBl[p] = *reinterpret_cast<const uint4*>(o.Bl + offB[p] + k0);
+        }
+    };
+    auto stash = [&](int st) {
+#pragma unroll
+        for (int p = 0; p < PA; ++p) {
+            *reinterpret_cast<uint4*>(&sA[st][0][rowA[p]][colP]) = rAh[p];
+            *reinterpret_cast<uint4*>(&sA[st][1][rowA[p]][colP]) = rAl[p];
+        }
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+            *reinterpret_cast<uint4*>(&sB[st][0][rowB[p]][colP]) = rBh[p];
+            *reinterpret_cast<uint4*>(&sB[st][1][rowB[p]][colP]) = rBl[p];
+        }
+    };
+    fetch(0);
+    stash(0);
     __syncthreads();
-    v4f acc[2][2];
+    v4f acc[TM][TN];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int b = 0; b < TN; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
     const int nks = o.K / 32;
     for (int ks = 0; ks < nks; ++ks) {
         const int cur = ks & 1;
         const bool more = ks + 1 < nks;
-        if (more) {
-            const int k1 = (ks + 1) * 32;
-            rAh = *reinterpret_cast<const uint4*>(o.Ah + offA + k1);
-            rAl = *reinterpret_cast<const uint4*>(o.Al + offA + k1);
-            rBh = *reinterpret_cast<const uint4*>(o.Bh + offB + k1);
-            rBl = *reinterpret_cast<const uint4*>(o.Bl + offB + k1);
-        }
-        bf8 ah[2], al[2], bh[2], bl[2];
+        if (more) fetch((ks + 1) * 32);
+        bf8 bh[TN], bl[TN];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            ah[t] = *reinterpret_cast<const bf8*>(&sA[cur][0][wm + t * 16 + li][8 * lk]);
-            al[t] = *reinterpret_cast<const bf8*>(&sA[cur][1][wm + t * 16 + li][8 * lk]);
-            bh[t] = *reinterpret_cast<const bf8*>(&sB[cur][0][wn + t * 16 + li][8 * lk]);
-            bl[t] = *reinterpret_cast<const bf8*>(&sB[cur][1][wn + t * 16 + li][8 * lk]);
+        for (int b = 0; b < TN; ++b) {
+            bh[b] = *reinterpret_cast<const bf8*>(&sB[cur][0][wn + b * 16 + li][8 * lk]);
+            bl[b] = *reinterpret_cast<const bf8*>(&sB[cur][1][wn + b * 16 + li][8 * lk]);
         }
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < TM; ++a) {
+            const bf8 ah = *reinterpret_cast<const bf8*>(&sA[cur][0][wm + a * 16 + li][8 * lk]);
+            const bf8 al = *reinterpret_cast<const bf8*>(&sA[cur][1][wm + a * 16 + li][8 * lk]);
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+            for (int b = 0; b < TN; ++b) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[b], acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[b], acc[a][b], 0, 0, 0);
             }
-        if (more) {
-            const int nxt = cur ^ 1;
-            *reinterpret_cast<uint4*>(&sA[nxt][0][lr][lc]) = rAh;
-            *reinterpret_cast<uint4*>(&sA[nxt][1][lr][lc]) = rAl;
-            *reinterpret_cast<uint4*>(&sB[nxt][0][lr][lc]) = rBh;
-            *reinterpret_cast<uint4*>(&sB[nxt][1][lr][lc]) = rBl;
         }
+        if (more) stash(cur ^ 1);
         __syncthreads();
     }
 
+    // v_mfma_f32_16x16x32_bf16 returns D[i][j] in lane j + 16 (i / 4), item i % 4
     EpiAcc ea{0., 0., 0.};
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < TN; ++b) {
             const int r0 = m0 + wm + a * 16 + 4 * lk;   // rows r0 .. r0 + 3
             const int c = n0 + wn + b * 16 + li;
-            if (o.mode == BF_ROWS) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int r = r0 + q;
-                    uint16_t h = 0, l = 0;
-                    if (r < o.M && c < o.N) bf_split((double)acc[a][b][q], h, l);
-                    o.Ch[(size_t)r * o.ld + c] = h;
-                    o.Cl[(size_t)r * o.ld + c] = l;
-                }
-            } else if (o.mode == BF_TAB_COLS) {
+            if (o.mode == BF_COLS) {
                 uint16_t h[4], l[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int r = r0 + q;
                     h[q] = l[q] = 0;
-                    if (r < o.M && c < o.N) {
-                        const double v = (double)acc[a][b][q] * o.tab[(size_t)r * o.ldt + c];
-                        if (o.mean_acc) ea.mean += o.colsum[r] * v * o.colsum[c];
-                        bf_split(v, h[q], l[q]);
-                    }
+                    if (r0 + q < o.M && c < o.N) bf_split((double)acc[a][b][q], h[q], l[q]);
                 }
                 const size_t at = (size_t)c * o.ld + r0;
                 *reinterpret_cast<ushort4*>(o.Ch + at) = make_ushort4(h[0], h[1], h[2], h[3]);
                 *reinterpret_cast<ushort4*>(o.Cl + at) = make_ushort4(l[0], l[1], l[2], l[3]);
+            } else if (o.mode == BF_TAB_ROWS) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = r0 + q;
+                    uint16_t h = 0, l = 0;
+                    if (r < o.M && c < o.N) {
+                        const double v = (double)acc[a][b][q] * o.tab[(size_t)r * o.ldt + c];
+                        if (o.mean_acc) ea.mean += o.colsum[r] * v * o.colsum[c];
+                        bf_split(v, h, l);
+                    }
+                    o.Ch[(size_t)r * o.ld + c] = h;
+                    o.Cl[(size_t)r * o.ld + c] = l;
+                }
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
@@ -401,7 +380,7 @@ bf_gemm_kernel(BfBatch batch) {
             }
         }
     const int wlane = tid & 63;
-    if (o.mode == BF_TAB_COLS && o.mean_acc) {
+    if (o.mode == BF_TAB_ROWS && o.mean_acc) {
         const double m = wave_sum64(ea.mean);
         if (wlane == 0) atomicAdd(o.mean_acc, m);
     }
@@ -414,11 +393,16 @@ bf_gemm_kernel(BfBatch batch) {
     }
 }
 
-// RHS of each field, split and stored transposed (X[c][r] for the RHS
-// value at grid (rowsA0 + r, colA0 + c)) through a 32 x 32 LDS tile, zero
-// in the padding; then the grid cells outside the field's data region (the
-// zero rim of Ez, Ex, Ey, Bx, By) get the final-pass treatment of F = 0.
+// The right-hand sides of all requested fields in one pass over the grid
+// (lcode.py:166-168, :234-268, :339-343, the same arithmetic as the FFT
+// solver's), written as BF16 planes X[r][c] of the value at grid
+// (rowsA0 + r, colA0 + c).  A 16 x 32-cell tile stages the four stencil
+// sources (rho + beam, jz + beam, jx, jy) with a one-cell halo in LDS; the
+// centre-only inputs are read directly.  Extra blocks give the grid cells
+// outside each field's data region (the zero rim of Ez, Ex, Ey, Bx, By)
+// the final-pass treatment of F = 0.
 struct BfRhsArgs {
+    int nfields;
     FieldPlan f[6];
     RhsInputs in;
     int N, Np;
@@ -428,64 +412,133 @@ struct BfRhsArgs {
     int variantA;
     const double* guess[6];
     double* resid;
+    int need_rho, need_jz, need_j;   // which stencil sources the fields use
 };
+
+constexpr int kRhsTI = 16, kRhsTJ = 32;
 
 __global__ void __launch_bounds__(256)
 bf_rhs_kernel(BfRhsArgs a) {
     if (!gate_open(a.gate)) return;
-    __shared__ double tile[32][33];
-    const int f = blockIdx.y;
-    const FieldPlan& fp = a.f[f];
-    const int tiles_side = a.Np / 32;
-    const int ntiles = tiles_side * tiles_side;
+    __shared__ double s_rho[kRhsTI + 2][kRhsTJ + 2], s_jz[kRhsTI + 2][kRhsTJ + 2];
+    __shared__ double s_jx[kRhsTI + 2][kRhsTJ + 2], s_jy[kRhsTI + 2][kRhsTJ + 2];
+    const int N = a.N;
     const int tid = threadIdx.x;
-    if (f == 0 && blockIdx.x == 0 && tid == 0 && a.mean_zero) *a.mean_zero = 0.;
+    const int tj_n = (N + kRhsTJ - 1) / kRhsTJ, ti_n = (N + kRhsTI - 1) / kRhsTI;
+    const int ntiles = tj_n * ti_n;
+    const RhsInputs& in = a.in;
+    if (blockIdx.x == 0 && tid == 0 && a.mean_zero) *a.mean_zero = 0.;
     if ((int)blockIdx.x < ntiles) {
-        const int r0 = (blockIdx.x / tiles_side) * 32, c0 = (blockIdx.x % tiles_side) * 32;
-        const int tx = tid & 31, ty = tid >> 5;
+        const int i0 = (blockIdx.x / tj_n) * kRhsTI, j0 = (blockIdx.x % tj_n) * kRhsTJ;
+        constexpr int kStage = (kRhsTI + 2) * (kRhsTJ + 2);
+        constexpr int kIter = (kStage + 255) / 256;
+        double r[kIter], z[kIter], x[kIter], y[kIter];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int r = r0 + ty + 8 * m, c = c0 + tx;
-            tile[ty + 8 * m][tx] = (r < fp.rowsA && c < fp.lenA)
-                ? rhs_value_dev(fp.kind, fp.rowsA0 + r, fp.colA0 + c, a.N, a.in) : 0.;
+        for (int it = 0; it < kIter; ++it) {   // all loads issued before any use
+            const int e = tid + 256 * it;
+            const int ii = e / (kRhsTJ + 2), jj = e - ii * (kRhsTJ + 2);
+            const int i = i0 - 1 + ii, j = j0 - 1 + jj;
+            r[it] = z[it] = x[it] = y[it] = 0.;
+            if (e < kStage && i >= 0 && i < N && j >= 0 && j < N) {
+                const size_t idx = (size_t)i * N + j;
+                if (a.need_rho) r[it] = rho_src(in, idx);
+                if (a.need_jz) z[it] = jz_src(in, idx);
+                if (a.need_j) { x[it] = in.jx[idx]; y[it] = in.jy[idx]; }
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kIter; ++it) {
+            const int e = tid + 256 * it;
+            if (e >= kStage) break;
+            const int ii = e / (kRhsTJ + 2), jj = e - ii * (kRhsTJ + 2);
+            s_rho[ii][jj] = r[it];
+            s_jz[ii][jj] = z[it];
+            s_jx[ii][jj] = x[it];
+            s_jy[ii][jj] = y[it];
         }
         __syncthreads();
+        const double h2 = in.h * 2;
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int c = c0 + ty + 8 * m, r = r0 + tx;
-            uint16_t h, l;
-            bf_split(tile[tx][ty + 8 * m], h, l);
-            a.Xh[f][(size_t)c * a.Np + r] = h;
-            a.Xl[f][(size_t)c * a.Np + r] = l;
+        for (int m = 0; m < 2; ++m) {
+            const int ti = (tid >> 5) + 8 * m, tj = tid & 31;
+            const int i = i0 + ti, j = j0 + tj;
+            if (i >= N || j >= N) continue;
+            const size_t idx = (size_t)i * N + j;
+            const int li = ti + 1, lj = tj + 1;
+            const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
+            for (int f = 0; f < a.nfields; ++f) {
+                const FieldPlan& fp = a.f[f];
+                const int r = i - fp.rowsA0, c = j - fp.colA0;
+                if (r < 0 || r >= fp.rowsA || c < 0 || c >= fp.lenA) continue;
+                double v;
+                switch (fp.kind) {
+                    case F_EX: {
+                        const double d = inner ? (s_rho[li + 1][lj] - s_rho[li - 1][lj]) / h2 : 0.;
+                        const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
+                        v = -((d - dj) - in.Ftrick[0][idx] * in.trick);
+                        break;
+                    }
+                    case F_EY: {
+                        const double d = inner ? (s_rho[li][lj + 1] - s_rho[li][lj - 1]) / h2 : 0.;
+                        const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
+                        v = -((d - dj) - in.Ftrick[1][idx] * in.trick);
+                        break;
+                    }
+                    case F_BX: {
+                        const double d = inner ? (s_jz[li][lj + 1] - s_jz[li][lj - 1]) / h2 : 0.;
+                        const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
+                        v = +((d - dj) + in.Ftrick[2][idx] * in.trick);
+                        break;
+                    }
+                    case F_BY: {
+                        const double d = inner ? (s_jz[li + 1][lj] - s_jz[li - 1][lj]) / h2 : 0.;
+                        const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
+                        v = -((d - dj) - in.Ftrick[3][idx] * in.trick);
+                        break;
+                    }
+                    case F_EZ:
+                        v = -((s_jx[li + 1][lj] - s_jx[li - 1][lj]) +
+                              (s_jy[li][lj + 1] - s_jy[li][lj - 1])) / h2;
+                        break;
+                    default: {
+                        const double djx_dy = inner ? s_jx[li][lj + 1] - s_jx[li][lj - 1] : 0.;
+                        const double djy_dx = inner ? s_jy[li + 1][lj] - s_jy[li - 1][lj] : 0.;
+                        v = -(djx_dy - djy_dx) / h2;
+                    }
+                }
+                uint16_t hb, lb;
+                bf_split(v, hb, lb);
+                a.Xh[f][(size_t)r * a.Np + c] = hb;
+                a.Xl[f][(size_t)r * a.Np + c] = lb;
+            }
         }
         return;
     }
-    // rim: cells of the four grid edges outside the data region
-    const int N = a.N;
+    // rim: cells of the four grid edges outside each field's data region
     const int e = ((int)blockIdx.x - ntiles) * 256 + tid;
+    const int side = e / N, t = e - side * N;
+    int i = 0, j = 0;
+    bool ok = e < 4 * N;
+    if (side < 2) { i = side ? N - 1 : 0; j = t; }
+    else { i = t; j = side == 3 ? N - 1 : 0; ok = ok && t > 0 && t < N - 1; }
     double rn = 0., rd = 0.;
-    if (e < 4 * N) {
-        const int side = e / N, t = e - side * N;
-        int i, j;
-        bool ok = true;
-        if (side < 2) { i = side ? N - 1 : 0; j = t; }
-        else { i = t; j = side == 3 ? N - 1 : 0; ok = t > 0 && t < N - 1; }
+    for (int f = 0; f < a.nfields; ++f) {
+        const FieldPlan& fp = a.f[f];
         const bool inside = i >= fp.rowsA0 && i < fp.rowsA0 + fp.rowsA &&
                             j >= fp.colA0 && j < fp.colA0 + fp.lenA;
-        if (ok && !inside) {
-            const size_t idx = (size_t)i * N + j;
-            const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
-                                    fp.kind == F_BX || fp.kind == F_BY;
-            double F = 0.;
-            if (a.variantA && transverse) F = -fp.prevF[idx];   // 2 * 0 - prev
-            if (fp.outF) fp.outF[idx] = F;
-            if (fp.outAvg) fp.outAvg[idx] = (F + fp.prevF[idx]) / 2;
-            const double* g = a.resid ? a.guess[fp.kind] : nullptr;
-            if (g) {
-                const double d = F - g[idx];
-                rn = d * d;
-                rd = F * F;
-            }
+        if (!ok || inside) continue;
+        const size_t idx = (size_t)i * N + j;
+        const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
+                                fp.kind == F_BX || fp.kind == F_BY;
+        double F = 0.;
+        if (a.variantA && transverse) F = -fp.prevF[idx];   // 2 * 0 - prev
+        if (fp.outF) fp.outF[idx] = F;
+        if (fp.outAvg) fp.outAvg[idx] = (F + fp.prevF[idx]) / 2;
+        const double* g = a.resid ? a.guess[fp.kind] : nullptr;
+        if (g) {
+            const double d = F - g[idx];
+            rn += d * d;
+            rd += F * F;
         }
     }
     if (a.resid) {
@@ -500,10 +553,13 @@ bf_rhs_kernel(BfRhsArgs a) {
 
 int launch_solve_bf16x3(int N, const DenseGeom& d, const SolveBatch& b,
                         hipStream_t s, Profiler* prof) {
+    constexpr int TM = LCODE_BF_TM, TN = LCODE_BF_TN;
+    constexpr int BM = 32 * TM, BN = 32 * TN;
     const int Np = d.Np;
     const size_t np2 = (size_t)Np * Np;
-    auto r64 = [](int x) { return (x + 63) / 64 * 64; };
+    auto up = [](int x, int m) { return (x + m - 1) / m * m; };
     BfRhsArgs ra{};
+    ra.nfields = b.nfields;
     ra.in = b.in;
     ra.N = N;
     ra.Np = Np;
@@ -519,58 +575,64 @@ int launch_solve_bf16x3(int N, const DenseGeom& d, const SolveBatch& b,
     int maxM = 0, maxN = 0;
     for (int f = 0; f < b.nfields; ++f) {
         const FieldPlan& fp = b.f[f];
-        const int R = fp.rowsA, C = fp.lenA, Rp = r64(R), Cp = r64(C);
-        maxM = maxM > Rp ? maxM : Rp;
-        maxN = maxN > Cp ? maxN : Cp;
+        const int R = fp.rowsA, C = fp.lenA;
+        const int kind = fp.kind;
         ra.f[f] = fp;
-        ra.Xh[f] = d.Xh + f * np2;
-        ra.Xl[f] = d.Xl + f * np2;
+        if (kind == F_EX || kind == F_EY) ra.need_rho = 1;
+        if (kind == F_BX || kind == F_BY) ra.need_jz = 1;
+        if (kind == F_EZ || kind == F_BZ) ra.need_j = 1;
+        // plane slot = field kind: one shape per slot, zeros elsewhere
+        uint16_t *Xh = d.Xh + kind * np2, *Xl = d.Xl + kind * np2;
+        uint16_t *Qh = d.Ph + kind * np2, *Ql = d.Pl + kind * np2;
+        ra.Xh[f] = Xh;
+        ra.Xl[f] = Xl;
         const bool dstB = fp.xfB == XF_DST, dstA = fp.xfA == XF_DST;
         const uint16_t* TBh = dstB ? d.Sh : d.Ch;
         const uint16_t* TBl = dstB ? d.Sl : d.Cl;
         const uint16_t* TAh = dstA ? d.Sh : d.Ch;
         const uint16_t* TAl = dstA ? d.Sl : d.Cl;
-        uint16_t *Xh = d.Xh + f * np2, *Xl = d.Xl + f * np2;
-        uint16_t *Ph = d.Ph + f * np2, *Pl = d.Pl + f * np2;
         BfOp o{};
-        o.ld = Np; o.M = R; o.N = C; o.Mp = Rp; o.Npad = Cp;
-        // G1: P = T_B R
-        o.Ah = TBh; o.Al = TBl; o.Bh = Xh; o.Bl = Xl; o.K = Rp;
-        o.mode = BF_ROWS; o.Ch = Ph; o.Cl = Pl;
-        g1.op[f] = o;
-        // G2: X = ((P T_A^T) o table)^T
+        o.ld = Np; o.M = R; o.N = C; o.Mp = up(R, BM); o.Npad = up(C, BN);
+        maxM = maxM > o.Mp ? maxM : o.Mp;
+        maxN = maxN > o.Npad ? maxN : o.Npad;
+        // G1: Q = X T_A^T, stored by columns
+        BfOp o1 = o;
+        o1.Ah = Xh; o1.Al = Xl; o1.Bh = TAh; o1.Bl = TAl; o1.K = up(C, 32);
+        o1.mode = BF_COLS; o1.Ch = Qh; o1.Cl = Ql;
+        g1.op[f] = o1;
+        // G2: P = (T_B Q) o table, stored by rows (over X: X is consumed)
         BfOp o2 = o;
-        o2.Ah = Ph; o2.Al = Pl; o2.Bh = TAh; o2.Bl = TAl; o2.K = Cp;
-        o2.mode = BF_TAB_COLS; o2.Ch = Xh; o2.Cl = Xl;
-        o2.tab = fp.kind == F_EY || fp.kind == F_BX ? d.mixed
-               : fp.kind == F_EX || fp.kind == F_BY ? d.mixed_t
+        o2.Ah = TBh; o2.Al = TBl; o2.Bh = Qh; o2.Bl = Ql; o2.K = up(R, 32);
+        o2.mode = BF_TAB_ROWS; o2.Ch = Xh; o2.Cl = Xl;
+        o2.tab = kind == F_EY || kind == F_BX ? d.mixed
+               : kind == F_EX || kind == F_BY ? d.mixed_t
                : static_cast<const double*>(fp.table);
         o2.ldt = C;
-        if (fp.kind == F_BZ) { o2.colsum = d.colsum; o2.mean_acc = d.mean; }
+        if (kind == F_BZ) { o2.colsum = d.colsum; o2.mean_acc = d.mean; }
         g2.op[f] = o2;
-        // G3: P = T_B X
-        BfOp o3 = o;
-        o3.Ah = TBh; o3.Al = TBl; o3.Bh = Xh; o3.Bl = Xl; o3.K = Rp;
-        o3.mode = BF_ROWS; o3.Ch = Ph; o3.Cl = Pl;
+        // G3: Q = P T_A^T, stored by columns
+        BfOp o3 = o1;
         g3.op[f] = o3;
-        // G4: field = P T_A^T on the grid
+        // G4: field = T_B Q on the grid
         BfOp o4 = o;
-        o4.Ah = Ph; o4.Al = Pl; o4.Bh = TAh; o4.Bl = TAl; o4.K = Cp;
+        o4.Ah = TBh; o4.Al = TBl; o4.Bh = Qh; o4.Bl = Ql; o4.K = up(R, 32);
         o4.mode = BF_FINAL;
         o4.row0 = fp.rowsA0; o4.col0 = fp.colA0; o4.Ngrid = N;
-        if (fp.kind == F_BZ) o4.mean_in = d.mean;
-        const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
-                                fp.kind == F_BX || fp.kind == F_BY;
+        if (kind == F_BZ) o4.mean_in = d.mean;
+        const bool transverse = kind == F_EX || kind == F_EY ||
+                                kind == F_BX || kind == F_BY;
         o4.variantA = b.variantA && transverse;
         o4.prevF = fp.prevF;
         o4.outF = fp.outF;
         o4.outAvg = fp.outAvg;
-        o4.guess = b.resid ? b.guess[fp.kind] : nullptr;
+        o4.guess = b.resid ? b.guess[kind] : nullptr;
         g4.op[f] = o4;
     }
-    const int tiles = (Np / 32) * (Np / 32);
-    const dim3 grid_rhs(tiles + (4 * N + 255) / 256, b.nfields);
-    const dim3 grid_mm(maxN / 64, maxM / 64, b.nfields);
+    const int tiles = ((N + kRhsTJ - 1) / kRhsTJ) * ((N + kRhsTI - 1) / kRhsTI);
+    const dim3 grid_rhs(tiles + (4 * N + 255) / 256);
+    g1.tiles_m = g2.tiles_m = g3.tiles_m = g4.tiles_m = maxM / BM;
+    g1.tiles_n = g2.tiles_n = g3.tiles_n = g4.tiles_n = maxN / BN;
+    const dim3 grid_mm((maxM / BM) * (maxN / BN) * b.nfields);
     {
         ProfScope ps(prof, "dense_rhs", s);
         bf_rhs_kernel<<<grid_rhs, 256, 0, s>>>(ra);
@@ -581,7 +643,7 @@ int launch_solve_bf16x3(int N, const DenseGeom& d, const SolveBatch& b,
     for (int k = 0; k < 4; ++k) {
         {
             ProfScope ps(prof, names[k], s);
-            bf_gemm_kernel<<<grid_mm, 256, 0, s>>>(*gs[k]);
+            bf_gemm_kernel<TM, TN><<<grid_mm, 256, 0, s>>>(*gs[k]);
         }
         LCODE_HIP(hipGetLastError());
     }
@@ -590,7 +652,7 @@ int launch_solve_bf16x3(int N, const DenseGeom& d, const SolveBatch& b,
 
 }  // namespace
 
-int launch_gemm(const GemmBatch& b, int prec, hipStream_t s, Profiler* prof,
+int launch_gemm(const GemmBatch& b, hipStream_t s, Profiler* prof,
                 const char* name) {
     int maxM = 0, maxN = 0;
     for (int k = 0; k < b.n; ++k) {
@@ -600,10 +662,7 @@ int launch_gemm(const GemmBatch& b, int prec, hipStream_t s, Profiler* prof,
     const dim3 grid((maxN + kBN - 1) / kBN, (maxM + kBM - 1) / kBM, b.n);
     {
         ProfScope ps(prof, name, s);
-        if (prec == DENSE_F64)
-            gemm_kernel<DENSE_F64><<<grid, 256, 0, s>>>(b);
-        else
-            gemm_kernel<DENSE_BF16X3><<<grid, 256, 0, s>>>(b);
+        gemm_kernel<<<grid, 256, 0, s>>>(b);
     }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
@@ -686,10 +745,10 @@ int launch_solve_dense(int N, const DenseGeom& d, const SolveBatch& b,
     }
     int rc;
     if ((rc = launch_dense_rhs(ra, b.nfields, max_total, s, prof))) return rc;
-    if ((rc = launch_gemm(g1, d.prec, s, prof, "dense_gemm_1"))) return rc;
-    if ((rc = launch_gemm(g2, d.prec, s, prof, "dense_gemm_2"))) return rc;
-    if ((rc = launch_gemm(g3, d.prec, s, prof, "dense_gemm_3"))) return rc;
-    return launch_gemm(g4, d.prec, s, prof, "dense_gemm_4");
+    if ((rc = launch_gemm(g1, s, prof, "dense_gemm_1"))) return rc;
+    if ((rc = launch_gemm(g2, s, prof, "dense_gemm_2"))) return rc;
+    if ((rc = launch_gemm(g3, s, prof, "dense_gemm_3"))) return rc;
+    return launch_gemm(g4, s, prof, "dense_gemm_4");
 }
 
 }  // namespace lcode

@@ -361,13 +361,8 @@ struct BlockPos {
 };
 
 __device__ __forceinline__ BlockPos xcd_block() {
-    constexpr int kXcd = 8;
     const int gx = gridDim.x, total = gx * gridDim.y;
-    const int lin = blockIdx.y * gx + blockIdx.x;
-    const int q = total / kXcd, r = total % kXcd;
-    const int xcd = lin % kXcd, slot = lin / kXcd;
-    const int logical = xcd < r ? xcd * (q + 1) + slot
-                                : r * (q + 1) + (xcd - r) * q + slot;
+    const int logical = xcd_logical(blockIdx.y * gx + blockIdx.x, total);
     return BlockPos{logical / gx, logical % gx};
 }
 

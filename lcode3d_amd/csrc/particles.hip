@@ -123,7 +123,11 @@ push_kernel(PushArgs a) {
         int cells = c < 1e6 ? (int)c : (1 << 20);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) cells = max(cells, __shfl_xor(cells, off, 64));
-        if ((threadIdx.x & 63) == 0) atomicMax(a.max_cells, cells);
+        // one atomic per wave, and only when it raises the bound: after the
+        // first waves the bound is reached and the rest only read it
+        if ((threadIdx.x & 63) == 0 &&
+            cells > __hip_atomic_load(a.max_cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(a.max_cells, cells);
     }
 }
 
