@@ -323,17 +323,28 @@ bf_gemm_kernel(BfBatch batch) {
             bh[b] = *reinterpret_cast<const bf8*>(&sB[cur][0][wn + b * 16 + li][8 * lk]);
             bl[b] = *reinterpret_cast<const bf8*>(&sB[cur][1][wn + b * 16 + li][8 * lk]);
         }
+        bf8 ah[TM], al[TM];
 #pragma unroll
         for (int a = 0; a < TM; ++a) {
-            const bf8 ah = *reinterpret_cast<const bf8*>(&sA[cur][0][wm + a * 16 + li][8 * lk]);
-            const bf8 al = *reinterpret_cast<const bf8*>(&sA[cur][1][wm + a * 16 + li][8 * lk]);
-#pragma unroll
-            for (int b = 0; b < TN; ++b) {
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[b], acc[a][b], 0, 0, 0);
-            }
+            ah[a] = *reinterpret_cast<const bf8*>(&sA[cur][0][wm + a * 16 + li][8 * lk]);
+            al[a] = *reinterpret_cast<const bf8*>(&sA[cur][1][wm + a * 16 + li][8 * lk]);
         }
+        // product-major order: consecutive MFMAs write different accumulators
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+            for (int b = 0; b < TN; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
         if (more) stash(cur ^ 1);
         __syncthreads();
     }
@@ -417,7 +428,9 @@ struct BfRhsArgs {
 
 constexpr int kRhsTI = 16, kRhsTJ = 32;
 
-__global__ void __launch_bounds__(256)
+constexpr int kRhsThreads = kRhsTI * kRhsTJ;   // one cell per thread
+
+__global__ void __launch_bounds__(kRhsThreads)
 bf_rhs_kernel(BfRhsArgs a) {
     if (!gate_open(a.gate)) return;
     __shared__ double s_rho[kRhsTI + 2][kRhsTJ + 2], s_jz[kRhsTI + 2][kRhsTJ + 2];
@@ -431,11 +444,11 @@ bf_rhs_kernel(BfRhsArgs a) {
     if ((int)blockIdx.x < ntiles) {
         const int i0 = (blockIdx.x / tj_n) * kRhsTI, j0 = (blockIdx.x % tj_n) * kRhsTJ;
         constexpr int kStage = (kRhsTI + 2) * (kRhsTJ + 2);
-        constexpr int kIter = (kStage + 255) / 256;
+        constexpr int kIter = (kStage + kRhsThreads - 1) / kRhsThreads;
         double r[kIter], z[kIter], x[kIter], y[kIter];
 #pragma unroll
         for (int it = 0; it < kIter; ++it) {   // all loads issued before any use
-            const int e = tid + 256 * it;
+            const int e = tid + kRhsThreads * it;
             const int ii = e / (kRhsTJ + 2), jj = e - ii * (kRhsTJ + 2);
             const int i = i0 - 1 + ii, j = j0 - 1 + jj;
             r[it] = z[it] = x[it] = y[it] = 0.;
@@ -448,7 +461,7 @@ bf_rhs_kernel(BfRhsArgs a) {
         }
 #pragma unroll
         for (int it = 0; it < kIter; ++it) {
-            const int e = tid + 256 * it;
+            const int e = tid + kRhsThreads * it;
             if (e >= kStage) break;
             const int ii = e / (kRhsTJ + 2), jj = e - ii * (kRhsTJ + 2);
             s_rho[ii][jj] = r[it];
@@ -458,11 +471,10 @@ bf_rhs_kernel(BfRhsArgs a) {
         }
         __syncthreads();
         const double h2 = in.h * 2;
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            const int ti = (tid >> 5) + 8 * m, tj = tid & 31;
+        {
+            const int ti = tid / kRhsTJ, tj = tid % kRhsTJ;
             const int i = i0 + ti, j = j0 + tj;
-            if (i >= N || j >= N) continue;
+            if (i >= N || j >= N) return;
             const size_t idx = (size_t)i * N + j;
             const int li = ti + 1, lj = tj + 1;
             const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
@@ -515,7 +527,7 @@ bf_rhs_kernel(BfRhsArgs a) {
         return;
     }
     // rim: cells of the four grid edges outside each field's data region
-    const int e = ((int)blockIdx.x - ntiles) * 256 + tid;
+    const int e = ((int)blockIdx.x - ntiles) * kRhsThreads + tid;
     const int side = e / N, t = e - side * N;
     int i = 0, j = 0;
     bool ok = e < 4 * N;
@@ -629,13 +641,13 @@ int launch_solve_bf16x3(int N, const DenseGeom& d, const SolveBatch& b,
         g4.op[f] = o4;
     }
     const int tiles = ((N + kRhsTJ - 1) / kRhsTJ) * ((N + kRhsTI - 1) / kRhsTI);
-    const dim3 grid_rhs(tiles + (4 * N + 255) / 256);
+    const dim3 grid_rhs(tiles + (4 * N + kRhsThreads - 1) / kRhsThreads);
     g1.tiles_m = g2.tiles_m = g3.tiles_m = g4.tiles_m = maxM / BM;
     g1.tiles_n = g2.tiles_n = g3.tiles_n = g4.tiles_n = maxN / BN;
     const dim3 grid_mm((maxM / BM) * (maxN / BN) * b.nfields);
     {
         ProfScope ps(prof, "dense_rhs", s);
-        bf_rhs_kernel<<<grid_rhs, 256, 0, s>>>(ra);
+        bf_rhs_kernel<<<grid_rhs, kRhsThreads, 0, s>>>(ra);
     }
     LCODE_HIP(hipGetLastError());
     const BfBatch* gs[4] = {&g1, &g2, &g3, &g4};
