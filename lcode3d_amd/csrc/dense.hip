@@ -179,7 +179,6 @@ dense_rhs_kernel(DenseRhsArgs a) {
 // ---------------------------------------------------------------------------
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 enum BfMode { BF_COLS = 0, BF_TAB_ROWS = 1, BF_FINAL = 2 };
 
@@ -189,7 +188,6 @@ enum BfMode { BF_COLS = 0, BF_TAB_ROWS = 1, BF_FINAL = 2 };
 #ifndef LCODE_BF_TN
 #define LCODE_BF_TN 2
 #endif
-
 
 struct BfOp {
     const uint16_t *Ah, *Al;   // A(r, k) = A[r ld + k]
@@ -281,47 +279,51 @@ bf_gemm_kernel(BfBatch batch) {
         rowB[p] = (tid >> 2) + 64 * p;
         offB[p] = (size_t)(n0 + rowB[p]) * o.ld + colP;
     }
-    // K steps of 32 staged through a double-buffered LDS tile; the global
-    // loads run four steps ahead in a ring of registers (slots 0-3 written
-    // with constant indices), so several HBM round trips are in flight
-    const int nks = o.K / 32;
-    u32x4 r0ah[PA], r0al[PA], r0bh[PB], r0bl[PB];
-    u32x4 r1ah[PA], r1al[PA], r1bh[PB], r1bl[PB];
-    u32x4 r2ah[PA], r2al[PA], r2bh[PB], r2bl[PB];
-    u32x4 r3ah[PA], r3al[PA], r3bh[PB], r3bl[PB];
-#define LCODE_BF_FETCH(U, KS)                                                      \
-    if ((KS) < nks) {                                                             \
-        const int k0_ = (KS) * 32;                                                \
-        _Pragma("unroll") for (int p = 0; p < PA; ++p) {                          \
-            U##ah[p] = *reinterpret_cast<const u32x4*>(o.Ah + offA[p] + k0_);     \
-            U##al[p] = *reinterpret_cast<const u32x4*>(o.Al + offA[p] + k0_);     \
-        }                                                                         \
-        _Pragma("unroll") for (int p = 0; p < PB; ++p) {                          \
-            U##bh[p] = *reinterpret_cast<const u32x4*>(o.Bh + offB[p] + k0_);     \
-            U##bl[p] = *reinterpret_cast<const u32x4*>(o.Bl + offB[p] + k0_);     \
-        }                                                                         \
-    }
-#define LCODE_BF_STASH(U, ST)                                                     \
-    _Pragma("unroll") for (int p = 0; p < PA; ++p) {                              \
-        *reinterpret_cast<u32x4*>(&sA[ST][0][rowA[p]][colP]) = U##ah[p];          \
-        *reinterpret_cast<u32x4*>(&sA[ST][1][rowA[p]][colP]) = U##al[p];          \
-    }                                                                             \
-    _Pragma("unroll") for (int p = 0; p < PB; ++p) {                              \
-        *reinterpret_cast<u32x4*>(&sB[ST][0][rowB[p]][colP]) = U##bh[p];          \
-        *reinterpret_cast<u32x4*>(&sB[ST][1][rowB[p]][colP]) = U##bl[p];          \
-    }
+    uint4 rAh[PA], rAl[PA], rBh[PB], rBl[PB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < PA; ++p) {
+            rAh[p] = *reinterpret_cast<const uint4*>(o.Ah + offA[p] + k0);
+            rAl[p] = *reinterpret_cast<const uint4*>(o.Al + offA[p] + k0);
+        }
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+            rBh[p] = *reinterpret_cast<const uint4*>(o.Bh + offB[p] + k0);
+            rBl[p] = *reinterpret_cast<const uint4*>(o.Bl + offB[p] + k0);
+        }
+    };
+    auto stash = [&](int st) {
+#pragma unroll
+        for (int p = 0; p < PA; ++p) {
+            *reinterpret_cast<uint4*>(&sA[st][0][rowA[p]][colP]) = rAh[p];
+            *reinterpret_cast<uint4*>(&sA[st][1][rowA[p]][colP]) = rAl[p];
+        }
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
+            *reinterpret_cast<uint4*>(&sB[st][0][rowB[p]][colP]) = rBh[p];
+            *reinterpret_cast<uint4*>(&sB[st][1][rowB[p]][colP]) = rBl[p];
+        }
+    };
+    fetch(0);
+    stash(0);
+    __syncthreads();
     v4f acc[TM][TN];
 #pragma unroll
     for (int a = 0; a < TM; ++a)
 #pragma unroll
         for (int b = 0; b < TN; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-    auto compute = [&](int cur) __attribute__((always_inline)) {
-        bf8 bh[TN], bl[TN], ah[TM], al[TM];
+    const int nks = o.K / 32;
+    for (int ks = 0; ks < nks; ++ks) {
+        const int cur = ks & 1;
+        const bool more = ks + 1 < nks;
+        if (more) fetch((ks + 1) * 32);
+        bf8 bh[TN], bl[TN];
 #pragma unroll
         for (int b = 0; b < TN; ++b) {
             bh[b] = *reinterpret_cast<const bf8*>(&sB[cur][0][wn + b * 16 + li][8 * lk]);
             bl[b] = *reinterpret_cast<const bf8*>(&sB[cur][1][wn + b * 16 + li][8 * lk]);
         }
+        bf8 ah[TM], al[TM];
 #pragma unroll
         for (int a = 0; a < TM; ++a) {
             ah[a] = *reinterpret_cast<const bf8*>(&sA[cur][0][wm + a * 16 + li][8 * lk]);
@@ -343,29 +345,9 @@ bf_gemm_kernel(BfBatch batch) {
 #pragma unroll
             for (int b = 0; b < TN; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-    };
-    LCODE_BF_FETCH(r0, 0)
-    LCODE_BF_FETCH(r1, 1)
-    LCODE_BF_FETCH(r2, 2)
-    LCODE_BF_FETCH(r3, 3)
-    // LDS buffer ks & 1 was last read by step ks - 2, which every wave
-    // finished before the previous barrier; the branch is block-uniform
-#define LCODE_BF_STEP(R, U)                                                       \
-    if (ks0 + (U) < nks) {                                                        \
-        LCODE_BF_STASH(R, (U) & 1)                                                \
-        LCODE_BF_FETCH(R, ks0 + (U) + 4)                                          \
-        __syncthreads();                                                          \
-        compute((U) & 1);                                                         \
+        if (more) stash(cur ^ 1);
+        __syncthreads();
     }
-    for (int ks0 = 0; ks0 < nks; ks0 += 4) {
-        LCODE_BF_STEP(r0, 0)
-        LCODE_BF_STEP(r1, 1)
-        LCODE_BF_STEP(r2, 2)
-        LCODE_BF_STEP(r3, 3)
-    }
-#undef LCODE_BF_STEP
-#undef LCODE_BF_STASH
-#undef LCODE_BF_FETCH
 
     // v_mfma_f32_16x16x32_bf16 returns D[i][j] in lane j + 16 (i / 4), item i % 4
     EpiAcc ea{0., 0., 0.};
