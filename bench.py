@@ -70,6 +70,24 @@ def kernel_bytes(name, nc, n):
     return 8 * table[name] if name in table else None
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at this workload from the two
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch)
+    recorded in profiles/pmc_traffic.json, each scaled by the factor the
+    same counter reads for a device copy of known size; None without it."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        k = t["kernels"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return 1024 * (k["FETCH_SIZE_KiB"] * t["fetch_factor"]
+                   + k["WRITE_SIZE_KiB"] * t["write_factor"])
+
+
 def dense_flops_per_solve(n):
     """Algorithmic flops of one six-field solve in the dense-matrix form
     (forward and inverse transform along both axes, 2 flops per
@@ -84,10 +102,12 @@ SOLVE_KERNELS = ("solve_pass_a", "solve_pass_b", "solve_pass_c", "dense_rhs",
 MFMA_PEAK_TFS = {"fp64": 78.6, "bf16x3": 2500.0}
 
 
-def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps):
+def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps,
+                bracket_us=0.0):
     """The same step with each field-solver back-end from the same developed
     state: whole-step throughput over `steps` slices and the six-field solve
-    time from HIP events over `prof_steps` slices."""
+    time from HIP events over `prof_steps` slices (less `bracket_us`, the
+    event-pair overhead per launch)."""
     import copy
     legs = []
     for transform, precision in (("fft", "fp64"), ("fft", "fp32"),
@@ -108,12 +128,13 @@ def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps):
         for k in range(prof_steps):
             st = lc.step(c, const, vp, st, beams[k % len(beams)])
         prof = lc.profile_end()
-        solve_ms = sum(v[1] for k, v in prof.items() if k in SOLVE_KERNELS)
-        solve_us = solve_ms / (2 * prof_steps) * 1e3
+        kus = {k: v[1] / v[0] * 1e3 - bracket_us for k, v in prof.items()
+               if k in SOLVE_KERNELS}
+        calls = {k: v[0] / (2 * prof_steps) for k, v in prof.items() if k in SOLVE_KERNELS}
+        solve_us = sum(kus[k] * calls[k] for k in kus)
         leg = {"transform": transform, "precision": precision,
                "xi_steps_per_s": steps / dt, "solve_us": solve_us,
-               "kernels_us": {k: v[1] / v[0] * 1e3 for k, v in prof.items()
-                              if k in SOLVE_KERNELS}}
+               "kernels_us": kus}
         if transform == "dense":
             tf = dense_flops_per_solve(cfg.grid_steps) / (solve_us * 1e-6) / 1e12
             leg["algorithmic_TFLOPs"] = tf
@@ -284,7 +305,7 @@ def main():
         f"step-level {step_bytes / (dt / args.steps) / 1e9:.1f} GB/s algorithmic")
 
     # per-kernel times over a separate region, HIP events on the step stream
-    roofline, kernels = None, {}
+    roofline, kernels, bracket_us = None, {}, 0.0
     if args.profile_steps:
         lc.profile_begin(64 * args.profile_steps)
         for k in range(args.profile_steps):
@@ -292,22 +313,35 @@ def main():
         prof = lc.profile_end()
         for name, (calls, kms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
             avg_us = kms / calls * 1e3
-            b = kernel_bytes(name, nc, n)
             kernels[name] = {"launches_per_step": calls / args.profile_steps,
                              "avg_us": avg_us,
-                             "share": kms / sum(v[1] for v in prof.values()),
-                             "GBps": (b / (avg_us * 1e-6) / 1e9) if b else None}
-            log(f"  {name:14s} {calls / args.profile_steps:4.1f}/step "
-                f"{avg_us:8.2f} us  {kernels[name]['share'] * 100:5.1f}%"
-                + (f"  {kernels[name]['GBps']:7.0f} GB/s" if b else ""))
+                             "share": kms / sum(v[1] for v in prof.values())}
+        # An event pair around a launch also times the launch's dispatch:
+        # the per-launch excess of the bracketed kernel times over the
+        # unbracketed timed loop is that overhead; subtracting it gives the
+        # kernel's own duration (what a rocprofv3 kernel trace reports).
+        launches = sum(v["launches_per_step"] for v in kernels.values())
+        bracket_sum = sum(v["launches_per_step"] * v["avg_us"] for v in kernels.values())
+        bracket_us = max(0.0, (bracket_sum - ms * 1e3) / launches)
+        for name, v in kernels.items():
+            v["kernel_us"] = v["avg_us"] - bracket_us
+            b = kernel_bytes(name, nc, n)
+            v["GBps"] = (b / (v["kernel_us"] * 1e-6) / 1e9) if b else None
+        log(f"  event-pair overhead {bracket_us:.2f} us per launch; kernel durations:")
+        for name, v in kernels.items():
+            log(f"  {name:14s} {v['launches_per_step']:4.1f}/step "
+                f"{v['kernel_us']:8.2f} us  {v['share'] * 100:5.1f}%"
+                + (f"  {v['GBps']:7.0f} GB/s" if v["GBps"] else ""))
         dom = max((k for k in kernels if kernels[k]["GBps"]),
                   key=lambda k: kernels[k]["share"])
         ach = kernels[dom]["GBps"]
         roofline = {"bound": "hbm", "kernel": dom, "achieved": ach,
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                    "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                     "bytes_per_launch": kernel_bytes(dom, nc, n),
-                    "avg_us": kernels[dom]["avg_us"]}
+                    "avg_us": kernels[dom]["kernel_us"],
+                    "avg_us_with_event_pair": kernels[dom]["avg_us"],
+                    "event_pair_overhead_us": bracket_us}
     if args.copy_probe_mib:
         src = torch.ones(args.copy_probe_mib << 17, dtype=torch.float64,
                          device="cuda")
@@ -320,7 +354,8 @@ def main():
     solvers = None
     if args.solver_steps:
         solvers = solver_legs(lc, torch, cfg, const, vp, state, beams,
-                              args.solver_steps, min(args.profile_steps or 10, 10))
+                              args.solver_steps, min(args.profile_steps or 10, 10),
+                              bracket_us)
 
     mixed = None
     if args.mixed_slices:

@@ -244,6 +244,69 @@ int lcode_step_guided_residual(lcode_workspace *ws, hipStream_t stream,
 int lcode_diag_zn(lcode_workspace *ws, const lcode_config *cfg,
                   const double *ro, double *zn_out, hipStream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Learned predictor kernels.  No counterpart in the reference: they train
+ * and run a small convolutional network whose output is a field guess for
+ * lcode_step_guided (the learned form of the predictor pass,
+ * lcode.py:841-881).  FP32 planes [C][H][W]; 3 x 3 convolutions with zero
+ * padding; weights w[cout][cin][3][3], biases b[cout]; 1 <= cin, cout <= 16.
+ * -------------------------------------------------------------------------*/
+
+/* out = conv(in, w) + b, then max(., 0) when relu != 0. */
+int lcode_pred_conv3x3_forward(const float *in, int32_t cin, const float *w,
+                               const float *b, float *out, int32_t cout,
+                               int32_t H, int32_t W, int32_t relu,
+                               hipStream_t stream);
+
+/* Gradient with respect to the layer input; when relu_input is not NULL
+ * (the layer's input came out of a ReLU) it is zeroed where relu_input <= 0. */
+int lcode_pred_conv3x3_backward_input(const float *grad_out, int32_t cout,
+                                      const float *w, float *grad_in,
+                                      int32_t cin, const float *relu_input,
+                                      int32_t H, int32_t W, hipStream_t stream);
+
+/* Gradients of w and b, written as grad_wb[cout*cin*9 + cout] (weights,
+ * then biases); `scratch` holds lcode_pred_backward_weight_scratch floats. */
+int lcode_pred_conv3x3_backward_weight(const float *in, int32_t cin,
+                                       const float *grad_out, int32_t cout,
+                                       float *scratch, float *grad_wb,
+                                       int32_t H, int32_t W, hipStream_t stream);
+int64_t lcode_pred_backward_weight_scratch(int32_t cin, int32_t cout,
+                                           int32_t H, int32_t W);
+
+/* g = 2 (y - t) / n and *loss_sum = sum (y - t)^2 (device float);
+ * `scratch` holds (n + 255) / 256 floats. */
+int lcode_pred_mse_grad(const float *y, const float *t, int64_t n, float *g,
+                        float *scratch, float *loss_sum, hipStream_t stream);
+
+/* One Adam step (step number t >= 1) on n parameters. */
+int lcode_pred_adam(float *p, const float *g, float *m, float *v, int32_t n,
+                    float lr, float beta1, float beta2, float eps, int32_t t,
+                    hipStream_t stream);
+
+/* FP64 fields <-> the network's FP32 planes.  inputs: planes[14][N][N] =
+ * F(n-1)/s, F(n-2)/s (6 each), beam/s_b, ro(n-1)/s_r.  targets:
+ * planes[6][N][N] = (F(n) - 2 F(n-1) + F(n-2)) / t.  guess: guess[f] =
+ * 2 F(n-1) - F(n-2) + t_f pred[f]. */
+typedef struct {
+    const double *f1[6];      /* fields of slice n-1 (Ex Ey Ez Bx By Bz) */
+    const double *f2[6];      /* fields of slice n-2 */
+    const double *beam;       /* beam of slice n (NULL: beam_scalar) */
+    double beam_scalar;
+    const double *ro;         /* plasma density of slice n-1 */
+    const double *next[6];    /* targets only: fields of slice n */
+    float scale[14];          /* input scales */
+    float tscale[6];          /* target scales */
+    float *planes;            /* inputs / targets output */
+    const float *pred;        /* guess only: network output [6][N][N] */
+    double *guess[6];         /* guess only: output fields */
+    int32_t n;                /* N * N */
+} lcode_pred_io;
+
+int lcode_pred_inputs(const lcode_pred_io *io, hipStream_t stream);
+int lcode_pred_targets(const lcode_pred_io *io, hipStream_t stream);
+int lcode_pred_guess(const lcode_pred_io *io, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

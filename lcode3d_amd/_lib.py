@@ -69,6 +69,15 @@ class LcodeGuess(ctypes.Structure):
                 ("nterms", ctypes.c_int32)]
 
 
+class LcodePredIo(ctypes.Structure):
+    _fields_ = [("f1", ctypes.c_void_p * 6), ("f2", ctypes.c_void_p * 6),
+                ("beam", ctypes.c_void_p), ("beam_scalar", ctypes.c_double),
+                ("ro", ctypes.c_void_p), ("next", ctypes.c_void_p * 6),
+                ("scale", ctypes.c_float * 14), ("tscale", ctypes.c_float * 6),
+                ("planes", ctypes.c_void_p), ("pred", ctypes.c_void_p),
+                ("guess", ctypes.c_void_p * 6), ("n", ctypes.c_int32)]
+
+
 class LcodeKernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("calls", ctypes.c_int64),
                 ("total_ms", ctypes.c_double)]
@@ -125,6 +134,27 @@ _SIGNATURES = {
                                    [_vp, _vp, ctypes.POINTER(ctypes.c_double)]),
     "lcode_diag_zn": (ctypes.c_int,
                       [_vp, ctypes.POINTER(LcodeConfig), _vp, _vp, _vp]),
+    "lcode_pred_conv3x3_forward": (ctypes.c_int,
+                                   [_vp, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int32,
+                                    ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "lcode_pred_conv3x3_backward_input": (ctypes.c_int,
+                                          [_vp, ctypes.c_int32, _vp, _vp, ctypes.c_int32,
+                                           _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "lcode_pred_conv3x3_backward_weight": (ctypes.c_int,
+                                           [_vp, ctypes.c_int32, _vp, ctypes.c_int32,
+                                            _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    "lcode_pred_backward_weight_scratch": (ctypes.c_int64,
+                                           [ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_int32, ctypes.c_int32]),
+    "lcode_pred_mse_grad": (ctypes.c_int,
+                            [_vp, _vp, ctypes.c_int64, _vp, _vp, _vp, _vp]),
+    "lcode_pred_adam": (ctypes.c_int,
+                        [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float,
+                         ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                         ctypes.c_int32, _vp]),
+    "lcode_pred_inputs": (ctypes.c_int, [ctypes.POINTER(LcodePredIo), _vp]),
+    "lcode_pred_targets": (ctypes.c_int, [ctypes.POINTER(LcodePredIo), _vp]),
+    "lcode_pred_guess": (ctypes.c_int, [ctypes.POINTER(LcodePredIo), _vp]),
 }
 
 SYMBOLS = tuple(_SIGNATURES)
