@@ -19,3 +19,9 @@ def pytest_configure(config):
 def oracle():
     from oracle import lcode_oracle
     return lcode_oracle
+
+
+@pytest.fixture(scope="session")
+def po():
+    from oracle import predictor_oracle
+    return predictor_oracle
