@@ -213,6 +213,101 @@ def mixed_guided_leg(lc, cosine_gaussian, torch, slices, timed, precision):
             "guess_accept_rate": accepted / max(guided, 1)}
 
 
+def learned_leg(lc, cosine_gaussian, torch, slices, timed, precision,
+                 train_runs=2, train_slices=220, epochs=3, hidden=(8, 8)):
+    """Mixed precision + the learned predictor: a ConvPredictor trained on
+    `train_runs` plain 641^2 runs (amplitude ~ U[0.01, 0.1], sigma ~
+    U[0.5, 2] from default_rng(0)), its guess in place of the predictor
+    pass with residual threshold GUIDED_THRESHOLD; accuracy against the
+    plain FP64 step over `slices` slices of the bench beam, throughput over
+    `timed` slices (guess inference included)."""
+    import collections
+    import numpy as np
+    from lcode3d_amd.predictor import ConvPredictor
+    cfg_p = make_config()
+    rng = np.random.default_rng(0)
+    params = [(rng.uniform(.01, .1), rng.uniform(.5, 2)) for _ in range(train_runs)]
+    net = ConvPredictor(cfg_p.grid_steps, hidden=hidden, seed=0, lr=2e-3)
+    t0 = time.perf_counter()
+
+    def windows(amp, sig):
+        _, _, const, vp, st = lc.init(cfg_p)
+        last = collections.deque(maxlen=4)
+        for i in range(train_slices):
+            beam = cosine_gaussian(cfg_p, i, amplitude=amp, sigma=sig)
+            st = lc.step(cfg_p, const, vp, st, beam)
+            last.append(st)
+            if i >= 23 and len(last) == 4:
+                yield last[3], last[2], last[1], last[0], beam
+    acc = net.scale_accumulator()
+    for s0, s1, s2, s3, _ in windows(*params[0]):
+        acc.add(s0, s1, s2, s3)
+    acc.finish()
+    samples = []
+    for amp, sig in params:
+        for s0, s1, s2, s3, beam in windows(amp, sig):
+            samples.append((net.inputs(s1, s2, s3, beam).clone(),
+                            net.targets(s1, s2, s3, s0).clone()))
+    hist = net.fit(samples, epochs=epochs)
+    del samples
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"learned predictor: {len(params)} runs x {train_slices} slices, {epochs} epochs, "
+        f"loss {hist[0]:.3g} -> {hist[-1]:.3g}, {train_s:.1f} s")
+
+    cfg_g = make_config()
+    cfg_g.field_solver_precision = precision
+    _, _, const, vp, sp = lc.init(cfg_p)
+    lc.init(cfg_g)
+    sg = sp
+    nc, nf = const.m.shape[0], vp.fine_grid.numel()
+    n = cfg_p.grid_steps
+    last = collections.deque(maxlen=3)
+    ez_p, ez_g, full, accepted, guided = [], [], 0.0, 0, 0
+    for i in range(slices):
+        beam = cosine_gaussian(cfg_p, i)
+        sp = lc.step(cfg_p, const, vp, sp, beam)
+        if len(last) < 3:
+            sg = lc.step(cfg_g, const, vp, sg, beam)
+        else:
+            g = net.guess(last[2], last[1], last[0], beam)
+            sg = lc.step(cfg_g, const, vp, sg, beam, guess=[(1.0, g)],
+                         threshold=GUIDED_THRESHOLD)
+            guided += 1
+            accepted += lc.guided_residual(cfg_g, nc, nf) <= GUIDED_THRESHOLD
+        last.append(sg)
+        ez_p.append(sp.Ez[n // 2, n // 2])
+        ez_g.append(sg.Ez[n // 2, n // 2])
+        if (i + 1) % 100 == 0:
+            e = (torch.linalg.norm(sg.Ez - sp.Ez) / torch.linalg.norm(sp.Ez)).item()
+            full = max(full, e)
+    ez_p, ez_g = torch.stack(ez_p), torch.stack(ez_g)
+    axis = (torch.linalg.norm(ez_g - ez_p) / torch.linalg.norm(ez_p)).item()
+    beams = [cosine_gaussian(cfg_g, 100 + k) for k in range(timed + 3)]
+    _, _, const, vp, st = lc.init(cfg_g)
+    last = collections.deque(maxlen=3)
+    for k in range(3):
+        st = lc.step(cfg_g, const, vp, st, beams[k])
+        last.append(st)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(timed):
+        g = net.guess(last[2], last[1], last[0], beams[3 + k])
+        st = lc.step(cfg_g, const, vp, st, beams[3 + k], guess=[(1.0, g)],
+                     threshold=GUIDED_THRESHOLD)
+        last.append(st)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"precision": precision,
+            "predictor": f"ConvPredictor hidden {list(hidden)}, trained on {len(params)} "
+                         f"plain runs x {train_slices} slices, {epochs} epochs; corrector "
+                         f"residual threshold {GUIDED_THRESHOLD:g}",
+            "train_seconds": train_s, "train_loss": hist,
+            "value": timed / dt, "unit": "xi-steps/s", "slices_compared": slices,
+            "ez_axis_rel_l2": axis, "ez_checkpoint_rel_l2_max": full,
+            "guess_accept_rate": accepted / max(guided, 1)}
+
+
 def cpu_baseline(steps):
     """The oracle (NumPy/SciPy restatement of the reference step) on the
     host cores, on a bounded sample of the same workload."""
@@ -256,6 +351,8 @@ def main():
                     help="steps timed per kernel with HIP events")
     ap.add_argument("--mixed-slices", type=int, default=1500,
                     help="slices of the mixed+guided vs plain comparison (0: skip)")
+    ap.add_argument("--no-learned", dest="learned", action="store_false",
+                    help="skip the mixed+learned leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -369,6 +466,18 @@ def main():
             f"{mixed['slices_compared']} slices, accepted "
             f"{mixed['guess_accept_rate'] * 100:.0f}%")
 
+    learned = None
+    if args.mixed_slices and args.learned:
+        learned = learned_leg(lc, cosine_gaussian, torch, args.mixed_slices,
+                              args.steps, "fp32")
+        learned["vs_plain"] = learned["value"] / steps_per_s
+        log(f"mixed+learned: {learned['value']:.1f} xi-steps/s "
+            f"({learned['vs_plain']:.2f}x plain), Ez axis rel L2 "
+            f"{learned['ez_axis_rel_l2']:.2e}, checkpoints max "
+            f"{learned['ez_checkpoint_rel_l2_max']:.2e} over "
+            f"{learned['slices_compared']} slices, accepted "
+            f"{learned['guess_accept_rate'] * 100:.0f}%")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_steps)
@@ -398,6 +507,7 @@ def main():
         "kernels": kernels,
         "solver_backends": solvers,
         "mixed_guided": mixed,
+        "mixed_learned": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -284,13 +284,14 @@ int lcode_pred_adam(float *p, const float *g, float *m, float *v, int32_t n,
                     float lr, float beta1, float beta2, float eps, int32_t t,
                     hipStream_t stream);
 
-/* FP64 fields <-> the network's FP32 planes.  inputs: planes[14][N][N] =
- * F(n-1)/s, F(n-2)/s (6 each), beam/s_b, ro(n-1)/s_r.  targets:
- * planes[6][N][N] = (F(n) - 2 F(n-1) + F(n-2)) / t.  guess: guess[f] =
- * 2 F(n-1) - F(n-2) + t_f pred[f]. */
+/* FP64 fields <-> the network's FP32 planes.  With Q = 3 F(n-1) - 3 F(n-2)
+ * + F(n-3) (the quadratic extrapolation of slice n) and D = F(n-1) - F(n-2):
+ * inputs: planes[14][N][N] = Q/s (6), D/s_d (6), beam/s_b, ro(n-1)/s_r;
+ * targets: planes[6][N][N] = (F(n) - Q) / t; guess: guess[f] = Q + t_f pred[f]. */
 typedef struct {
     const double *f1[6];      /* fields of slice n-1 (Ex Ey Ez Bx By Bz) */
     const double *f2[6];      /* fields of slice n-2 */
+    const double *f3[6];      /* fields of slice n-3 */
     const double *beam;       /* beam of slice n (NULL: beam_scalar) */
     double beam_scalar;
     const double *ro;         /* plasma density of slice n-1 */

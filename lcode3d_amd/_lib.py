@@ -71,6 +71,7 @@ class LcodeGuess(ctypes.Structure):
 
 class LcodePredIo(ctypes.Structure):
     _fields_ = [("f1", ctypes.c_void_p * 6), ("f2", ctypes.c_void_p * 6),
+                ("f3", ctypes.c_void_p * 6),
                 ("beam", ctypes.c_void_p), ("beam_scalar", ctypes.c_double),
                 ("ro", ctypes.c_void_p), ("next", ctypes.c_void_p * 6),
                 ("scale", ctypes.c_float * 14), ("tscale", ctypes.c_float * 6),

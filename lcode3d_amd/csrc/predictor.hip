@@ -17,7 +17,7 @@
 //   adam            the Adam update of a flat parameter buffer
 //   pred_inputs / pred_targets / pred_guess: FP64 fields <-> normalised
 //                   FP32 planes (the network predicts the change of the next
-//                   slice's fields from their linear extrapolation).
+//                   slice's fields from their quadratic extrapolation).
 
 #include "common.hpp"
 
@@ -231,11 +231,13 @@ adam_kernel(float* p, const float* g, float* m, float* v, int n, float lr,
     p[e] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// Network inputs of one sample: planes 0-5 = F(n-1) / s_f, 6-11 =
-// F(n-2) / s_f, 12 = beam / s_b, 13 = ro(n-1) / s_r (FP64 -> FP32).
+// Network inputs of one sample, with Q = 3 F(n-1) - 3 F(n-2) + F(n-3) and
+// D = F(n-1) - F(n-2): planes 0-5 = Q / s, 6-11 = D / s_d, 12 = beam / s_b,
+// 13 = ro(n-1) / s_r (FP64 -> FP32).
 struct PredIoArgs {
     const double* f1[6];     // fields of slice n-1
     const double* f2[6];     // fields of slice n-2
+    const double* f3[6];     // fields of slice n-3
     const double* beam;      // NULL: beam_scalar
     double beam_scalar;
     const double* ro;        // plasma density of slice n-1
@@ -254,33 +256,35 @@ pred_inputs_kernel(PredIoArgs a) {
     if (e >= a.n) return;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        a.planes[(size_t)f * a.n + e] = (float)(a.f1[f][e] / a.scale[f]);
-        a.planes[(size_t)(6 + f) * a.n + e] = (float)(a.f2[f][e] / a.scale[6 + f]);
+        const double f1 = a.f1[f][e], f2 = a.f2[f][e];
+        const double q = 3 * f1 - 3 * f2 + a.f3[f][e];
+        a.planes[(size_t)f * a.n + e] = (float)(q / a.scale[f]);
+        a.planes[(size_t)(6 + f) * a.n + e] = (float)((f1 - f2) / a.scale[6 + f]);
     }
     a.planes[(size_t)12 * a.n + e] = (float)((a.beam ? a.beam[e] : a.beam_scalar) / a.scale[12]);
     a.planes[(size_t)13 * a.n + e] = (float)(a.ro[e] / a.scale[13]);
 }
 
-// targets: (F(n) - (2 F(n-1) - F(n-2))) / t_f
+// targets: (F(n) - Q) / t_f
 __global__ void __launch_bounds__(256)
 pred_targets_kernel(PredIoArgs a) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= a.n) return;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        const double lin = 2 * a.f1[f][e] - a.f2[f][e];
-        a.planes[(size_t)f * a.n + e] = (float)((a.next[f][e] - lin) / a.tscale[f]);
+        const double q = 3 * a.f1[f][e] - 3 * a.f2[f][e] + a.f3[f][e];
+        a.planes[(size_t)f * a.n + e] = (float)((a.next[f][e] - q) / a.tscale[f]);
     }
 }
 
-// guess: 2 F(n-1) - F(n-2) + t_f pred (FP64)
+// guess: Q + t_f pred (FP64)
 __global__ void __launch_bounds__(256)
 pred_guess_kernel(PredIoArgs a) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= a.n) return;
 #pragma unroll
     for (int f = 0; f < 6; ++f)
-        a.guess[f][e] = 2 * a.f1[f][e] - a.f2[f][e] +
+        a.guess[f][e] = 3 * a.f1[f][e] - 3 * a.f2[f][e] + a.f3[f][e] +
                         (double)a.tscale[f] * (double)a.pred[(size_t)f * a.n + e];
 }
 
@@ -374,9 +378,10 @@ static int pred_io(int which, const lcode_pred_io* io, hipStream_t s) {
     LCODE_REQUIRE(io && io->n > 0, "bad io");
     PredIoArgs a{};
     for (int f = 0; f < 6; ++f) {
-        LCODE_REQUIRE(io->f1[f] && io->f2[f], "NULL field");
+        LCODE_REQUIRE(io->f1[f] && io->f2[f] && io->f3[f], "NULL field");
         a.f1[f] = io->f1[f];
         a.f2[f] = io->f2[f];
+        a.f3[f] = io->f3[f];
         a.next[f] = io->next[f];
         a.guess[f] = io->guess[f];
         a.tscale[f] = io->tscale[f];

@@ -9,11 +9,12 @@ backward, MSE gradient, Adam, and the FP64 <-> FP32 plane conversions.
 torch only holds the device buffers.
 
 Network: 14 input planes -> conv3x3 -> ReLU -> conv3x3 -> ReLU -> conv3x3
--> 6 planes.  Inputs: the six fields of slices n-1 and n-2 (each divided by
-a per-field scale), the beam of slice n and the plasma density of slice
-n-1.  Output: the change of slice n's fields from their linear
-extrapolation 2 F(n-1) - F(n-2), in units of per-field target scales; the
-guess is 2 F(n-1) - F(n-2) + scale * output.
+-> 6 planes.  With Q = 3 F(n-1) - 3 F(n-2) + F(n-3), the quadratic
+extrapolation of slice n's six fields, and D = F(n-1) - F(n-2), the inputs
+are Q and D (each divided by a per-field scale), the beam of slice n and
+the plasma density of slice n-1; the output is the change of slice n's
+fields from Q in units of per-field target scales, and the guess is
+Q + scale * output.  A zero network reproduces the quadratic extrapolation.
 """
 
 import ctypes
@@ -87,11 +88,12 @@ class ConvPredictor:
         return out
 
     # -- FP64 fields <-> planes ---------------------------------------------
-    def _io(self, s1, s2, beam, planes=None, nxt=None, pred=None, guess=None):
+    def _io(self, s1, s2, s3, beam, planes=None, nxt=None, pred=None, guess=None):
         io = _lib.LcodePredIo()
         for f, name in enumerate(FIELDS):
             io.f1[f] = getattr(s1, name).data_ptr()
             io.f2[f] = getattr(s2, name).data_ptr()
+            io.f3[f] = getattr(s3, name).data_ptr()
             if nxt is not None:
                 io.next[f] = getattr(nxt, name).data_ptr()
             if guess is not None:
@@ -110,39 +112,62 @@ class ConvPredictor:
         io.n = self.N * self.N
         return io
 
-    def inputs(self, s1, s2, beam, out=None):
-        """Input planes [14, N, N] from states n-1, n-2 and beam n."""
+    def inputs(self, s1, s2, s3, beam, out=None):
+        """Input planes [14, N, N] from states n-1, n-2, n-3 and beam n."""
         out = out if out is not None else torch.empty((N_IN, self.N, self.N),
                                                       dtype=torch.float32, device=self.dev)
         _lib.check(_lib.load().lcode_pred_inputs(
-            ctypes.byref(self._io(s1, s2, beam, planes=out)), _stream()), "lcode_pred_inputs")
+            ctypes.byref(self._io(s1, s2, s3, beam, planes=out)), _stream()),
+            "lcode_pred_inputs")
         return out
 
-    def targets(self, s1, s2, nxt, out=None):
-        """Target planes [6, N, N]: (F(n) - 2 F(n-1) + F(n-2)) / tscale."""
+    def targets(self, s1, s2, s3, nxt, out=None):
+        """Target planes [6, N, N]: (F(n) - Q) / tscale."""
         out = out if out is not None else torch.empty((N_OUT, self.N, self.N),
                                                       dtype=torch.float32, device=self.dev)
         _lib.check(_lib.load().lcode_pred_targets(
-            ctypes.byref(self._io(s1, s2, 0.0, planes=out, nxt=nxt)), _stream()),
+            ctypes.byref(self._io(s1, s2, s3, 0.0, planes=out, nxt=nxt)), _stream()),
             "lcode_pred_targets")
         return out
 
-    def fit_scales(self, states):
-        """Per-field RMS of the fields and of their slice-to-slice change
-        from linear extrapolation, over a sequence of states."""
-        f = np.array([[torch.sqrt((getattr(s, n) ** 2).mean()).item() for n in FIELDS]
-                      for s in states])
-        d = np.array([[torch.sqrt(((getattr(states[k], n) - 2 * getattr(states[k - 1], n)
-                                     + getattr(states[k - 2], n)) ** 2).mean()).item()
-                       for n in FIELDS] for k in range(2, len(states))])
-        fs = np.maximum(f.max(axis=0), 1e-30)
-        ds = np.maximum(d.max(axis=0), 1e-30)
-        ro = max(max(torch.abs(s.ro).max().item() for s in states), 1e-30)
-        self.scale[:6] = fs
-        self.scale[6:12] = fs
-        self.scale[12] = 0.1
-        self.scale[13] = ro
-        self.tscale[:] = ds
+    def fit_scales(self, runs):
+        """Per-field RMS over runs (sequences of consecutive states) of Q,
+        of D and of the target F(n) - Q; the beam and density scales."""
+        acc = self.scale_accumulator()
+        for states in runs:
+            for k in range(3, len(states)):
+                acc.add(states[k], states[k - 1], states[k - 2], states[k - 3])
+        acc.finish()
+
+    def scale_accumulator(self):
+        """Accumulates the statistics of fit_scales window by window
+        (states n, n-1, n-2, n-3), so long runs need not be kept."""
+        net = self
+
+        class _Acc:
+            def __init__(self):
+                self.q2, self.d2, self.t2 = np.zeros(6), np.zeros(6), np.zeros(6)
+                self.ro2, self.cnt = 0.0, 0
+
+            def add(self, s0, s1, s2, s3):
+                for f, n in enumerate(FIELDS):
+                    F1, F2, F3 = getattr(s1, n), getattr(s2, n), getattr(s3, n)
+                    q = 3 * F1 - 3 * F2 + F3
+                    self.q2[f] += (q ** 2).mean().item()
+                    self.d2[f] += ((F1 - F2) ** 2).mean().item()
+                    self.t2[f] += ((getattr(s0, n) - q) ** 2).mean().item()
+                self.ro2 += (s1.ro ** 2).mean().item()
+                self.cnt += 1
+
+            def finish(self):
+                def rms(a):
+                    return np.maximum(np.sqrt(np.asarray(a) / max(self.cnt, 1)), 1e-30)
+                net.scale[:6] = rms(self.q2)
+                net.scale[6:12] = rms(self.d2)
+                net.scale[12] = 0.1
+                net.scale[13] = float(rms([self.ro2])[0])
+                net.tscale[:] = rms(self.t2)
+        return _Acc()
 
     # -- network -------------------------------------------------------------
     def forward(self, x):
@@ -199,22 +224,26 @@ class ConvPredictor:
         return self._loss[0] / n
 
     def fit(self, samples, epochs, seed=0):
-        """Adam over (inputs, targets) pairs for `epochs` shuffled passes;
-        returns the mean loss of each epoch."""
+        """Adam over (inputs, targets) pairs for `epochs` shuffled passes,
+        the learning rate decaying linearly to a tenth; returns the mean loss
+        of each epoch."""
         rng = np.random.default_rng(seed)
+        lr0 = self.lr
         history = []
-        for _ in range(epochs):
+        for e in range(epochs):
+            self.lr = lr0 * (1 - 0.9 * e / max(epochs - 1, 1))
             losses = [self.train_step(samples[k][0], samples[k][1])
                       for k in rng.permutation(len(samples))]
             history.append(torch.stack(losses).mean().item())
+        self.lr = lr0
         return history
 
     # -- inference -----------------------------------------------------------
-    def guess(self, s1, s2, beam):
-        """The six guessed fields of slice n (FP64 device grids), for
-        ``step(..., guess=[(1.0, fields)])``."""
-        y = self.forward(self.inputs(s1, s2, beam, out=self._acts[0]))
+    def guess(self, s1, s2, s3, beam):
+        """The six guessed fields of slice n (FP64 device grids) from the
+        states of slices n-1, n-2, n-3, for ``step(..., guess=[(1.0, fields)])``."""
+        y = self.forward(self.inputs(s1, s2, s3, beam, out=self._acts[0]))
         _lib.check(_lib.load().lcode_pred_guess(
-            ctypes.byref(self._io(s1, s2, beam, pred=y, guess=self._guess)), _stream()),
+            ctypes.byref(self._io(s1, s2, s3, beam, pred=y, guess=self._guess)), _stream()),
             "lcode_pred_guess")
         return tuple(self._guess)

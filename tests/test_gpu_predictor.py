@@ -8,9 +8,9 @@
 * plane conversions (inputs, targets, guess) against their formulas;
 * at 129^2 (grid_step_size 0.05, xi_step_size 0.05) a network trained on
   three plain runs (amplitude and sigma drawn from default_rng(0)) guesses
-  the next slice's fields of a fourth, held-out run better than the linear
-  extrapolation it starts from; the test prints its error beside the
-  linear and quadratic extrapolations'.
+  the next slice's fields of a fourth, held-out run better than the
+  quadratic extrapolation it starts from (a zero network); the test prints
+  its error beside the linear and quadratic extrapolations'.
 """
 
 import types
@@ -109,25 +109,27 @@ def test_plane_conversions(pred_mod):
         return types.SimpleNamespace(**{n: torch.as_tensor(rng.standard_normal((N, N)),
                                                            device="cuda")
                                         for n in pred_mod.FIELDS + ("ro",)})
-    s1, s2, s0 = state(), state(), state()
+    s1, s2, s3, s0 = state(), state(), state(), state()
     beam = torch.as_tensor(rng.standard_normal((N, N)), device="cuda")
-    x = host(net.inputs(s1, s2, beam))
+
+    def quad(n):
+        return 3 * host(getattr(s1, n)) - 3 * host(getattr(s2, n)) + host(getattr(s3, n))
+    x = host(net.inputs(s1, s2, s3, beam))
     for f, n in enumerate(pred_mod.FIELDS):
-        assert np.allclose(x[f], host(getattr(s1, n)) / net.scale[f], rtol=1e-6)
-        assert np.allclose(x[6 + f], host(getattr(s2, n)) / net.scale[6 + f], rtol=1e-6)
+        assert np.allclose(x[f], quad(n) / net.scale[f], rtol=1e-6)
+        assert np.allclose(x[6 + f], (host(getattr(s1, n)) - host(getattr(s2, n)))
+                           / net.scale[6 + f], rtol=1e-6)
     assert np.allclose(x[12], host(beam) / net.scale[12], rtol=1e-6)
     assert np.allclose(x[13], host(s1.ro) / net.scale[13], rtol=1e-6)
-    t = host(net.targets(s1, s2, s0))
+    t = host(net.targets(s1, s2, s3, s0))
     for f, n in enumerate(pred_mod.FIELDS):
-        want = (host(getattr(s0, n)) - 2 * host(getattr(s1, n)) + host(getattr(s2, n))) \
-            / net.tscale[f]
+        want = (host(getattr(s0, n)) - quad(n)) / net.tscale[f]
         assert np.allclose(t[f], want, rtol=1e-5, atol=1e-6)
-    # guess of a zero network output is the linear extrapolation
+    # the guess of a zero network is the quadratic extrapolation
     net.params.zero_()
-    g = net.guess(s1, s2, beam)
+    g = net.guess(s1, s2, s3, beam)
     for f, n in enumerate(pred_mod.FIELDS):
-        assert np.allclose(host(g[f]), 2 * host(getattr(s1, n)) - host(getattr(s2, n)),
-                           rtol=0, atol=1e-12)
+        assert np.allclose(host(g[f]), quad(n), rtol=0, atol=1e-12)
 
 
 def _cfg129():
@@ -150,7 +152,7 @@ def _run(lc, cfg, amplitude, sigma, slices):
     return states, beams
 
 
-def test_trained_network_beats_linear_extrapolation(pred_mod):
+def test_trained_network_beats_extrapolation(pred_mod):
     import lcode3d_amd as lc
     say("predictor training at 129^2")
     cfg = _cfg129()
@@ -158,19 +160,20 @@ def test_trained_network_beats_linear_extrapolation(pred_mod):
     params = [(rng.uniform(.01, .1), rng.uniform(.5, 2)) for _ in range(4)]
     runs = [_run(lc, cfg, a, s, 120) for a, s in params]
     net = pred_mod.ConvPredictor(129, hidden=(16, 16), seed=0, lr=2e-3)
-    net.fit_scales([s for states, _ in runs[:3] for s in states[20:]])
+    net.fit_scales([states[20:] for states, _ in runs[:3]])
     samples = []
     for states, beams in runs[:3]:
-        for n in range(22, 120):
-            samples.append((net.inputs(states[n - 1], states[n - 2], beams[n]).clone(),
-                            net.targets(states[n - 1], states[n - 2], states[n]).clone()))
+        for n in range(23, 120):
+            s1, s2, s3 = states[n - 1], states[n - 2], states[n - 3]
+            samples.append((net.inputs(s1, s2, s3, beams[n]).clone(),
+                            net.targets(s1, s2, s3, states[n]).clone()))
     hist = net.fit(samples, epochs=8)
     say("  loss by epoch:", [f"{h:.3g}" for h in hist])
     states, beams = runs[3]
     err = {"learned": [], "linear": [], "quadratic": []}
-    for n in range(22, 120):
+    for n in range(23, 120):
         s1, s2, s3, s0 = states[n - 1], states[n - 2], states[n - 3], states[n]
-        g = net.guess(s1, s2, beams[n])
+        g = net.guess(s1, s2, s3, beams[n])
         num = {"learned": 0., "linear": 0., "quadratic": 0.}
         den = 0.
         for f, name in enumerate(pred_mod.FIELDS):
@@ -182,7 +185,8 @@ def test_trained_network_beats_linear_extrapolation(pred_mod):
         for k in err:
             err[k].append((num[k] / den) ** .5)
     mean = {k: float(np.mean(v)) for k, v in err.items()}
-    say("  held-out guess rel L2 (mean over 98 slices):",
+    say("  held-out guess rel L2 (mean over 97 slices):",
         {k: f"{v:.3e}" for k, v in mean.items()})
-    assert hist[-1] < 0.5 * hist[0]
-    assert mean["learned"] < mean["linear"]
+    assert hist[-1] < hist[0]
+    # measured: learned 2.1e-4, quadratic 3.3e-4, linear 2.9e-3
+    assert mean["learned"] < 0.9 * mean["quadratic"]
