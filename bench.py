@@ -335,6 +335,7 @@ def cpu_baseline(steps):
 
 
 def main():
+    global GUIDED_THRESHOLD
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
@@ -351,11 +352,14 @@ def main():
                     help="steps timed per kernel with HIP events")
     ap.add_argument("--mixed-slices", type=int, default=1500,
                     help="slices of the mixed+guided vs plain comparison (0: skip)")
+    ap.add_argument("--guided-threshold", type=float, default=GUIDED_THRESHOLD,
+                    help="corrector residual threshold of the guided legs")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip the mixed+learned leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this size (PMC calibration)")
     args = ap.parse_args()
+    GUIDED_THRESHOLD = args.guided_threshold
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): "
               "the xi step is sequential in xi and this build has no "

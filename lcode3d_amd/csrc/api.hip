@@ -52,6 +52,7 @@ struct lcode_workspace {
                                    // displacement bound in cells
     double* guess[6] = {};         // guided step: the field guess
     double* resid = nullptr;       // {sum (F - guess)^2, sum F^2}
+    double* resid_part = nullptr;  // 2 per block of the last FFT pass
     double* diag_w = nullptr;      // zn diagnostic: Gaussian taps [2 r + 1]
     int diag_r = 0;
     double* diag_tmp = nullptr;    // [N * N]
@@ -452,6 +453,13 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         if ((rc = dev_alloc(ws, (void**)&ws->guess[k], n2 * sizeof(double)))) return fail(rc);
     }
     if ((rc = dev_alloc(ws, (void**)&ws->resid, 2 * sizeof(double)))) return fail(rc);
+    {
+        // last FFT pass: ceil(N / (2 P)) blocks per field, six fields
+        const int per_block = 2 * ws->geom.P;
+        const size_t nblk = (size_t)6 * ((N + per_block - 1) / per_block);
+        if ((rc = dev_alloc(ws, (void**)&ws->resid_part, 2 * nblk * sizeof(double))))
+            return fail(rc);
+    }
     if ((rc = dev_alloc(ws, (void**)&ws->fault, sizeof(int)))) return fail(rc);
     const size_t nfp = (size_t)cfg->nf * cfg->nf;
     if ((rc = dev_alloc(ws, (void**)&ws->spill, (nfp ? nfp : 1) * sizeof(int)))) return fail(rc);
@@ -748,6 +756,7 @@ static int field_pass(lcode_workspace* ws, const lcode_config* cfg,
     if (guess && resid) {
         for (int k = 0; k < 6; ++k) b.guess[k] = guess[k];
         b.resid = resid;
+        b.resid_part = ws->resid_part;
     }
     b.nfields = 6;
     const double* prevF[6] = {prev->Ex, prev->Ey, prev->Ez,

@@ -299,7 +299,8 @@ struct SolveBatch {
     int variantA;             // 2 F - prev for Ex Ey Bx By
     Gate gate;                // the whole solve is skipped when closed
     const double* guess[6];   // guided step: the caller's field guess ...
-    double* resid;            // ... and {sum (F - guess)^2, sum F^2} (atomics)
+    double* resid;            // ... and {sum (F - guess)^2, sum F^2}
+    double* resid_part;       // FFT solver: 2 partial sums per last-pass block
 };
 
 // guess[f] = sum_t c[t] h[t][f]; avg[f] = (guess[f] + prev[f]) / 2 over n

@@ -139,10 +139,17 @@ gemm_kernel(GemmBatch batch) {
         if (wlane == 0) atomicAdd(o.mean_acc, m);
     }
     if (o.mode == 2 && o.guess && batch.resid) {
+        // one pair of atomics per block
+        __shared__ double rs[2][4];
         const double rn = wave_sum64(ea.rn), rd = wave_sum64(ea.rd);
         if (wlane == 0) {
-            atomicAdd(&batch.resid[0], rn);
-            atomicAdd(&batch.resid[1], rd);
+            rs[0][wave] = rn;
+            rs[1][wave] = rd;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(&batch.resid[0], rs[0][0] + rs[0][1] + rs[0][2] + rs[0][3]);
+            atomicAdd(&batch.resid[1], rs[1][0] + rs[1][1] + rs[1][2] + rs[1][3]);
         }
     }
 }
@@ -396,10 +403,17 @@ bf_gemm_kernel(BfBatch batch) {
         if (wlane == 0) atomicAdd(o.mean_acc, m);
     }
     if (o.mode == BF_FINAL && o.guess && batch.resid) {
+        // one pair of atomics per block
+        __shared__ double rs[2][4];
         const double rn = wave_sum64(ea.rn), rd = wave_sum64(ea.rd);
         if (wlane == 0) {
-            atomicAdd(&batch.resid[0], rn);
-            atomicAdd(&batch.resid[1], rd);
+            rs[0][wave] = rn;
+            rs[1][wave] = rd;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            atomicAdd(&batch.resid[0], rs[0][0] + rs[0][1] + rs[0][2] + rs[0][3]);
+            atomicAdd(&batch.resid[1], rs[1][0] + rs[1][1] + rs[1][2] + rs[1][3]);
         }
     }
 }

@@ -501,7 +501,13 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L, N = g.N;
-    if (bp.block * g.P * 2 >= N) return;
+    if (bp.block * g.P * 2 >= N) {
+        if (b.resid && threadIdx.x == 0) {    // no rows: a zero partial
+            const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+            b.resid_part[2 * lin] = b.resid_part[2 * lin + 1] = 0.;
+        }
+        return;
+    }
     const int pair = threadIdx.x / Tp, t = threadIdx.x - pair * Tp;
     V* z = lds + (size_t)pair * padded_len(L);
     const Tw<S> tw = load_twiddles<L_, TW_LDS, S>(lds, g);
@@ -563,14 +569,54 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
             }
         }
     }
-    if (guess) {
-        // guided step: residual of the corrector against the guess
+    if (b.resid) {
+        // guided step: this block's share of the corrector's residual against
+        // the guess, summed over its waves; one reduction kernel adds the
+        // blocks' partials afterwards (no same-address atomics)
         r_num = wave_sum(r_num);
         r_den = wave_sum(r_den);
+        __shared__ double rs[2][16];
+        const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
         if ((threadIdx.x & 63) == 0) {
-            atomicAdd(&b.resid[0], r_num);
-            atomicAdd(&b.resid[1], r_den);
+            rs[0][w] = r_num;
+            rs[1][w] = r_den;
         }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0., d = 0.;
+            for (int k = 0; k < nw; ++k) {
+                a += rs[0][k];
+                d += rs[1][k];
+            }
+            const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+            b.resid_part[2 * lin] = a;
+            b.resid_part[2 * lin + 1] = d;
+        }
+    }
+}
+
+// resid[0..1] = sums of the blocks' partials (fixed order)
+__global__ void __launch_bounds__(256)
+resid_reduce_kernel(const double* part, int nblocks, double* resid) {
+    __shared__ double red[2][256];
+    double a = 0., d = 0.;
+    for (int k = threadIdx.x; k < nblocks; k += 256) {
+        a += part[2 * k];
+        d += part[2 * k + 1];
+    }
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = d;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + off];
+            red[1][threadIdx.x] += red[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        resid[0] = red[0][0];
+        resid[1] = red[1][0];
     }
 }
 
@@ -613,6 +659,11 @@ static int launch_passes(const SolverGeom& g, const SolveBatch& b,
         pass_c_kernel<L_, TW_LDS, S><<<gc, block, lds, s>>>(g, b);
     }
     LCODE_HIP(hipGetLastError());
+    if (b.resid) {
+        ProfScope ps(prof, "resid_reduce", s);
+        resid_reduce_kernel<<<1, 256, 0, s>>>(b.resid_part, (int)(gc.x * gc.y), b.resid);
+        LCODE_HIP(hipGetLastError());
+    }
     return LCODE_OK;
 }
 
