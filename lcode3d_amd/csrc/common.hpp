@@ -118,9 +118,23 @@ __device__ __forceinline__ int cell_of(double x, int N, double h) {
 // (lcode.py:404), or x * inv_h + .5 when inv_h is given (used by the
 // deposition; a 1-ulp change of x_h moves no weight measurably: the TSC
 // weights are continuous across cell edges).
+// The nine weights from the in-cell offsets (x_loc, y_loc) in [-.5, .5).
+__device__ __forceinline__ void tsc_w(double x_loc, double y_loc, double* w) {
+    const double wx0 = .75 - x_loc * x_loc, wy0 = .75 - y_loc * y_loc;
+    const double wxP = (.5 + x_loc) * (.5 + x_loc) / 2;
+    const double wyP = (.5 + y_loc) * (.5 + y_loc) / 2;
+    const double wxM = (.5 - x_loc) * (.5 - x_loc) / 2;
+    const double wyM = (.5 - y_loc) * (.5 - y_loc) / 2;
+    w[0] = wxM * wyP; w[1] = wx0 * wyP; w[2] = wxP * wyP;
+    w[3] = wxM * wy0; w[4] = wx0 * wy0; w[5] = wxP * wy0;
+    w[6] = wxM * wyM; w[7] = wx0 * wyM; w[8] = wxP * wyM;
+}
+
 __device__ __forceinline__ void tsc_weights(double x, double y, int N,
                                             double h, Tsc& t, int* fault,
-                                            double inv_h = 0.) {
+                                            double inv_h = 0.,
+                                            double* xl = nullptr,
+                                            double* yl = nullptr) {
     const double x_h = inv_h != 0. ? x * inv_h + .5 : x / h + .5;
     const double y_h = inv_h != 0. ? y * inv_h + .5 : y / h + .5;
     const double fx = floor(x_h), fy = floor(y_h);
@@ -133,14 +147,9 @@ __device__ __forceinline__ void tsc_weights(double x, double y, int N,
     }
     t.i = i;
     t.j = j;
-    const double wx0 = .75 - x_loc * x_loc, wy0 = .75 - y_loc * y_loc;
-    const double wxP = (.5 + x_loc) * (.5 + x_loc) / 2;
-    const double wyP = (.5 + y_loc) * (.5 + y_loc) / 2;
-    const double wxM = (.5 - x_loc) * (.5 - x_loc) / 2;
-    const double wyM = (.5 - y_loc) * (.5 - y_loc) / 2;
-    t.w[0] = wxM * wyP; t.w[1] = wx0 * wyP; t.w[2] = wxP * wyP;
-    t.w[3] = wxM * wy0; t.w[4] = wx0 * wy0; t.w[5] = wxP * wy0;
-    t.w[6] = wxM * wyM; t.w[7] = wx0 * wyM; t.w[8] = wxP * wyM;
+    tsc_w(x_loc, y_loc, t.w);
+    if (xl) *xl = x_loc;
+    if (yl) *yl = y_loc;
 }
 
 // 9-point gather in the reference's summation order (lcode.py:421-430).
