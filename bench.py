@@ -308,6 +308,45 @@ def learned_leg(lc, cosine_gaussian, torch, slices, timed, precision,
             "guess_accept_rate": accepted / max(guided, 1)}
 
 
+def precision_table(lc, cosine_gaussian, torch, slices):
+    """Error of the mixed-precision field solvers against the plain FP64
+    step (itself within 1e-10 of the FP64 oracle) on the 641^2 workload for
+    beam amplitudes 0.01, 0.05, 0.1 x sigma 0.5, 1, 2: relative L2 of each
+    field and of the momenta, worst over checkpoints every 100 slices."""
+    names = ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "px", "py", "pz")
+    modes = {"fft-fp32": ("fft", "fp32"), "dense-bf16x3": ("dense", "bf16x3")}
+    table = []
+    for amp in (0.01, 0.05, 0.1):
+        for sig in (0.5, 1.0, 2.0):
+            cfgs = {"fp64": make_config()}
+            for m, (t, p) in modes.items():
+                c = make_config()
+                c.field_solver_transform, c.field_solver_precision = t, p
+                cfgs[m] = c
+            states, const, vp = {}, None, None
+            for m, c in cfgs.items():
+                _, _, const, vp, states[m] = lc.init(c)
+            worst = {m: {n: 0.0 for n in names} for m in modes}
+            for i in range(slices):
+                beam = cosine_gaussian(cfgs["fp64"], i, amplitude=amp, sigma=sig)
+                for m, c in cfgs.items():
+                    states[m] = lc.step(c, const, vp, states[m], beam)
+                if (i + 1) % 100 == 0 or i + 1 == slices:
+                    ref = states["fp64"]
+                    for m in modes:
+                        for n in names:
+                            r = getattr(ref, n)
+                            floor = torch.linalg.norm(ref.Bx) if n == "Bz" else 0
+                            den = max(torch.linalg.norm(r).item(), float(floor))
+                            e = torch.linalg.norm(getattr(states[m], n) - r).item() / max(den, 1e-300)
+                            worst[m][n] = max(worst[m][n], e)
+            row = {"amplitude": amp, "sigma": sig, "slices": slices, "rel_l2_max": worst}
+            table.append(row)
+            log(f"  precision amp {amp:g} sigma {sig:g}: " + "; ".join(
+                f"{m} " + " ".join(f"{n} {worst[m][n]:.1e}" for n in names) for m in modes))
+    return table
+
+
 def cpu_baseline(steps):
     """The oracle (NumPy/SciPy restatement of the reference step) on the
     host cores, on a bounded sample of the same workload."""
@@ -354,6 +393,8 @@ def main():
                     help="slices of the mixed+guided vs plain comparison (0: skip)")
     ap.add_argument("--guided-threshold", type=float, default=GUIDED_THRESHOLD,
                     help="corrector residual threshold of the guided legs")
+    ap.add_argument("--precision-table", type=int, default=0, metavar="SLICES",
+                    help="mixed-precision error table over SLICES slices (0: skip)")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip the mixed+learned leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -482,6 +523,10 @@ def main():
             f"{learned['slices_compared']} slices, accepted "
             f"{learned['guess_accept_rate'] * 100:.0f}%")
 
+    ptable = None
+    if args.precision_table:
+        ptable = precision_table(lc, cosine_gaussian, torch, args.precision_table)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_steps)
@@ -512,6 +557,7 @@ def main():
         "solver_backends": solvers,
         "mixed_guided": mixed,
         "mixed_learned": learned,
+        "precision_table": ptable,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
