@@ -66,6 +66,9 @@ namespace {
 #ifndef LCODE_TW_LDS_OK
 #define LCODE_TW_LDS_OK 1
 #endif
+#ifndef LCODE_SOLVER_MAX_LDS
+#define LCODE_SOLVER_MAX_LDS 65536
+#endif
 
 // Cells a fine particle may move from its initial cell and still be
 // deposited through the LDS tiles (beyond: the atomic spill path).
@@ -262,7 +265,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     }
     g.Tp = ((L + 3) / 4 + 63) / 64 * 64;
     g.static_L = solver_static_length(L) ? L : 0;
-    const int max_lds = 65536;
+    const int max_lds = LCODE_SOLVER_MAX_LDS;   // per block, bytes
     auto pairs_for = [&](int fixed_bytes) {
         int P = 1024 / g.Tp;
         const int by_lds = (max_lds - fixed_bytes) / (solver_padded_len(L) * cplx);
