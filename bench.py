@@ -347,6 +347,50 @@ def precision_table(lc, cosine_gaussian, torch, slices):
     return table
 
 
+SMALL_WORKLOAD = dict(grid_steps=129, grid_step_size=.05, xi_step_size=.05,
+                      plasma_coarseness=2, plasma_fineness=2,
+                      field_solver_subtraction_trick=1, field_solver_variant_A=True,
+                      reflect_padding_steps=5, plasma_padding_steps=10)
+
+
+def small_workload_leg(lc, cosine_gaussian, torch, with_cpu):
+    """The 129^2 reference-runnable case in full (200 slices of the same
+    beam shape): the HIP step's throughput and, when asked, the oracle's
+    on the host cores, plus their on-axis Ez agreement."""
+    import types
+    cfg = types.SimpleNamespace(**SMALL_WORKLOAD)
+    _, _, const, vp, st = lc.init(cfg)
+    beams = [cosine_gaussian(cfg, i) for i in range(200)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ez = []
+    for i in range(200):
+        st = lc.step(cfg, const, vp, st, beams[i])
+        ez.append(st.Ez[64, 64])
+    torch.cuda.synchronize()
+    gpu = 200 / (time.perf_counter() - t0)
+    out = {"workload": "129^2, coarseness 2, fineness 2, 200 slices", "gpu_value": gpu,
+           "unit": "xi-steps/s"}
+    if with_cpu:
+        import numpy as np
+        from oracle import lcode_oracle as oracle
+        cores = len(os.sched_getaffinity(0))
+        oracle.set_fft_workers(cores)
+        ocfg = types.SimpleNamespace(**SMALL_WORKLOAD)
+        xs, ys, oconst, ovp, ost = oracle.init(ocfg)
+        t0 = time.perf_counter()
+        oez = []
+        for i in range(200):
+            ost = oracle.step(ocfg, oconst, ovp, ost, oracle.beam(ocfg, i, xs, ys))
+            oez.append(ost.Ez[64, 64])
+        out["cpu_value"] = 200 / (time.perf_counter() - t0)
+        out["cpu_cores"] = cores
+        g = torch.stack(ez).cpu().numpy()
+        out["ez_axis_max_rel_diff"] = float(np.abs(g - np.array(oez)).max()
+                                            / np.abs(np.array(oez)).max())
+    return out
+
+
 def cpu_baseline(steps):
     """The oracle (NumPy/SciPy restatement of the reference step) on the
     host cores, on a bounded sample of the same workload."""
@@ -523,6 +567,12 @@ def main():
             f"{learned['slices_compared']} slices, accepted "
             f"{learned['guess_accept_rate'] * 100:.0f}%")
 
+    small = small_workload_leg(lc, cosine_gaussian, torch, not args.no_cpu_baseline)
+    log(f"129^2 x 200 slices: {small['gpu_value']:.1f} xi-steps/s on the GPU"
+        + (f", {small['cpu_value']:.2f} on {small['cpu_cores']} host cores (oracle), "
+           f"Ez(xi,0,0) agreement {small['ez_axis_max_rel_diff']:.1e}"
+           if "cpu_value" in small else ""))
+
     ptable = None
     if args.precision_table:
         ptable = precision_table(lc, cosine_gaussian, torch, args.precision_table)
@@ -558,6 +608,7 @@ def main():
         "mixed_guided": mixed,
         "mixed_learned": learned,
         "precision_table": ptable,
+        "small_workload": small,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

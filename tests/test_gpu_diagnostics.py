@@ -75,3 +75,29 @@ def test_main_driver_matches_step_loop(lc, capsys):
     assert lines[-1].split("|")[1] == diags.diags_peak_msg(ez)
     zn_last = diags.zn(cfg, ref.ro).item()
     assert float(lines[-1].split("zn=")[1]) >= round(zn_last, 3) - 1e-3
+
+
+def test_checkpoint_resume_continues_the_run(lc, tmp_path):
+    """save_state/load_state mid-run: resuming from the file reproduces the
+    uninterrupted run (to 1e-10 of its peak: FP64 atomics round in any order)."""
+    from lcode3d_amd import state_io
+    from lcode3d_amd.beams import cosine_gaussian
+    say("checkpoint/resume 129^2")
+    cfg = _cfg(129, .05)
+    xs, ys, const, vp, st = lc.init(cfg)
+    for i in range(30):
+        st = lc.step(cfg, const, vp, st, cosine_gaussian(cfg, i))
+    path = tmp_path / "slice_29.npz"
+    state_io.save_state(path, st, cfg, xi_i=29, const=const, virt_params=vp)
+    ref = st
+    for i in range(30, 50):
+        ref = lc.step(cfg, const, vp, ref, cosine_gaussian(cfg, i))
+    header, st2, const2, vp2 = state_io.load_state(path)
+    assert header["xi_i"] == 29
+    cfg2 = _cfg(129, .05)
+    cfg2.reflect_boundary = header["config"]["reflect_boundary"]
+    for i in range(header["xi_i"] + 1, 50):
+        st2 = lc.step(cfg2, const2, vp2, st2, cosine_gaussian(cfg2, i))
+    for name in ("Ez", "Ex", "ro", "x_offt", "pz"):
+        a, b = getattr(st2, name), getattr(ref, name)
+        assert torch.allclose(a, b, rtol=0, atol=1e-10 * b.abs().max().item()), name
