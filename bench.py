@@ -111,12 +111,12 @@ def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps,
     import copy
     legs = []
     for transform, precision in (("fft", "fp64"), ("fft", "fp32"),
-                                 ("dense", "fp64"), ("dense", "bf16x3")):
+                                 ("dense", "bf16x3"), ("dense", "fp64")):
         c = copy.copy(cfg)
         c.field_solver_transform = transform
         c.field_solver_precision = precision
         st = state
-        for k in range(3):
+        for k in range(10):
             st = lc.step(c, const, vp, st, beams[k])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
