@@ -664,7 +664,7 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     // after a push of this workspace: a halo of the pushes' displacement
     a.halo_cells = adaptive_halo ? ws->spill_ctr + 2 : nullptr;
     // the tile's coarse rectangle has at most ext / coarseness + 3 rows
-    a.staged = (32 + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= 18;
+    a.staged = (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= 18;
     a.gate = gate;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
     return launch_deposit(a, stream, &ws->prof);

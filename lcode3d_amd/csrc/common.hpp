@@ -82,6 +82,16 @@ __device__ __forceinline__ bool gate_open(const Gate& g) {
     return !(num <= g.thr2 * den);
 }
 
+// Deposition tile side in cells and threads per tile block (build options;
+// 28 x 28 cells with 512 threads measured fastest at 641^2, coarseness 3).
+#ifndef LCODE_DEP_TILE
+#define LCODE_DEP_TILE 28
+#endif
+#ifndef LCODE_DEP_BLOCK
+#define LCODE_DEP_BLOCK 512
+#endif
+constexpr int kDepTile = LCODE_DEP_TILE;
+
 // XCD-aware block order.  Workgroups go to the 8 XCDs round-robin by linear
 // id and each XCD has its own L2: map linear id `lin` of `total` to a
 // logical id such that every XCD gets one contiguous range of logical ids.

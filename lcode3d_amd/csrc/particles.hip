@@ -176,8 +176,8 @@ __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
 // Tiles of the owner-computes deposition: a block owns kTile x kTile cells
 // and deposits, in LDS, every fine particle whose cell moved at most
 // a.halo cells from its initial cell and whose stencil touches the tile.
-constexpr int kTile = 32;
-constexpr int kDepBlock = 512;
+constexpr int kTile = kDepTile;
+constexpr int kDepBlock = LCODE_DEP_BLOCK;
 
 // First fine index whose initial cell is >= c (fg is increasing).
 __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
