@@ -48,7 +48,10 @@ estimate_kernel(EstimateArgs a) {
 // gather, two fixed-point iterations of the half-step momentum, offset and
 // momentum update, reflection with strict > / < flipping the momentum.
 // One-wave blocks spread the (few) coarse particles over all CUs.
-constexpr int kPushBlock = 64;
+#ifndef LCODE_PUSH_BLOCK
+#define LCODE_PUSH_BLOCK 64
+#endif
+constexpr int kPushBlock = LCODE_PUSH_BLOCK;
 
 // One coarse particle's push; returns max(|x_offt|, |y_offt|) after it.
 __device__ __forceinline__ double push_one(const PushArgs& a, int k) {
