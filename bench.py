@@ -441,6 +441,8 @@ def main():
                     help="mixed-precision error table over SLICES slices (0: skip)")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip the mixed+learned leg")
+    ap.add_argument("--no-small-workload", dest="small_workload", action="store_false",
+                    help="skip the 129^2 leg (GPU and oracle)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -567,11 +569,13 @@ def main():
             f"{learned['slices_compared']} slices, accepted "
             f"{learned['guess_accept_rate'] * 100:.0f}%")
 
-    small = small_workload_leg(lc, cosine_gaussian, torch, not args.no_cpu_baseline)
-    log(f"129^2 x 200 slices: {small['gpu_value']:.1f} xi-steps/s on the GPU"
-        + (f", {small['cpu_value']:.2f} on {small['cpu_cores']} host cores (oracle), "
-           f"Ez(xi,0,0) agreement {small['ez_axis_max_rel_diff']:.1e}"
-           if "cpu_value" in small else ""))
+    small = None
+    if args.small_workload:
+        small = small_workload_leg(lc, cosine_gaussian, torch, not args.no_cpu_baseline)
+        log(f"129^2 x 200 slices: {small['gpu_value']:.1f} xi-steps/s on the GPU"
+            + (f", {small['cpu_value']:.2f} on {small['cpu_cores']} host cores (oracle), "
+               f"Ez(xi,0,0) agreement {small['ez_axis_max_rel_diff']:.1e}"
+               if "cpu_value" in small else ""))
 
     ptable = None
     if args.precision_table:
