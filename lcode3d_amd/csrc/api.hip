@@ -665,6 +665,7 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.halo_cells = adaptive_halo ? ws->spill_ctr + 2 : nullptr;
     // the tile's coarse rectangle has at most ext / coarseness + 3 rows
     a.staged = (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= 18;
+    a.group = cfg->plasma_fineness;
     a.gate = gate;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
     return launch_deposit(a, stream, &ws->prof);

@@ -82,15 +82,24 @@ __device__ __forceinline__ bool gate_open(const Gate& g) {
     return !(num <= g.thr2 * den);
 }
 
-// Deposition tile side in cells and threads per tile block (build options;
-// 28 x 28 cells with 512 threads measured fastest at 641^2, coarseness 3).
+// Deposition tile side in cells and threads per tile block (build options).
+// At 641^2, coarseness 3, with grouped particles: 28 x 28 cells with 256
+// threads (three blocks per CU: 165 VGPRs, 51 KiB LDS each) 43.8 us per
+// launch; 512 threads 58.9 us; 24 or 32 cells 49 us.
 #ifndef LCODE_DEP_TILE
 #define LCODE_DEP_TILE 28
 #endif
 #ifndef LCODE_DEP_BLOCK
-#define LCODE_DEP_BLOCK 512
+#define LCODE_DEP_BLOCK 256
 #endif
 constexpr int kDepTile = LCODE_DEP_TILE;
+// 1: each deposition thread takes a group of fine particles (fineness x
+// fineness, one initial cell) and sums successive particles with a common
+// stencil centre in registers before the LDS atomics (43.8 us per launch at
+// 641^2); 0: one thread per fine particle (57.7 us).
+#ifndef LCODE_DEP_GROUP
+#define LCODE_DEP_GROUP 1
+#endif
 
 // XCD-aware block order.  Workgroups go to the 8 XCDs round-robin by linear
 // id and each XCD has its own L2: map linear id `lin` of `total` to a
@@ -215,6 +224,7 @@ struct DepositArgs {
     int* halo_cells;               // NULL, or the pushes' displacement bound
                                    // (read, then reset to 0 by the last block)
     int staged;                    // coarse rectangle of a tile fits LDS
+    int group;                     // fine particles per group side (fineness)
     Gate gate;
 };
 

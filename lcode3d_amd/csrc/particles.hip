@@ -181,6 +181,11 @@ __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
 // a.halo cells from its initial cell and whose stencil touches the tile.
 constexpr int kTile = kDepTile;
 constexpr int kDepBlock = LCODE_DEP_BLOCK;
+#ifdef LCODE_DEP_WAVES
+#define LCODE_DEP_ATTR __attribute__((amdgpu_waves_per_eu(LCODE_DEP_WAVES, LCODE_DEP_WAVES)))
+#else
+#define LCODE_DEP_ATTR
+#endif
 
 // First fine index whose initial cell is >= c (fg is increasing).
 __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
@@ -338,14 +343,11 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         }
         __syncthreads();
         const double sm = a.smallness;
-        for (int p = threadIdx.x; p < ni * nj; p += blockDim.x) {
-            const int li = p / nj, lj = p - li * nj;
-            const int ci = ti.cell[li], cj = tj.cell[lj];   // initial cell
-            const bool home = ci >= ci0 && ci < ci0 + kTile &&
-                              cj >= cj0 && cj < cj0 + kTile;
+        // coarse->fine interpolation of tile-local fine particle (li, lj):
+        // position and deposition values (lcode.py:591-621, :651-655)
+        auto particle = [&](int li, int lj, double& x, double& y, double* v) {
             const double A = ti.ip[li] * tj.ip[lj], B = ti.ip[li] * tj.in[lj];
             const double C = ti.in[li] * tj.ip[lj], D = ti.in[li] * tj.in[lj];
-            double x, y, v[4];
             if constexpr (STAGED) {
                 const int last = kCoarseMax * kCoarseMax - 1;
                 const int pp = min(max((ti.P[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
@@ -370,15 +372,80 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
                                sm * LCODE_MIX(a.pz), v);
 #undef LCODE_MIX
             }
-            Tsc t;
+        };
+        // TSC centre and weights of the particle; false when it moved more
+        // than the halo from its initial cell (its home tile lists it)
+        auto locate = [&](int li, int lj, double x, double y, Tsc& t) {
+            const int ci = ti.cell[li], cj = tj.cell[lj];   // initial cell
+            const bool home = ci >= ci0 && ci < ci0 + kTile &&
+                              cj >= cj0 && cj < cj0 + kTile;
             tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
             if (abs(t.i - ci) > halo || abs(t.j - cj) > halo) {
                 if (home)
                     a.spill[atomicAdd(a.spill_count, 1)] = (fi_a + li) * a.nf + fj_a + lj;
-                continue;
+                return false;
             }
+            return true;
+        };
+#if LCODE_DEP_GROUP
+        // One thread per group of gs x gs fine particles with consecutive
+        // indices (the tile's fine range starts at a cell edge, so with gs =
+        // fineness a group is the particles of one initial cell).  The
+        // thread sums the 3 x 3 x 4 stencil values of successive particles
+        // with a common stencil centre in registers and adds them to LDS
+        // when the centre changes and after the group's last particle.
+        // Adjacent lanes hold adjacent cells, so the atomics of one
+        // instruction hit distinct LDS addresses.
+        const int gs = a.group > 0 ? a.group : 1;
+        const int ngi = (ni + gs - 1) / gs, ngj = (nj + gs - 1) / gs;
+        for (int g = threadIdx.x; g < ngi * ngj; g += blockDim.x) {
+            const int gi = g / ngj, gj = g - gi * ngj;
+            double pat[4][9];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int k = 0; k < 9; ++k) pat[c][k] = 0.;
+            int ai = -1, aj = -1;   // centre of the sums in pat (-1: none)
+            for (int q = 0; q <= gs * gs; ++q) {
+                const int li = gi * gs + q / gs, lj = gj * gs + q % gs;
+                double v[4] = {0., 0., 0., 0.};
+                Tsc t;
+                t.i = t.j = -1;
+                if (q < gs * gs && li < ni && lj < nj) {
+                    double x, y;
+                    particle(li, lj, x, y, v);
+                    if (!locate(li, lj, x, y, t)) t.i = -1;
+                }
+                if (ai >= 0 && (t.i != ai || t.j != aj)) {
+                    if (q == gs * gs || t.i >= 0) {
+                        lds_deposit(acc, ai, aj, ci0, cj0, nullptr, nullptr, pat);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+#pragma unroll
+                            for (int k = 0; k < 9; ++k) pat[c][k] = 0.;
+                        ai = -1;
+                    }
+                }
+                if (t.i >= 0) {
+                    ai = t.i;
+                    aj = t.j;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) pat[c][k] += v[c] * t.w[k];
+                }
+            }
+        }
+#else
+        for (int p = threadIdx.x; p < ni * nj; p += blockDim.x) {
+            const int li = p / nj, lj = p - li * nj;
+            double x, y, v[4];
+            particle(li, lj, x, y, v);
+            Tsc t;
+            if (!locate(li, lj, x, y, t)) continue;
             lds_deposit(acc, t.i, t.j, ci0, cj0, v, t.w, nullptr);
         }
+#endif
     }
     __syncthreads();
     for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
@@ -422,7 +489,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
 }
 
 template <bool STAGED>
-__global__ void __launch_bounds__(kDepBlock)
+__global__ void __launch_bounds__(kDepBlock) LCODE_DEP_ATTR
 deposit_tile_kernel(DepositArgs a) {
     deposit_tile_body<STAGED>(a);
 }
