@@ -48,8 +48,10 @@ struct lcode_workspace {
     double* favg[6] = {};          // field averages of the current pass
     int* fault = nullptr;
     int* spill = nullptr;          // nf * nf spilled fine particle indices
-    int* spill_ctr = nullptr;      // [0] count, [1] blocks done, [2] pushes'
-                                   // displacement bound in cells
+    int* spill_ctr = nullptr;      // [0] count, [1] blocks done, [2], [3]
+                                   // the pushes' displacement bound in
+                                   // cells, two slots used in turn
+    int bound_slot = 0;            // slot the next push raises
     double* guess[6] = {};         // guided step: the field guess
     double* resid = nullptr;       // {sum (F - guess)^2, sum F^2}
     double* resid_part = nullptr;  // 2 per block of the last FFT pass
@@ -466,9 +468,9 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     if ((rc = dev_alloc(ws, (void**)&ws->fault, sizeof(int)))) return fail(rc);
     const size_t nfp = (size_t)cfg->nf * cfg->nf;
     if ((rc = dev_alloc(ws, (void**)&ws->spill, (nfp ? nfp : 1) * sizeof(int)))) return fail(rc);
-    if ((rc = dev_alloc(ws, (void**)&ws->spill_ctr, 3 * sizeof(int)))) return fail(rc);
+    if ((rc = dev_alloc(ws, (void**)&ws->spill_ctr, 4 * sizeof(int)))) return fail(rc);
     if (hipMemset(ws->fault, 0, sizeof(int)) != hipSuccess ||
-        hipMemset(ws->spill_ctr, 0, 3 * sizeof(int)) != hipSuccess ||
+        hipMemset(ws->spill_ctr, 0, 4 * sizeof(int)) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
         set_error("hipMemset of the workspace counters failed");
         return fail(LCODE_E_HIP);
@@ -610,7 +612,7 @@ static int do_push(lcode_workspace* ws, const lcode_config* cfg,
     a.out_x = out[0]; a.out_y = out[1];
     a.out_px = out[2]; a.out_py = out[3]; a.out_pz = out[4];
     a.fault = ws->fault;
-    a.max_cells = ws->spill_ctr + 2;
+    a.max_cells = ws->spill_ctr + 2 + ws->bound_slot;
     a.gate = gate;
     return launch_push(a, stream, &ws->prof);
 }
@@ -662,13 +664,20 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.spill_done = ws->spill_ctr + 1;
     a.halo = kDepositHalo;
     // after a push of this workspace: a halo of the pushes' displacement
-    a.halo_cells = adaptive_halo ? ws->spill_ctr + 2 : nullptr;
+    // bound, read from the slot the pushes raised; the deposition zeroes
+    // the other slot, which the next pushes raise
+    if (adaptive_halo) {
+        a.halo_cells = ws->spill_ctr + 2 + ws->bound_slot;
+        a.halo_next = ws->spill_ctr + 2 + (1 - ws->bound_slot);
+    }
     // the tile's coarse rectangle has at most ext / coarseness + 3 rows
     a.staged = (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= 18;
     a.group = cfg->plasma_fineness;
     a.gate = gate;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
-    return launch_deposit(a, stream, &ws->prof);
+    const int rc = launch_deposit(a, stream, &ws->prof);
+    if (rc == LCODE_OK && adaptive_halo) ws->bound_slot = 1 - ws->bound_slot;
+    return rc;
 }
 
 int lcode_deposit(lcode_workspace* ws, const lcode_config* cfg,

@@ -222,7 +222,8 @@ struct DepositArgs {
     int* spill_done;               // zero between calls
     int halo;                      // tile halo in cells (maximum)
     int* halo_cells;               // NULL, or the pushes' displacement bound
-                                   // (read, then reset to 0 by the last block)
+    int* halo_next;                // NULL, or the bound slot of the next
+                                   // pushes: set to 0 by the deposition
     int staged;                    // coarse rectangle of a tile fits LDS
     int group;                     // fine particles per group side (fineness)
     Gate gate;

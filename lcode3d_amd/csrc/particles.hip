@@ -283,9 +283,17 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     const int N = a.N;
     const double inv_h = 1. / a.h;
     const int ci0 = blockIdx.y * kTile, cj0 = blockIdx.x * kTile;
-    // halo: the displacement bound of the last push when one is given
-    // (particles beyond it take the exact spill path either way)
-    const int halo = a.halo_cells ? min(max(*a.halo_cells, 0), a.halo) : a.halo;
+    // halo: the displacement bound of the last pushes when one is given.
+    // A fine particle's offset is a convex mix of coarse offsets, so its
+    // cell moved at most `bound` cells: with bound <= a.halo no particle
+    // can leave the halo, and every block (all read the same bound) skips
+    // the spill list.  Otherwise particles beyond the halo take the exact
+    // spill path.
+    const int bound = a.halo_cells ? max(*a.halo_cells, 0) : a.halo + 1;
+    const int halo = min(bound, a.halo);
+    const bool may_spill = bound > a.halo;
+    if (a.halo_next && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+        *a.halo_next = 0;   // for the next pushes; nothing reads it here
     const int ext = kTile + 2 * halo + 2;
     const int ci_lo = ci0 - 1 - halo, cj_lo = cj0 - 1 - halo;
     for (int k = threadIdx.x; k < 4 * kTile * kTile; k += blockDim.x)
@@ -381,8 +389,14 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
                               cj >= cj0 && cj < cj0 + kTile;
             tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
             if (abs(t.i - ci) > halo || abs(t.j - cj) > halo) {
-                if (home)
-                    a.spill[atomicAdd(a.spill_count, 1)] = (fi_a + li) * a.nf + fj_a + lj;
+                // (beyond a bound the pushes reported: cannot happen; counted
+                // as a fault rather than listed for a pass that does not run)
+                if (home) {
+                    if (may_spill)
+                        a.spill[atomicAdd(a.spill_count, 1)] = (fi_a + li) * a.nf + fj_a + lj;
+                    else
+                        atomicAdd(a.fault, 1);
+                }
                 return false;
             }
             return true;
@@ -448,22 +462,36 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
 #endif
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < kTile * kTile; k += blockDim.x) {
+    // store the tile, adding the ion background to ro last (lcode.py:687);
+    // all background loads are issued before the first store
+    constexpr int kOut = (kTile * kTile + kDepBlock - 1) / kDepBlock;
+    double base[kOut];
+#pragma unroll
+    for (int u = 0; u < kOut; ++u) {
+        const int k = threadIdx.x + u * kDepBlock;
         const int r = k / kTile, c = k % kTile;
         const int i = ci0 + r, j = cj0 + c;
-        if (i >= N || j >= N) continue;
+        base[u] = 0.;
+        if (k < kTile * kTile && i < N && j < N && a.ro_initial)
+            base[u] = a.ro_initial[(size_t)i * N + j];
+    }
+#pragma unroll
+    for (int u = 0; u < kOut; ++u) {
+        const int k = threadIdx.x + u * kDepBlock;
+        const int r = k / kTile, c = k % kTile;
+        const int i = ci0 + r, j = cj0 + c;
+        if (k >= kTile * kTile || i >= N || j >= N) continue;
         const size_t idx = (size_t)i * N + j;
         if (overfull) {
             a.ro[idx] = a.jx[idx] = a.jy[idx] = a.jz[idx] = __builtin_nan("");
             continue;
         }
-        double ro = acc[0][r][c];
-        if (a.ro_initial) ro += a.ro_initial[idx];
-        a.ro[idx] = ro;
+        a.ro[idx] = a.ro_initial ? acc[0][r][c] + base[u] : acc[0][r][c];
         a.jx[idx] = acc[1][r][c];
         a.jy[idx] = acc[2][r][c];
         a.jz[idx] = acc[3][r][c];
     }
+    if (!may_spill) return;
     // the last block to finish deposits the spilled particles: the barrier
     // orders this block's stores before thread 0's device-scope release
     // fence, which makes them visible before the block counts itself done
@@ -484,7 +512,6 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     if (threadIdx.x == 0) {
         atomicExch(a.spill_count, 0);
         atomicExch(a.spill_done, 0);
-        if (a.halo_cells) atomicExch(a.halo_cells, 0);   // for the next push
     }
 }
 
