@@ -259,52 +259,72 @@ struct RhsInputs {
 
 // Right-hand sides (lcode.py:166-168, :234-268, :339-343), FP64.
 
+// Without averaging a source is averaged with itself, (r + r) / 2 == r
+// exactly, and without a beam array the beam term is x * 0 + beam_scalar:
+// the same values as the conditional forms, with no branch between loads.
+__device__ __forceinline__ double beam_at(const RhsInputs& in, size_t idx) {
+    const double* b = in.beam ? in.beam : in.ro;
+    return b[idx] * (in.beam ? 1. : 0.) + (in.beam ? 0. : in.beam_scalar);
+}
+
 __device__ __forceinline__ double rho_src(const RhsInputs& in, size_t idx) {
-    double r = in.ro[idx];
-    if (in.ro_prev) r = (r + in.ro_prev[idx]) / 2;     // lcode.py:862
-    return r + (in.beam ? in.beam[idx] : in.beam_scalar);
+    const double* prev = in.ro_prev ? in.ro_prev : in.ro;
+    const double r = (in.ro[idx] + prev[idx]) / 2;        // lcode.py:862
+    return r + beam_at(in, idx);
 }
 
 __device__ __forceinline__ double jz_src(const RhsInputs& in, size_t idx) {
-    double r = in.jz[idx];
-    if (in.jz_prev) r = (r + in.jz_prev[idx]) / 2;     // lcode.py:863
-    return r + (in.beam ? in.beam[idx] : in.beam_scalar);
+    const double* prev = in.jz_prev ? in.jz_prev : in.jz;
+    const double r = (in.jz[idx] + prev[idx]) / 2;        // lcode.py:863
+    return r + beam_at(in, idx);
+}
+
+// The right-hand side of field `kind` at (i, j).  Outside the interior the
+// derivative terms are zero; their neighbour offsets then collapse to the
+// point itself, so every load stays in bounds for any (i, j) of the grid.
+template <int KIND>
+__device__ __forceinline__ double rhs_value_k(int i, int j, int N,
+                                              const RhsInputs& in) {
+    const size_t idx = (size_t)i * N + j;
+    const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
+    const size_t oi = inner ? (size_t)N : 0, oj = inner ? 1 : 0;
+    const double h2 = in.h * 2;
+    if constexpr (KIND == F_EX) {
+        const double d = (rho_src(in, idx + oi) - rho_src(in, idx - oi)) / h2;
+        const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
+        return -(((inner ? d : 0.) - dj) - in.Ftrick[0][idx] * in.trick);
+    } else if constexpr (KIND == F_EY) {
+        const double d = (rho_src(in, idx + oj) - rho_src(in, idx - oj)) / h2;
+        const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
+        return -(((inner ? d : 0.) - dj) - in.Ftrick[1][idx] * in.trick);
+    } else if constexpr (KIND == F_BX) {
+        const double d = (jz_src(in, idx + oj) - jz_src(in, idx - oj)) / h2;
+        const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
+        return +(((inner ? d : 0.) - dj) + in.Ftrick[2][idx] * in.trick);
+    } else if constexpr (KIND == F_BY) {
+        const double d = (jz_src(in, idx + oi) - jz_src(in, idx - oi)) / h2;
+        const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
+        return -(((inner ? d : 0.) - dj) - in.Ftrick[3][idx] * in.trick);
+    } else if constexpr (KIND == F_EZ) {
+        // the interior only carries this right-hand side (zero elsewhere)
+        return -((in.jx[idx + oi] - in.jx[idx - oi]) +
+                 (in.jy[idx + oj] - in.jy[idx - oj])) / h2;
+    } else {  // F_BZ
+        const double djx_dy = in.jx[idx + oj] - in.jx[idx - oj];
+        const double djy_dx = in.jy[idx + oi] - in.jy[idx - oi];
+        return -((inner ? djx_dy : 0.) - (inner ? djy_dx : 0.)) / h2;
+    }
 }
 
 __device__ __forceinline__ double rhs_value_dev(int kind, int i, int j, int N,
-                                            const RhsInputs& in) {
-    const size_t idx = (size_t)i * N + j;
-    const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
-    const double h2 = in.h * 2;
+                                                const RhsInputs& in) {
     switch (kind) {
-        case F_EX: {
-            const double d = inner ? (rho_src(in, idx + N) - rho_src(in, idx - N)) / h2 : 0.;
-            const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
-            return -((d - dj) - in.Ftrick[0][idx] * in.trick);
-        }
-        case F_EY: {
-            const double d = inner ? (rho_src(in, idx + 1) - rho_src(in, idx - 1)) / h2 : 0.;
-            const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
-            return -((d - dj) - in.Ftrick[1][idx] * in.trick);
-        }
-        case F_BX: {
-            const double d = inner ? (jz_src(in, idx + 1) - jz_src(in, idx - 1)) / h2 : 0.;
-            const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
-            return +((d - dj) + in.Ftrick[2][idx] * in.trick);
-        }
-        case F_BY: {
-            const double d = inner ? (jz_src(in, idx + N) - jz_src(in, idx - N)) / h2 : 0.;
-            const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
-            return -((d - dj) - in.Ftrick[3][idx] * in.trick);
-        }
-        case F_EZ:  // only interior (i, j) reach here
-            return -((in.jx[idx + N] - in.jx[idx - N]) +
-                     (in.jy[idx + 1] - in.jy[idx - 1])) / h2;
-        default: {  // F_BZ
-            const double djx_dy = inner ? in.jx[idx + 1] - in.jx[idx - 1] : 0.;
-            const double djy_dx = inner ? in.jy[idx + N] - in.jy[idx - N] : 0.;
-            return -(djx_dy - djy_dx) / h2;
-        }
+        case F_EX: return rhs_value_k<F_EX>(i, j, N, in);
+        case F_EY: return rhs_value_k<F_EY>(i, j, N, in);
+        case F_BX: return rhs_value_k<F_BX>(i, j, N, in);
+        case F_BY: return rhs_value_k<F_BY>(i, j, N, in);
+        case F_EZ: return rhs_value_k<F_EZ>(i, j, N, in);
+        default: return rhs_value_k<F_BZ>(i, j, N, in);
     }
 }
 

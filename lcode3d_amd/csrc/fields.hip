@@ -209,6 +209,12 @@ __host__ __device__ constexpr int threads_per_pair(int L) {
     return ((L + 3) / 4 + 63) / 64 * 64;
 }
 
+// Input points per thread of a pair: a transform input has at most
+// L / 2 + 1 points.
+__host__ __device__ constexpr int points_per_thread(int L) {
+    return (L / 2 + 1 + threads_per_pair(L) - 1) / threads_per_pair(L);
+}
+
 template <int L, int SPAN, typename V>
 __device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
                                            int t) {
@@ -279,9 +285,19 @@ __device__ __forceinline__ Tw<S> load_twiddles(typename Vec<S>::T* lds,
         for (int k = threadIdx.x; k <= (g.fft.L >> 2); k += blockDim.x) ct[k] = src[k];
         r.ct = ct;
     } else if constexpr (TW_LDS) {
+        // L - 1 entries over P * Tp >= L / 4 threads: at most four per
+        // thread, all loaded (clamped in bounds) before the LDS stores
         V* d = lds + (size_t)g.P * padded_len(L);
         const V* src = TwSrc<S>::tw(g);
-        for (int k = threadIdx.x; k < L - 1; k += blockDim.x) d[k] = src[k];
+        const int nt = blockDim.x;
+        V v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = src[min((int)threadIdx.x + u * nt, L - 2)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = threadIdx.x + u * nt;
+            if (k < L - 1) d[k] = v[u];
+        }
         r.tw = d;
     } else {
         r.tw = TwSrc<S>::tw(g);
@@ -346,6 +362,37 @@ __device__ __forceinline__ double rhs_value(int kind, int i, int j, int N,
     return rhs_value_dev(kind, i, j, N, in);
 }
 
+// Right-hand sides of rows ia and ib at columns j0 + t + u Tp, u < K, as
+// straight-line code: columns clamped to [j0, j0 + len), so all loads of
+// the thread are issued before their values are used.
+template <int KIND, int K>
+__device__ __forceinline__ void rhs_rows_k(int ia, int ib, int j0, int len,
+                                           int t, int Tp, int N,
+                                           const RhsInputs& in, double* a,
+                                           double* c) {
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+        const int j = j0 + min(t + u * Tp, len - 1);
+        a[u] = rhs_value_k<KIND>(ia, j, N, in);
+        c[u] = rhs_value_k<KIND>(ib, j, N, in);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void rhs_rows(int kind, int ia, int ib, int j0,
+                                         int len, int t, int Tp, int N,
+                                         const RhsInputs& in, double* a,
+                                         double* c) {
+    switch (kind) {
+        case F_EX: rhs_rows_k<F_EX, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_EY: rhs_rows_k<F_EY, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_BX: rhs_rows_k<F_BX, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_BY: rhs_rows_k<F_BY, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_EZ: rhs_rows_k<F_EZ, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        default: rhs_rows_k<F_BZ, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -387,11 +434,24 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     const Tw<S> tw = load_twiddles<L_, TW_LDS, S>(lds, g);
     const int ra = (bp.block * g.P + pair) * 2, rb = ra + 1;
     const bool va = ra < lines, vb = rb < lines;
-    for (int n = t; n < fp.lenA; n += Tp) {
-        const int j = fp.colA0 + n;
-        const double a = va ? rhs_value(fp.kind, fp.rowsA0 + ra, j, N, b.in) : 0.;
-        const double c = vb ? rhs_value(fp.kind, fp.rowsA0 + rb, j, N, b.in) : 0.;
-        put_ext(z, fp.xfA, n, fp.lenA, L, a, c);
+    if constexpr (L_ != 0) {
+        // rows without data read row rowsA0 (in bounds) and are dropped
+        constexpr int kPer = points_per_thread(L_);
+        double a[kPer], c[kPer];
+        rhs_rows<kPer>(fp.kind, fp.rowsA0 + (va ? ra : 0), fp.rowsA0 + (vb ? rb : 0),
+                       fp.colA0, fp.lenA, t, Tp, N, b.in, a, c);
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int n = t + u * Tp;
+            if (n < fp.lenA) put_ext(z, fp.xfA, n, fp.lenA, L, va ? a[u] : 0., vb ? c[u] : 0.);
+        }
+    } else {
+        for (int n = t; n < fp.lenA; n += Tp) {
+            const int j = fp.colA0 + n;
+            const double a = va ? rhs_value(fp.kind, fp.rowsA0 + ra, j, N, b.in) : 0.;
+            const double c = vb ? rhs_value(fp.kind, fp.rowsA0 + rb, j, N, b.in) : 0.;
+            put_ext(z, fp.xfA, n, fp.lenA, L, a, c);
+        }
     }
     put_ext_zeros(z, fp.xfA, fp.lenA, t);
     __syncthreads();
@@ -415,6 +475,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     V* lds = reinterpret_cast<V*>(lds_raw);
     __shared__ double red[2][16];
     if (!gate_open(b.gate)) return;
+    constexpr int kPts = L_ != 0 ? points_per_thread(L_) : kMaxPerThread;
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L;
@@ -430,27 +491,60 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     const S* taba = static_cast<const S*>(fp.table) + (size_t)la * len;
     const S* tabb = static_cast<const S*>(fp.table) + (size_t)lb * len;
 
-    for (int n = t; n < len; n += Tp)
-        put_ext(z, xf, n, len, L, va ? (double)rowa[n] : 0., vb ? (double)rowb[n] : 0.);
+    // the spectral multipliers of this thread's points, loaded up front
+    // (they do not depend on the transform); rows and points clamped in
+    // bounds, unused values dropped
+    S ta[kPts], tb[kPts];
+    {
+        const size_t oa = (size_t)(va ? la : 0) * len, ob = (size_t)(vb ? lb : 0) * len;
+        const S* tab = static_cast<const S*>(fp.table);
+#pragma unroll
+        for (int u = 0; u < kPts; ++u) {
+            const int kc = min(t + u * Tp, len - 1);
+            ta[u] = tab[oa + kc];
+            tb[u] = tab[ob + kc];
+        }
+    }
+    if constexpr (L_ != 0) {
+        constexpr int kPer = points_per_thread(L_);
+        const S* T = static_cast<const S*>(fp.T);
+        const size_t oa = (size_t)(va ? la : 0) * len, ob = (size_t)(vb ? lb : 0) * len;
+        S xa[kPer], xb[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int nc = min(t + u * Tp, len - 1);
+            xa[u] = T[oa + nc];
+            xb[u] = T[ob + nc];
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int n = t + u * Tp;
+            if (n < len)
+                put_ext(z, xf, n, len, L, va ? (double)xa[u] : 0., vb ? (double)xb[u] : 0.);
+        }
+    } else {
+        for (int n = t; n < len; n += Tp)
+            put_ext(z, xf, n, len, L, va ? (double)rowa[n] : 0., vb ? (double)rowb[n] : 0.);
+    }
     put_ext_zeros(z, xf, len, t);
     __syncthreads();
     fft_any<L_, S>(z, g, tw, t);
 
-    S ha[kMaxPerThread], hb[kMaxPerThread];
+    S ha[kPts], hb[kPts];
 #pragma unroll
-    for (int u = 0; u < kMaxPerThread; ++u) {
+    for (int u = 0; u < kPts; ++u) {
         const int k = t + u * Tp;
         ha[u] = hb[u] = 0;
         if (k < len) {
             double a, c;
             get_ext(z, xf, k, a, c);
-            if (va) ha[u] = (S)a * taba[k];
-            if (vb) hb[u] = (S)c * tabb[k];
+            if (va) ha[u] = (S)a * ta[u];
+            if (vb) hb[u] = (S)c * tb[u];
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int u = 0; u < kMaxPerThread; ++u) {
+    for (int u = 0; u < kPts; ++u) {
         const int k = t + u * Tp;
         if (k < len) put_ext(z, xf, k, len, L, (double)ha[u], (double)hb[u]);
     }
@@ -517,10 +611,31 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     const bool hb = rb < N && cb >= 0 && cb < fp.rowsA;
     const int stride = fp.rowsA;
     const S* T = static_cast<const S*>(fp.T);
-    for (int n = t; n < fp.lenA; n += Tp) {
-        const S* col = T + (size_t)n * stride;
-        put_ext(z, fp.xfA, n, fp.lenA, L, ha ? (double)col[ca] : 0.,
-                hb ? (double)col[cb] : 0.);
+    if constexpr (L_ != 0) {
+        // every column load of the thread first (addresses clamped in
+        // bounds, unused values dropped), then the LDS stores
+        constexpr int kPer = points_per_thread(L_);
+        const int cac = ha ? ca : 0, cbc = hb ? cb : 0;
+        S xa[kPer], xb[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const S* col = T + (size_t)min(t + u * Tp, fp.lenA - 1) * stride;
+            xa[u] = col[cac];
+            xb[u] = col[cbc];
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int n = t + u * Tp;
+            if (n < fp.lenA)
+                put_ext(z, fp.xfA, n, fp.lenA, L, ha ? (double)xa[u] : 0.,
+                        hb ? (double)xb[u] : 0.);
+        }
+    } else {
+        for (int n = t; n < fp.lenA; n += Tp) {
+            const S* col = T + (size_t)n * stride;
+            put_ext(z, fp.xfA, n, fp.lenA, L, ha ? (double)col[ca] : 0.,
+                    hb ? (double)col[cb] : 0.);
+        }
     }
     put_ext_zeros(z, fp.xfA, fp.lenA, t);
     __syncthreads();
@@ -544,29 +659,72 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     const bool varA = b.variantA && transverse;
     const double* guess = b.resid ? b.guess[fp.kind] : nullptr;
     double r_num = 0., r_den = 0.;
-    for (int j = t; j < N; j += Tp) {
-        double va = 0., vb = 0.;
+    // one output point (i, j) of this pair: F from the spectrum value v,
+    // prev and guess values p, q of that point
+    auto emit = [&](int i, int j, double v, double p, double q) {
+        const size_t idx = (size_t)i * N + j;
+        double F = v;
+        if (fp.partial) F -= mean;                         // lcode.py:362
+        if (varA) F = 2 * F - p;                           // lcode.py:869-871
+        if (fp.outF) fp.outF[idx] = F;
+        if (fp.outAvg) fp.outAvg[idx] = (F + p) / 2;       // lcode.py:876-881
+        if (guess) {
+            const double d = F - q;
+            r_num += d * d;
+            r_den += F * F;
+        }
+    };
+    // the field value at column j: a row without data shares its FFT with
+    // a partner row; keep it an exact zero instead of the partner's noise
+    auto spec = [&](int j, double& va, double& vb) {
+        va = vb = 0.;
         const int k = j - fp.colA0;
         if (k >= 0 && k < fp.lenA) get_ext(z, fp.xfA, k, va, vb);
-        // a row without data shares its FFT with a partner row; keep it an
-        // exact zero instead of the partner's rounding noise
         if (!ha) va = 0.;
         if (!hb) vb = 0.;
+    };
+    if constexpr (L_ != 0) {
+        // the prev and guess loads of every point of the thread first
+        // (rows and columns clamped in bounds), then the stores
+        constexpr int kPer = points_per_thread(L_);
+        const size_t oa = (size_t)min(ra, N - 1) * N, ob = (size_t)min(rb, N - 1) * N;
+        double pa[kPer], pb[kPer], qa[kPer], qb[kPer];
 #pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const int i = side ? rb : ra;
-            if (i >= N) continue;
-            const size_t idx = (size_t)i * N + j;
-            double F = side ? vb : va;
-            if (fp.partial) F -= mean;                         // lcode.py:362
-            if (varA) F = 2 * F - fp.prevF[idx];               // lcode.py:869-871
-            if (fp.outF) fp.outF[idx] = F;
-            if (fp.outAvg) fp.outAvg[idx] = (F + fp.prevF[idx]) / 2;  // :876-881
-            if (guess) {
-                const double d = F - guess[idx];
-                r_num += d * d;
-                r_den += F * F;
+        for (int u = 0; u < kPer; ++u) pa[u] = pb[u] = qa[u] = qb[u] = 0.;
+        if (fp.prevF) {
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int jc = min(t + u * Tp, N - 1);
+                pa[u] = fp.prevF[oa + jc];
+                pb[u] = fp.prevF[ob + jc];
             }
+        }
+        if (guess) {
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int jc = min(t + u * Tp, N - 1);
+                qa[u] = guess[oa + jc];
+                qb[u] = guess[ob + jc];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int j = t + u * Tp;
+            if (j >= N) continue;
+            double va, vb;
+            spec(j, va, vb);
+            if (ra < N) emit(ra, j, va, pa[u], qa[u]);
+            if (rb < N) emit(rb, j, vb, pb[u], qb[u]);
+        }
+    } else {
+        for (int j = t; j < N; j += Tp) {
+            double va, vb;
+            spec(j, va, vb);
+            const size_t ia = (size_t)ra * N + j, ib = (size_t)rb * N + j;
+            if (ra < N)
+                emit(ra, j, va, fp.prevF ? fp.prevF[ia] : 0., guess ? guess[ia] : 0.);
+            if (rb < N)
+                emit(rb, j, vb, fp.prevF ? fp.prevF[ib] : 0., guess ? guess[ib] : 0.);
         }
     }
     if (b.resid) {
