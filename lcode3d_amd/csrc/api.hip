@@ -277,7 +277,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     int P;
     if (g.static_L) {
         // stage twiddles in LDS when that still leaves two line pairs
-        P = pairs_for((L - 1) * cplx);
+        P = pairs_for(solver_tw_count(L) * cplx);
         g.tw_lds = LCODE_TW_LDS_OK && P >= 2;
         if (!g.tw_lds) P = pairs_for(0);
     } else {
@@ -294,19 +294,19 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         return fail(LCODE_E_UNSUPPORTED);
     }
     if (g.static_L) {
-        // stage s with span sp and radix R: exp(-2 pi i q k / (sp R)) at
-        // sp - 1 + k (R - 1) + q - 1
-        std::vector<double2> tw(L - 1);
-        int sp = 1;
+        // stage with span sp and radix R: exp(-2 pi i k / (sp R)), k < sp,
+        // after the entries of the earlier stages (the kernels form the
+        // powers q = 2 .. R - 1)
+        std::vector<double2> tw(solver_tw_count(L));
+        int sp = 1, off = 0;
         for (int st = 0; st < g.fft.nstages; ++st) {
             const int R = g.fft.radix[st];
-            for (int k = 0; k < sp; ++k)
-                for (int q = 1; q < R; ++q) {
-                    const long double a = 2.0L * 3.14159265358979323846264338327950288L *
-                                          q * k / ((long double)sp * R);
-                    tw[sp - 1 + k * (R - 1) + q - 1] =
-                        make_double2((double)std::cos(a), (double)-std::sin(a));
-                }
+            for (int k = 0; k < sp; ++k) {
+                const long double a = 2.0L * 3.14159265358979323846264338327950288L *
+                                      k / ((long double)sp * R);
+                tw[off + k] = make_double2((double)std::cos(a), (double)-std::sin(a));
+            }
+            off += sp;
             sp *= R;
         }
         if ((rc = upload(ws, &ws->stage_tw, tw))) return fail(rc);

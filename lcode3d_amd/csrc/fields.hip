@@ -205,15 +205,25 @@ __device__ void fft_lds(V* z, const FftPlan& plan, const decltype(V::x)* ct,
 
 // ---------------------------------------------------------------------------
 // Compile-time plans for the common transform lengths: the radix of each
-// stage and its span are constants, so the index arithmetic folds away,
-// and the twiddles of a stage with span s and radix R,
-// exp(-2 pi i q k / (s R)) for k < s, 1 <= q < R, sit contiguously at
-// tw[s - 1 + k (R - 1) + q - 1] (L - 1 entries for the whole plan).
+// stage and its span are constants, so the index arithmetic folds away.
+// A stage with span s and radix R needs the twiddles w^q, 1 <= q < R, of
+// w = exp(-2 pi i k / (s R)) for k < s: the table holds w alone, at
+// tw[tw_offset(L, s) + k] (the spans of the earlier stages summed; 341
+// entries for L = 1280 instead of L - 1 for every power), and the stage
+// forms w^2 = w w, w^3 = w^2 w, w^4 = w^2 w^2 in registers.
 // ---------------------------------------------------------------------------
 
 __host__ __device__ constexpr int next_radix(int rem) {
     return rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
 }
+
+__host__ __device__ constexpr int tw_offset(int L, int span) {
+    int off = 0;
+    for (int sp = 1; sp < span && sp < L; sp *= next_radix(L / sp)) off += sp;
+    return off;
+}
+
+__host__ __device__ constexpr int tw_count(int L) { return tw_offset(L, L); }
 
 __host__ __device__ constexpr int threads_per_pair(int L) {
     return ((L + 3) / 4 + 63) / 64 * 64;
@@ -233,17 +243,24 @@ __device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
         constexpr int T = L / R;
         constexpr int Tp = threads_per_pair(L);
         constexpr int ITER = (T + Tp - 1) / Tp;
-        const V* tws = tw + (SPAN - 1);
+        constexpr int OFF = tw_offset(L, SPAN);
         V v[ITER][R];
 #pragma unroll
         for (int it = 0; it < ITER; ++it) {
             const int b = t + it * Tp;
             if (b < T) {
                 const int k = b % SPAN;
+                V w[R];
+                if constexpr (SPAN > 1) {
+                    w[1] = tw[OFF + k];
+                    if constexpr (R > 2) w[2] = cmul(w[1], w[1]);
+                    if constexpr (R > 3) w[3] = cmul(w[2], w[1]);
+                    if constexpr (R > 4) w[4] = cmul(w[2], w[2]);
+                }
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     V x = z[zi(b + q * T)];
-                    if (SPAN > 1 && q) x = cmul(x, tws[k * (R - 1) + q - 1]);
+                    if (SPAN > 1 && q) x = cmul(x, w[q]);
                     v[it][q] = x;
                 }
             }
@@ -295,18 +312,20 @@ __device__ __forceinline__ Tw<S> load_twiddles(typename Vec<S>::T* lds,
         for (int k = threadIdx.x; k <= (g.fft.L >> 2); k += blockDim.x) ct[k] = src[k];
         r.ct = ct;
     } else if constexpr (TW_LDS) {
-        // L - 1 entries over P * Tp >= L / 4 threads: at most four per
-        // thread, all loaded (clamped in bounds) before the LDS stores
+        // tw_count(L) <= 2 L / 3 entries over P * Tp >= L / 4 threads: at
+        // most three per thread, all loaded before the LDS stores
+        constexpr int n = tw_count(L);
+        static_assert(n <= 3 * (L / 4), "twiddle table larger than three per thread");
         V* d = lds + (size_t)g.P * padded_len(L);
         const V* src = TwSrc<S>::tw(g);
         const int nt = blockDim.x;
-        V v[4];
+        V v[3];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = src[min((int)threadIdx.x + u * nt, L - 2)];
+        for (int u = 0; u < 3; ++u) v[u] = src[min((int)threadIdx.x + u * nt, n - 1)];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 3; ++u) {
             const int k = threadIdx.x + u * nt;
-            if (k < L - 1) d[k] = v[u];
+            if (k < n) d[k] = v[u];
         }
         r.tw = d;
     } else {
@@ -839,13 +858,20 @@ int solver_lds_bytes(const SolverGeom& g) {
     const size_t cplx = g.fp32 ? sizeof(float2) : sizeof(double2);
     const size_t data = (size_t)g.P * padded_len(g.fft.L) * cplx;
     if (g.static_L == 0) return (int)(data + ((size_t)g.fft.L / 4 + 1) * cplx / 2);
-    if (g.tw_lds) return (int)(data + (size_t)(g.fft.L - 1) * cplx);
+    if (g.tw_lds) return (int)(data + (size_t)tw_count(g.fft.L) * cplx);
     return (int)data;
 }
 
 int solver_padded_len(int L) { return padded_len(L); }
 
+int solver_tw_count(int L) { return tw_count(L); }
+
+#ifndef LCODE_FFT_STATIC_PLANS
+#define LCODE_FFT_STATIC_PLANS 1
+#endif
+
 bool solver_static_length(int L) {
+    if (!LCODE_FFT_STATIC_PLANS) return false;
     switch (L) {
         case 32: case 64: case 128: case 256: case 512: case 640:
         case 1024: case 1280: case 2048: case 2560:
