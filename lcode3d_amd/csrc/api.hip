@@ -298,16 +298,15 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         // after the entries of the earlier stages (the kernels form the
         // powers q = 2 .. R - 1)
         std::vector<double2> tw(solver_tw_count(L));
-        int sp = 1, off = 0;
-        for (int st = 0; st < g.fft.nstages; ++st) {
-            const int R = g.fft.radix[st];
+        int off = 0;
+        for (int sp = 1, R = 1; sp < L; sp *= R) {
+            R = solver_static_radix(L / sp);
             for (int k = 0; k < sp; ++k) {
                 const long double a = 2.0L * 3.14159265358979323846264338327950288L *
                                       k / ((long double)sp * R);
                 tw[off + k] = make_double2((double)std::cos(a), (double)-std::sin(a));
             }
             off += sp;
-            sp *= R;
         }
         if ((rc = upload(ws, &ws->stage_tw, tw))) return fail(rc);
         g.stage_tw = ws->stage_tw;

@@ -92,6 +92,30 @@ template <typename V> struct Dft<4, V> {
     }
 };
 
+// Radix 8 as two radix-4 DFTs of the even and odd points combined with
+// the eighth roots of unity: y_p = e_p + w8^p o_p, y_(p+4) = e_p - w8^p o_p.
+template <typename V> struct Dft<8, V> {
+    static __device__ __forceinline__ void run(V* v) {
+        using S = decltype(V::x);
+        constexpr S r = (S)0.70710678118654752440;   // 1 / sqrt(2)
+        V e[4] = {v[0], v[2], v[4], v[6]};
+        V o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, V>::run(e);
+        Dft<4, V>::run(o);
+        const V o1 = mk<V>(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));    // (1 - i) / sqrt 2
+        const V o2 = mul_mi(o[2]);                                           // -i
+        const V o3 = mk<V>(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));   // -(1 + i) / sqrt 2
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
+    }
+};
+
 template <typename V> struct Dft<3, V> {
     static __device__ __forceinline__ void run(V* v) {
         using S = decltype(V::x);
@@ -213,8 +237,15 @@ __device__ void fft_lds(V* z, const FftPlan& plan, const decltype(V::x)* ct,
 // forms w^2 = w w, w^3 = w^2 w, w^4 = w^2 w^2 in registers.
 // ---------------------------------------------------------------------------
 
+// Radix of the next static stage (build option LCODE_FFT_RADIX8: radix 8
+// first where it divides; measured at 641^2: pass B 44.6 us with
+// 8-8-4-5, 41.8 us with 4-4-4-4-5, so off by default).
+#ifndef LCODE_FFT_RADIX8
+#define LCODE_FFT_RADIX8 0
+#endif
 __host__ __device__ constexpr int next_radix(int rem) {
-    return rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
+    return (LCODE_FFT_RADIX8 && rem % 8 == 0) ? 8
+           : rem % 4 == 0 ? 4 : rem % 2 == 0 ? 2 : rem % 3 == 0 ? 3 : 5;
 }
 
 __host__ __device__ constexpr int tw_offset(int L, int span) {
@@ -256,6 +287,9 @@ __device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
                     if constexpr (R > 2) w[2] = cmul(w[1], w[1]);
                     if constexpr (R > 3) w[3] = cmul(w[2], w[1]);
                     if constexpr (R > 4) w[4] = cmul(w[2], w[2]);
+                    if constexpr (R > 5) w[5] = cmul(w[4], w[1]);
+                    if constexpr (R > 6) w[6] = cmul(w[4], w[2]);
+                    if constexpr (R > 7) w[7] = cmul(w[4], w[3]);
                 }
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
@@ -865,6 +899,8 @@ int solver_lds_bytes(const SolverGeom& g) {
 int solver_padded_len(int L) { return padded_len(L); }
 
 int solver_tw_count(int L) { return tw_count(L); }
+
+int solver_static_radix(int rem) { return next_radix(rem); }
 
 #ifndef LCODE_FFT_STATIC_PLANS
 #define LCODE_FFT_STATIC_PLANS 1
