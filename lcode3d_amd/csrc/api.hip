@@ -40,6 +40,9 @@ struct lcode_workspace {
     void* tab_dir = nullptr;       // [M][M]
     void* tab_neu = nullptr;       // [N][N]
     double* colsum = nullptr;      // [N]
+    double* lam_i = nullptr;       // [M] eigenvalues k = 1 .. N - 2 (DST)
+    double* lam_j = nullptr;       // [N] eigenvalues k = 0 .. N - 1 (DCT)
+    double lam_add = 0, lam_norm = 1;
     void* T = nullptr;             // 6 * N * N transform scratch (S)
     double* partial = nullptr;     // [N]
     double* p1[5] = {};            // predictor particles (nc * nc)
@@ -185,26 +188,31 @@ FieldPlan plan_for(lcode_workspace* ws, int kind, int slot) {
             p.rowsA0 = 1; p.rowsA = M;
             p.xfA = XF_DCT; p.lenA = N; p.colA0 = 0;
             p.xfB = XF_DST; p.table = ws->tab_mixed_t;
+            p.lamL = ws->lam_j; p.lamK = ws->lam_i; p.lam_add = ws->lam_add;
             break;
         case F_EX:
         case F_BY:  // solved on the transposed RHS (lcode.py:287-288)
             p.rowsA0 = 0; p.rowsA = N;
             p.xfA = XF_DST; p.lenA = M; p.colA0 = 1;
             p.xfB = XF_DCT; p.table = ws->tab_mixed;
+            p.lamL = ws->lam_i; p.lamK = ws->lam_j; p.lam_add = ws->lam_add;
             break;
         case F_EZ:  // DST-I both ways on the interior (lcode.py:166-182)
             p.rowsA0 = 1; p.rowsA = M;
             p.xfA = XF_DST; p.lenA = M; p.colA0 = 1;
             p.xfB = XF_DST; p.table = ws->tab_dir;
+            p.lamL = ws->lam_i; p.lamK = ws->lam_i;
             break;
         default:    // F_BZ: DCT-I both ways, zero mode dropped (:339-362)
             p.rowsA0 = 0; p.rowsA = N;
             p.xfA = XF_DCT; p.lenA = N; p.colA0 = 0;
             p.xfB = XF_DCT; p.table = ws->tab_neu;
+            p.lamL = ws->lam_j; p.lamK = ws->lam_j; p.zero_mode = 1;
             p.colsum = ws->colsum;
             p.partial = ws->partial;
             break;
     }
+    p.lam_norm = ws->lam_norm;
     return p;
 }
 
@@ -368,6 +376,10 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     if ((rc = upload_table(ws, &ws->tab_dir, dir))) return fail(rc);
     if ((rc = upload_table(ws, &ws->tab_neu, neu))) return fail(rc);
     if ((rc = upload(ws, &ws->colsum, colsum))) return fail(rc);
+    if ((rc = upload(ws, &ws->lam_i, li))) return fail(rc);
+    if ((rc = upload(ws, &ws->lam_j, lj))) return fail(rc);
+    ws->lam_add = add;
+    ws->lam_norm = norm;
 
     {
         // zn diagnostic taps (scipy.ndimage.gaussian_filter1d: truncate 4)

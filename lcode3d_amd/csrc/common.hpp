@@ -334,6 +334,12 @@ struct FieldPlan {
     int xfA, lenA, colA0;     // transform along j: type, length, first j
     int xfB;                  // transform along i (length rowsA)
     const void* table;        // [lenA][rowsA] spectral multiplier (S)
+    // the same multiplier as 1 / (lamL[line] + lamK[k] + lam_add) / lam_norm
+    // (0 at line = k = 0 when zero_mode): the eigenvalues of the two
+    // transforms, the trick term and the normalisation
+    const double *lamL, *lamK;
+    double lam_add, lam_norm;
+    int zero_mode;
     void* T;                  // [lenA][rowsA] scratch spectra (S)
     const double* colsum;     // Bz only: column sums of the last DCT
     double* partial;          // Bz only: [lenA] mean partials
