@@ -37,41 +37,60 @@ struct ConvArgs {
                                          // backward-data: mask by in > 0
 };
 
-// Stage a (kT + 2)^2 tile of `c` planes with a one-pixel zero halo.
+// Stage a (kT + 2)^2 tile of `c` planes with a one-pixel zero halo: eight
+// loads per thread issued together (addresses clamped in bounds, values
+// outside the planes replaced by zero), then their LDS stores.
 __device__ __forceinline__ void stage_tile(float (*t)[kT + 2][kT + 2],
-                                           const float* src, int c, int H,
-                                           int W, int i0, int j0) {
-    for (int e = threadIdx.x; e < c * (kT + 2) * (kT + 2); e += 256) {
-        const int ch = e / ((kT + 2) * (kT + 2));
-        const int r = e - ch * (kT + 2) * (kT + 2);
-        const int ii = r / (kT + 2), jj = r - ii * (kT + 2);
-        const int i = i0 - 1 + ii, j = j0 - 1 + jj;
-        t[ch][ii][jj] = (i >= 0 && i < H && j >= 0 && j < W)
-                            ? src[((size_t)ch * H + i) * W + j] : 0.f;
+                                           const float* __restrict__ src, int c,
+                                           int H, int W, int i0, int j0) {
+    constexpr int kE = (kT + 2) * (kT + 2);
+    constexpr int kU = 8;
+    const int total = c * kE;
+    for (int base = 0; base < total; base += 256 * kU) {
+        float v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int e = base + (int)threadIdx.x + u * 256;
+            const int ch = e / kE, r = e - ch * kE;
+            const int i = i0 - 1 + r / (kT + 2), j = j0 - 1 + r % (kT + 2);
+            const bool ok = e < total && i >= 0 && i < H && j >= 0 && j < W;
+            const float x = src[ok ? ((size_t)ch * H + i) * W + j : 0];
+            v[u] = ok ? x : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int e = base + (int)threadIdx.x + u * 256;
+            if (e < total) {
+                const int ch = e / kE, r = e - ch * kE;
+                t[ch][r / (kT + 2)][r % (kT + 2)] = v[u];
+            }
+        }
     }
 }
 
+// CO = a.cout as a constant: the accumulators are unrolled registers.
+template <int CO>
 __global__ void __launch_bounds__(256)
 conv3x3_fwd_kernel(ConvArgs a) {
     __shared__ float t[kPredMaxC][kT + 2][kT + 2];
-    __shared__ float w[kPredMaxC * kPredMaxC * 9];
+    // the weights are the same for every thread: read through the scalar
+    // path from global memory rather than once per multiply from LDS
+    const float* __restrict__ w = a.w;
     const int tiles_w = (a.W + kT - 1) / kT;
     const int i0 = (blockIdx.x / tiles_w) * kT, j0 = (blockIdx.x % tiles_w) * kT;
     stage_tile(t, a.in, a.cin, a.H, a.W, i0, j0);
-    for (int e = threadIdx.x; e < a.cout * a.cin * 9; e += 256) w[e] = a.w[e];
     __syncthreads();
     const int ti = threadIdx.x / kT, tj = threadIdx.x % kT;
     const int i = i0 + ti, j = j0 + tj;
-    float acc[kPredMaxC];
+    float acc[CO];
 #pragma unroll
-    for (int co = 0; co < kPredMaxC; ++co) acc[co] = co < a.cout ? a.b[co] : 0.f;
+    for (int co = 0; co < CO; ++co) acc[co] = a.b[co];
     for (int ci = 0; ci < a.cin; ++ci) {
         float x[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) x[k] = t[ci][ti + k / 3][tj + k % 3];
 #pragma unroll
-        for (int co = 0; co < kPredMaxC; ++co) {
-            if (co >= a.cout) break;
+        for (int co = 0; co < CO; ++co) {
             const float* wk = w + (co * a.cin + ci) * 9;
 #pragma unroll
             for (int k = 0; k < 9; ++k) acc[co] = fmaf(wk[k], x[k], acc[co]);
@@ -79,8 +98,7 @@ conv3x3_fwd_kernel(ConvArgs a) {
     }
     if (i >= a.H || j >= a.W) return;
 #pragma unroll
-    for (int co = 0; co < kPredMaxC; ++co) {
-        if (co >= a.cout) break;
+    for (int co = 0; co < CO; ++co) {
         const float v = acc[co];
         a.out[((size_t)co * a.H + i) * a.W + j] = a.relu ? fmaxf(v, 0.f) : v;
     }
@@ -98,37 +116,36 @@ struct ConvBwdInArgs {
     int H, W;
 };
 
+// CI = a.cin as a constant, as the forward kernel's CO.
+template <int CI>
 __global__ void __launch_bounds__(256)
 conv3x3_bwd_in_kernel(ConvBwdInArgs a) {
     __shared__ float t[kPredMaxC][kT + 2][kT + 2];
-    __shared__ float w[kPredMaxC * kPredMaxC * 9];
+    const float* __restrict__ w = a.w;   // uniform: scalar path, as forward
     const int tiles_w = (a.W + kT - 1) / kT;
     const int i0 = (blockIdx.x / tiles_w) * kT, j0 = (blockIdx.x % tiles_w) * kT;
     stage_tile(t, a.gout, a.cout, a.H, a.W, i0, j0);
-    for (int e = threadIdx.x; e < a.cout * a.cin * 9; e += 256) w[e] = a.w[e];
     __syncthreads();
     const int ti = threadIdx.x / kT, tj = threadIdx.x % kT;
     const int i = i0 + ti, j = j0 + tj;
-    float acc[kPredMaxC];
+    float acc[CI];
 #pragma unroll
-    for (int ci = 0; ci < kPredMaxC; ++ci) acc[ci] = 0.f;
+    for (int ci = 0; ci < CI; ++ci) acc[ci] = 0.f;
     for (int co = 0; co < a.cout; ++co) {
         float g[9];
         // gout at p - d_k: tile offset (1 - (k/3 - 1), 1 - (k%3 - 1))
 #pragma unroll
         for (int k = 0; k < 9; ++k) g[k] = t[co][ti + 2 - k / 3][tj + 2 - k % 3];
 #pragma unroll
-        for (int ci = 0; ci < kPredMaxC; ++ci) {
-            if (ci >= a.cin) break;
-            const float* wk = w + (co * a.cin + ci) * 9;
+        for (int ci = 0; ci < CI; ++ci) {
+            const float* wk = w + (co * CI + ci) * 9;
 #pragma unroll
             for (int k = 0; k < 9; ++k) acc[ci] = fmaf(wk[k], g[k], acc[ci]);
         }
     }
     if (i >= a.H || j >= a.W) return;
 #pragma unroll
-    for (int ci = 0; ci < kPredMaxC; ++ci) {
-        if (ci >= a.cin) break;
+    for (int ci = 0; ci < CI; ++ci) {
         const size_t idx = ((size_t)ci * a.H + i) * a.W + j;
         float v = acc[ci];
         if (a.mask && !(a.mask[idx] > 0.f)) v = 0.f;
@@ -151,11 +168,23 @@ conv3x3_bwd_w_kernel(ConvBwdWArgs a) {
     const int tiles_w = (a.W + kT - 1) / kT;
     const int i0 = (blockIdx.x / tiles_w) * kT, j0 = (blockIdx.x % tiles_w) * kT;
     stage_tile(t, a.in, a.cin, a.H, a.W, i0, j0);
-    for (int e = threadIdx.x; e < a.cout * kT * kT; e += 256) {
+    // the output-gradient tile: all loads of a thread before its stores
+    constexpr int kG = kPredMaxC * kT * kT / 256;
+    float gv[kG];
+#pragma unroll
+    for (int u = 0; u < kG; ++u) {
+        const int e = threadIdx.x + u * 256;
         const int co = e / (kT * kT), r = e - co * kT * kT;
         const int i = i0 + r / kT, j = j0 + r % kT;
-        g[co][r / kT][r % kT] = (i < a.H && j < a.W)
-                                    ? a.gout[((size_t)co * a.H + i) * a.W + j] : 0.f;
+        const bool ok = co < a.cout && i < a.H && j < a.W;
+        const float x = a.gout[ok ? ((size_t)co * a.H + i) * a.W + j : 0];
+        gv[u] = ok ? x : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kG; ++u) {
+        const int e = threadIdx.x + u * 256;
+        const int co = e / (kT * kT), r = e - co * kT * kT;
+        if (co < a.cout) g[co][r / kT][r % kT] = gv[u];
     }
     __syncthreads();
     const int nw = a.cout * a.cin * 9, np = nw + a.cout;
@@ -312,7 +341,13 @@ int lcode_pred_conv3x3_forward(const float* in, int32_t cin, const float* w,
     LCODE_REQUIRE(channels_ok(cin) && channels_ok(cout), "channels must be 1..16");
     LCODE_REQUIRE(H > 0 && W > 0, "empty plane");
     ConvArgs a{in, cin, w, b, out, cout, H, W, relu};
-    conv3x3_fwd_kernel<<<tiles(H, W), 256, 0, stream>>>(a);
+    switch (cout) {
+#define LCODE_FWD(C) case C: conv3x3_fwd_kernel<C><<<tiles(H, W), 256, 0, stream>>>(a); break;
+        LCODE_FWD(1) LCODE_FWD(2) LCODE_FWD(3) LCODE_FWD(4) LCODE_FWD(5) LCODE_FWD(6)
+        LCODE_FWD(7) LCODE_FWD(8) LCODE_FWD(9) LCODE_FWD(10) LCODE_FWD(11) LCODE_FWD(12)
+        LCODE_FWD(13) LCODE_FWD(14) LCODE_FWD(15) LCODE_FWD(16)
+#undef LCODE_FWD
+    }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
@@ -325,7 +360,13 @@ int lcode_pred_conv3x3_backward_input(const float* gout, int32_t cout,
     LCODE_REQUIRE(channels_ok(cin) && channels_ok(cout), "channels must be 1..16");
     LCODE_REQUIRE(H > 0 && W > 0, "empty plane");
     ConvBwdInArgs a{gout, cout, w, gin, cin, relu_input, H, W};
-    conv3x3_bwd_in_kernel<<<tiles(H, W), 256, 0, stream>>>(a);
+    switch (cin) {
+#define LCODE_BWD(C) case C: conv3x3_bwd_in_kernel<C><<<tiles(H, W), 256, 0, stream>>>(a); break;
+        LCODE_BWD(1) LCODE_BWD(2) LCODE_BWD(3) LCODE_BWD(4) LCODE_BWD(5) LCODE_BWD(6)
+        LCODE_BWD(7) LCODE_BWD(8) LCODE_BWD(9) LCODE_BWD(10) LCODE_BWD(11) LCODE_BWD(12)
+        LCODE_BWD(13) LCODE_BWD(14) LCODE_BWD(15) LCODE_BWD(16)
+#undef LCODE_BWD
+    }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
