@@ -538,8 +538,14 @@ constexpr int kMaxPerThread = 4;
 #define LCODE_FFT_TABLE_INLINE 0
 #endif
 
+// LCODE_PASS_B_WAVES: waves per SIMD the register allocation must allow
+// (8: at most 64 VGPRs, three 640-thread blocks per CU).
+#ifndef LCODE_PASS_B_WAVES
+#define LCODE_PASS_B_WAVES 8
+#endif
 template <int L_, bool TW_LDS, typename S>
 __global__ void __launch_bounds__(1024)
+__attribute__((amdgpu_waves_per_eu(LCODE_PASS_B_WAVES)))
 pass_b_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
     extern __shared__ __align__(16) unsigned char lds_raw[];
