@@ -155,6 +155,8 @@ int check_cfg(const lcode_config* c) {
                   "plasma_coarseness and plasma_fineness must be positive");
     LCODE_REQUIRE(c->plasma_fineness <= 3,
                   "plasma_fineness > 3 is not supported (deposition tile tables)");
+    LCODE_REQUIRE(c->plasma_fineness * (kDepTile + 2 * kDepositHalo + 2) <= kDepFineMax,
+                  "plasma_fineness too large for this build's deposition tiles");
     LCODE_REQUIRE(c->solver == LCODE_SOLVER_FFT64 || c->solver == LCODE_SOLVER_FFT32 ||
                       c->solver == LCODE_SOLVER_DENSE64 ||
                       c->solver == LCODE_SOLVER_DENSE_BF16X3,
@@ -682,7 +684,7 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
         a.halo_next = ws->spill_ctr + 2 + (1 - ws->bound_slot);
     }
     // the tile's coarse rectangle has at most ext / coarseness + 3 rows
-    a.staged = (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= 18;
+    a.staged = (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= kDepCoarseMax;
     a.group = cfg->plasma_fineness;
     a.gate = gate;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");

@@ -93,6 +93,18 @@ __device__ __forceinline__ bool gate_open(const Gate& g) {
 #define LCODE_DEP_BLOCK 256
 #endif
 constexpr int kDepTile = LCODE_DEP_TILE;
+// LDS sizes of a deposition tile: fine indices per side (at least fineness
+// (kDepTile + 2 halo + 2) for fineness <= 3, halo <= 3) and coarse
+// particles per side of the staged coarse rectangle (a tile whose rectangle
+// is larger takes the unstaged kernel).
+#ifndef LCODE_DEP_FINE_MAX
+#define LCODE_DEP_FINE_MAX 128
+#endif
+#ifndef LCODE_DEP_COARSE_MAX
+#define LCODE_DEP_COARSE_MAX 18
+#endif
+constexpr int kDepFineMax = LCODE_DEP_FINE_MAX;
+constexpr int kDepCoarseMax = LCODE_DEP_COARSE_MAX;
 // 1: each deposition thread takes a group of fine particles (fineness x
 // fineness, one initial cell) and sums successive particles with a common
 // stencil centre in registers before the LDS atomics (43.8 us per launch at

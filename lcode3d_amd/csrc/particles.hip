@@ -210,8 +210,8 @@ __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
 // displaced by more than a.halo cells from its initial cell is appended to
 // the spill list by its home tile (the tile owning its initial cell); the
 // last block to finish deposits the list with global atomics and resets it.
-constexpr int kFineMax = 128;    // fine indices per side: fineness (kTile + 2 halo + 2)
-constexpr int kCoarseMax = 18;   // coarse particles per side staged in LDS
+constexpr int kFineMax = kDepFineMax;       // fine indices per side
+constexpr int kCoarseMax = kDepCoarseMax;   // coarse particles per side staged in LDS
 constexpr int kExtMax = 48;      // initial cells per side: kTile + 2 halo + 2
 
 struct FineTab {                 // per fine index along one axis
