@@ -348,6 +348,22 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
 #pragma unroll
                 for (int c = 0; c < 7; ++c) cs[c][k] = src[c][g];
             }
+            // every thread has read the rectangle's corners: turn the
+            // neighbour indices into offsets in it, rows scaled by its
+            // width, each clamped into it, so that a row part plus a column
+            // part is an in-bounds index of cs
+            __syncthreads();
+            const int hmax = max(cH - 1, 0), wmax = max(cW - 1, 0);
+            for (int k = threadIdx.x; k < ni + nj; k += blockDim.x) {
+                if (k < ni) {
+                    ti.P[k] = min(max(ti.P[k] - cI0, 0), hmax) * cW;
+                    ti.Nn[k] = min(max(ti.Nn[k] - cI0, 0), hmax) * cW;
+                } else {
+                    const int s = k - ni;
+                    tj.P[s] = min(max(tj.P[s] - cJ0, 0), wmax);
+                    tj.Nn[s] = min(max(tj.Nn[s] - cJ0, 0), wmax);
+                }
+            }
         }
         __syncthreads();
         const double sm = a.smallness;
@@ -357,11 +373,10 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             const double A = ti.ip[li] * tj.ip[lj], B = ti.ip[li] * tj.in[lj];
             const double C = ti.in[li] * tj.ip[lj], D = ti.in[li] * tj.in[lj];
             if constexpr (STAGED) {
-                const int last = kCoarseMax * kCoarseMax - 1;
-                const int pp = min(max((ti.P[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
-                const int pn = min(max((ti.P[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
-                const int np = min(max((ti.Nn[li] - cI0) * cW + tj.P[lj] - cJ0, 0), last);
-                const int nn = min(max((ti.Nn[li] - cI0) * cW + tj.Nn[lj] - cJ0, 0), last);
+                // row offset + column offset (both clamped into the staged
+                // rectangle above)
+                const int pp = ti.P[li] + tj.P[lj], pn = ti.P[li] + tj.Nn[lj];
+                const int np = ti.Nn[li] + tj.P[lj], nn = ti.Nn[li] + tj.Nn[lj];
 #define LCODE_MIX(c) (A * cs[c][pp] + B * cs[c][pn] + C * cs[c][np] + D * cs[c][nn])
                 x = ti.fg[li] + LCODE_MIX(0);
                 y = tj.fg[lj] + LCODE_MIX(1);
@@ -420,8 +435,13 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) pat[c][k] = 0.;
             int ai = -1, aj = -1;   // centre of the sums in pat (-1: none)
+            int qi = 0, qj = 0;     // particle (qi, qj) of the group
             for (int q = 0; q <= gs * gs; ++q) {
-                const int li = gi * gs + q / gs, lj = gj * gs + q % gs;
+                const int li = gi * gs + qi, lj = gj * gs + qj;
+                if (++qj == gs) {
+                    qj = 0;
+                    ++qi;
+                }
                 double v[4] = {0., 0., 0., 0.};
                 Tsc t;
                 t.i = t.j = -1;
