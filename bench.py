@@ -353,6 +353,46 @@ SMALL_WORKLOAD = dict(grid_steps=129, grid_step_size=.05, xi_step_size=.05,
                       reflect_padding_steps=5, plasma_padding_steps=10)
 
 
+def guided_trace_leg(lc, cosine_gaussian, torch, slices):
+    """The guided step in the developed wake: `slices` plain FP32-transform
+    steps (beam slices 103 on), then `slices` guided steps (quadratic
+    extrapolation, threshold GUIDED_THRESHOLD), each block timed without a
+    synchronisation inside; then `slices` more guided steps with the
+    residual read back after each, for the acceptance count."""
+    cfg = make_config()
+    cfg.field_solver_precision = "fp32"
+    _, _, const, vp, st = lc.init(cfg)
+    nc, nf = const.m.shape[0], vp.fine_grid.numel()
+    ex = lc.FieldExtrapolator(order=2)
+    beams = [cosine_gaussian(cfg, 100 + k) for k in range(3 + 3 * slices)]
+    for k in range(3):
+        st = lc.step(cfg, const, vp, st, beams[k])
+        ex.push(st)
+    out = {"workload": "641^2, FP32 transforms, slices 103 on", "slices": slices,
+           "threshold": GUIDED_THRESHOLD}
+    for b, mode in enumerate(("plain", "guided")):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(slices):
+            beam = beams[3 + b * slices + k]
+            if mode == "plain":
+                st = lc.step(cfg, const, vp, st, beam)
+            else:
+                st = lc.step(cfg, const, vp, st, beam, guess=ex.guess(),
+                             threshold=GUIDED_THRESHOLD)
+            ex.push(st)
+        torch.cuda.synchronize()
+        out[f"{mode}_us_per_slice"] = (time.perf_counter() - t0) / slices * 1e6
+    accepted = 0
+    for k in range(slices):
+        st = lc.step(cfg, const, vp, st, beams[3 + 2 * slices + k],
+                     guess=ex.guess(), threshold=GUIDED_THRESHOLD)
+        ex.push(st)
+        accepted += lc.guided_residual(cfg, nc, nf) <= GUIDED_THRESHOLD
+    out["accepted_after"] = accepted
+    return out
+
+
 def small_workload_leg(lc, cosine_gaussian, torch, with_cpu):
     """The 129^2 reference-runnable case in full (200 slices of the same
     beam shape): the HIP step's throughput and, when asked, the oracle's
@@ -443,6 +483,9 @@ def main():
                     help="skip the mixed+learned leg")
     ap.add_argument("--no-small-workload", dest="small_workload", action="store_false",
                     help="skip the 129^2 leg (GPU and oracle)")
+    ap.add_argument("--guided-trace", type=int, default=0, metavar="SLICES",
+                    help="only time plain against guided steps in the developed "
+                         "wake (SLICES each) and count accepted guesses")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -458,6 +501,13 @@ def main():
     from lcode3d_amd.beams import cosine_gaussian
 
     torch.cuda.set_device(0)
+    if args.guided_trace:
+        r = guided_trace_leg(lc, cosine_gaussian, torch, args.guided_trace)
+        log(f"plain {r['plain_us_per_slice']:.1f} us/slice, guided "
+            f"{r['guided_us_per_slice']:.1f} us/slice, accepted "
+            f"{r['accepted_after']} of {r['slices']} after")
+        print(json.dumps({"guided_trace": r}))
+        return
     cfg = make_config()
     cfg.field_solver_precision = args.precision
     xs, ys, const, vp, state = lc.init(cfg)
