@@ -150,38 +150,32 @@ def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps,
 
 
 GUIDED_THRESHOLD = 2e-3   # above the one-pass corrector's own residual
+GUIDED_BACKOFF = 8        # plain slices after a rejected guess (doubling to 64)
                           # (about 0.1 xi_step_size = 5.5e-4, p90 1.1e-3)
 
 
-def mixed_guided_leg(lc, cosine_gaussian, torch, slices, timed, precision):
-    """Mixed precision (FP32 transforms) + guided predictor (quadratic
-    extrapolation of the last three slices' fields in place of the predictor
-    pass, residual threshold GUIDED_THRESHOLD) against the plain FP64 step over the same
-    `slices` slices from init: on-axis Ez(xi) relative L2, the worst full-Ez
-    relative L2 of the 100-slice checkpoints, the acceptance rate, and the
-    throughput over `timed` slices of the bunch."""
+def _guided_runs(lc, cosine_gaussian, torch, slices, timed, precision,
+                 predictor=None):
+    """The guided step (GuidedStepper: the guess on the slices the backoff
+    policy selects, GUIDED_THRESHOLD, GUIDED_BACKOFF) against the plain FP64
+    step over `slices` slices from init: on-axis Ez(xi) relative L2 and the
+    worst full-Ez relative L2 of the 100-slice checkpoints; then its
+    throughput over `timed` slices of the bunch (beam slices 103 on), guess
+    inference and the residual read-backs included."""
     cfg_p = make_config()
     cfg_g = make_config()
     cfg_g.field_solver_precision = precision
     _, _, const, vp, sp = lc.init(cfg_p)
     lc.init(cfg_g)
     sg = sp
-    nc, nf = const.m.shape[0], vp.fine_grid.numel()
-    ex = lc.FieldExtrapolator(order=2)
     n = cfg_p.grid_steps
-    ez_p, ez_g, full, accepted, guided = [], [], 0.0, 0, 0
+    gs = lc.GuidedStepper(cfg_g, const, vp, GUIDED_THRESHOLD, predictor=predictor,
+                          backoff=GUIDED_BACKOFF)
+    ez_p, ez_g, full = [], [], 0.0
     for i in range(slices):
         beam = cosine_gaussian(cfg_p, i)
         sp = lc.step(cfg_p, const, vp, sp, beam)
-        g = ex.guess()
-        if g is None:
-            sg = lc.step(cfg_g, const, vp, sg, beam)
-        else:
-            sg = lc.step(cfg_g, const, vp, sg, beam, guess=g,
-                         threshold=GUIDED_THRESHOLD)
-            guided += 1
-            accepted += lc.guided_residual(cfg_g, nc, nf) <= GUIDED_THRESHOLD
-        ex.push(sg)
+        sg = gs.step(sg, beam)
         ez_p.append(sp.Ez[n // 2, n // 2])
         ez_g.append(sg.Ez[n // 2, n // 2])
         if (i + 1) % 100 == 0:
@@ -189,28 +183,38 @@ def mixed_guided_leg(lc, cosine_gaussian, torch, slices, timed, precision):
             full = max(full, e)
     ez_p, ez_g = torch.stack(ez_p), torch.stack(ez_g)
     axis = (torch.linalg.norm(ez_g - ez_p) / torch.linalg.norm(ez_p)).item()
-    # throughput of the mixed + guided step inside the bunch
+    out = {"precision": precision, "slices_compared": slices,
+           "ez_axis_rel_l2": axis, "ez_checkpoint_rel_l2_max": full,
+           "guided_slices": gs.guided, "accepted": gs.accepted, "plain_slices": gs.plain,
+           "guess_accept_rate": gs.accepted / max(gs.guided, 1),
+           "backoff": GUIDED_BACKOFF}
     beams = [cosine_gaussian(cfg_g, 100 + k) for k in range(timed + 3)]
     _, _, const, vp, st = lc.init(cfg_g)
-    ex = lc.FieldExtrapolator(order=2)
+    gs = lc.GuidedStepper(cfg_g, const, vp, GUIDED_THRESHOLD, predictor=predictor,
+                          backoff=GUIDED_BACKOFF)
     for k in range(3):
         st = lc.step(cfg_g, const, vp, st, beams[k])
-        ex.push(st)
+        gs.push(st)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(timed):
-        st = lc.step(cfg_g, const, vp, st, beams[3 + k], guess=ex.guess(),
-                     threshold=GUIDED_THRESHOLD)
-        ex.push(st)
+        st = gs.step(st, beams[3 + k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return {"precision": precision, "predictor": "quadratic extrapolation "
-            "of the last 3 slices' fields, corrector residual threshold "
-            f"{GUIDED_THRESHOLD:g}",
-            "value": timed / dt, "unit": "xi-steps/s",
-            "slices_compared": slices,
-            "ez_axis_rel_l2": axis, "ez_checkpoint_rel_l2_max": full,
-            "guess_accept_rate": accepted / max(guided, 1)}
+    out.update({"value": timed / dt, "unit": "xi-steps/s",
+                "timed_guided_slices": gs.guided, "timed_accepted": gs.accepted,
+                "timed_plain_slices": gs.plain})
+    return out
+
+
+def mixed_guided_leg(lc, cosine_gaussian, torch, slices, timed, precision):
+    """Mixed precision (FP32 transforms) + the guided step with the
+    quadratic extrapolation of the last three slices' fields as the guess."""
+    out = _guided_runs(lc, cosine_gaussian, torch, slices, timed, precision)
+    out["predictor"] = ("quadratic extrapolation of the last 3 slices' fields, "
+                        f"corrector residual threshold {GUIDED_THRESHOLD:g}, "
+                        f"backoff {GUIDED_BACKOFF} slices after a rejection")
+    return out
 
 
 def learned_leg(lc, cosine_gaussian, torch, slices, timed, precision,
@@ -255,57 +259,14 @@ def learned_leg(lc, cosine_gaussian, torch, slices, timed, precision,
     log(f"learned predictor: {len(params)} runs x {train_slices} slices, {epochs} epochs, "
         f"loss {hist[0]:.3g} -> {hist[-1]:.3g}, {train_s:.1f} s")
 
-    cfg_g = make_config()
-    cfg_g.field_solver_precision = precision
-    _, _, const, vp, sp = lc.init(cfg_p)
-    lc.init(cfg_g)
-    sg = sp
-    nc, nf = const.m.shape[0], vp.fine_grid.numel()
-    n = cfg_p.grid_steps
-    last = collections.deque(maxlen=3)
-    ez_p, ez_g, full, accepted, guided = [], [], 0.0, 0, 0
-    for i in range(slices):
-        beam = cosine_gaussian(cfg_p, i)
-        sp = lc.step(cfg_p, const, vp, sp, beam)
-        if len(last) < 3:
-            sg = lc.step(cfg_g, const, vp, sg, beam)
-        else:
-            g = net.guess(last[2], last[1], last[0], beam)
-            sg = lc.step(cfg_g, const, vp, sg, beam, guess=[(1.0, g)],
-                         threshold=GUIDED_THRESHOLD)
-            guided += 1
-            accepted += lc.guided_residual(cfg_g, nc, nf) <= GUIDED_THRESHOLD
-        last.append(sg)
-        ez_p.append(sp.Ez[n // 2, n // 2])
-        ez_g.append(sg.Ez[n // 2, n // 2])
-        if (i + 1) % 100 == 0:
-            e = (torch.linalg.norm(sg.Ez - sp.Ez) / torch.linalg.norm(sp.Ez)).item()
-            full = max(full, e)
-    ez_p, ez_g = torch.stack(ez_p), torch.stack(ez_g)
-    axis = (torch.linalg.norm(ez_g - ez_p) / torch.linalg.norm(ez_p)).item()
-    beams = [cosine_gaussian(cfg_g, 100 + k) for k in range(timed + 3)]
-    _, _, const, vp, st = lc.init(cfg_g)
-    last = collections.deque(maxlen=3)
-    for k in range(3):
-        st = lc.step(cfg_g, const, vp, st, beams[k])
-        last.append(st)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(timed):
-        g = net.guess(last[2], last[1], last[0], beams[3 + k])
-        st = lc.step(cfg_g, const, vp, st, beams[3 + k], guess=[(1.0, g)],
-                     threshold=GUIDED_THRESHOLD)
-        last.append(st)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    return {"precision": precision,
-            "predictor": f"ConvPredictor hidden {list(hidden)}, trained on {len(params)} "
-                         f"plain runs x {train_slices} slices, {epochs} epochs; corrector "
-                         f"residual threshold {GUIDED_THRESHOLD:g}",
-            "train_seconds": train_s, "train_loss": hist,
-            "value": timed / dt, "unit": "xi-steps/s", "slices_compared": slices,
-            "ez_axis_rel_l2": axis, "ez_checkpoint_rel_l2_max": full,
-            "guess_accept_rate": accepted / max(guided, 1)}
+    out = _guided_runs(lc, cosine_gaussian, torch, slices, timed, precision,
+                       predictor=net.guess)
+    out["predictor"] = (f"ConvPredictor hidden {list(hidden)}, trained on {len(params)} "
+                        f"plain runs x {train_slices} slices, {epochs} epochs; corrector "
+                        f"residual threshold {GUIDED_THRESHOLD:g}, backoff "
+                        f"{GUIDED_BACKOFF} slices after a rejection")
+    out.update({"train_seconds": train_s, "train_loss": hist})
+    return out
 
 
 def precision_table(lc, cosine_gaussian, torch, slices):
@@ -458,7 +419,7 @@ def cpu_baseline(steps):
 
 
 def main():
-    global GUIDED_THRESHOLD
+    global GUIDED_THRESHOLD, GUIDED_BACKOFF
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
@@ -483,6 +444,8 @@ def main():
                     help="skip the mixed+learned leg")
     ap.add_argument("--no-small-workload", dest="small_workload", action="store_false",
                     help="skip the 129^2 leg (GPU and oracle)")
+    ap.add_argument("--guided-backoff", type=int, default=GUIDED_BACKOFF,
+                    help="plain slices after a rejected guess (0: guess every slice)")
     ap.add_argument("--guided-trace", type=int, default=0, metavar="SLICES",
                     help="only time plain against guided steps in the developed "
                          "wake (SLICES each) and count accepted guesses")
@@ -490,6 +453,7 @@ def main():
                     help="run a device copy of this size (PMC calibration)")
     args = ap.parse_args()
     GUIDED_THRESHOLD = args.guided_threshold
+    GUIDED_BACKOFF = args.guided_backoff
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): "
               "the xi step is sequential in xi and this build has no "
@@ -604,8 +568,10 @@ def main():
             f"({mixed['vs_plain']:.2f}x plain), Ez axis rel L2 "
             f"{mixed['ez_axis_rel_l2']:.2e}, checkpoints max "
             f"{mixed['ez_checkpoint_rel_l2_max']:.2e} over "
-            f"{mixed['slices_compared']} slices, accepted "
-            f"{mixed['guess_accept_rate'] * 100:.0f}%")
+            f"{mixed['slices_compared']} slices ({mixed['accepted']} of "
+            f"{mixed['guided_slices']} guesses accepted, {mixed['plain_slices']} plain "
+            f"slices); timed: {mixed['timed_accepted']} of {mixed['timed_guided_slices']} "
+            f"accepted, {mixed['timed_plain_slices']} plain")
 
     learned = None
     if args.mixed_slices and args.learned:
@@ -616,8 +582,10 @@ def main():
             f"({learned['vs_plain']:.2f}x plain), Ez axis rel L2 "
             f"{learned['ez_axis_rel_l2']:.2e}, checkpoints max "
             f"{learned['ez_checkpoint_rel_l2_max']:.2e} over "
-            f"{learned['slices_compared']} slices, accepted "
-            f"{learned['guess_accept_rate'] * 100:.0f}%")
+            f"{learned['slices_compared']} slices ({learned['accepted']} of "
+            f"{learned['guided_slices']} guesses accepted, {learned['plain_slices']} plain "
+            f"slices); timed: {learned['timed_accepted']} of {learned['timed_guided_slices']} "
+            f"accepted, {learned['timed_plain_slices']} plain")
 
     small = None
     if args.small_workload:

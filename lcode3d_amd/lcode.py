@@ -490,6 +490,78 @@ class FieldExtrapolator:
         return [(c, self.history[k]) for c, k in self.COEFS[n]]
 
 
+class GuessBackoff:
+    """When to offer a guess: after a rejected guess the next `wait` slices
+    run the plain predictor; `wait` starts at `backoff`, doubles with every
+    further rejection up to `max_backoff`, and goes back to 0 (guess every
+    slice) after an accepted one.  backoff = 0: guess every slice."""
+
+    def __init__(self, backoff=8, max_backoff=64):
+        self.backoff = int(backoff)
+        self.max_backoff = max(int(max_backoff), self.backoff)
+        self.next_wait = self.backoff
+        self.wait = 0
+
+    def want_guess(self):
+        """True when this slice should be guided; counts a skipped slice."""
+        if self.wait > 0:
+            self.wait -= 1
+            return False
+        return True
+
+    def observe(self, accepted):
+        """Outcome of the guided slice just run."""
+        if accepted:
+            self.next_wait = self.backoff
+        else:
+            self.wait = self.next_wait
+            self.next_wait = min(2 * self.next_wait, self.max_backoff) if self.backoff else 0
+
+
+class GuidedStepper:
+    """step() with a guess on the slices GuessBackoff selects: the guess is
+    the quadratic extrapolation of the last three states' fields, or the
+    fields a predictor(s1, s2, s3, beam) returns (states n-1, n-2, n-3;
+    e.g. predictor.ConvPredictor.guess).  Each guided slice reads the
+    corrector's residual back (one synchronisation) to decide on the next
+    ones; the others run the plain step, asynchronously.  Counts guided,
+    accepted and plain slices."""
+
+    def __init__(self, config, const, virt_params, threshold, predictor=None,
+                 backoff=8, max_backoff=64):
+        self.config, self.const, self.vp = config, const, virt_params
+        self.threshold = float(threshold)
+        self.predictor = predictor
+        self.policy = GuessBackoff(backoff, max_backoff)
+        self.extrap = FieldExtrapolator(order=2)
+        self.guided = self.accepted = self.plain = 0
+
+    def push(self, state):
+        """Record a state computed outside the stepper (history only)."""
+        self.extrap.push(state)
+
+    def step(self, prev, beam_ro):
+        h = self.extrap.history
+        ready = len(h) >= 3
+        if ready and self.policy.want_guess():
+            if self.predictor is None:
+                guess = self.extrap.guess()
+            else:
+                guess = [(1.0, self.predictor(h[0], h[1], h[2], beam_ro))]
+            st = step(self.config, self.const, self.vp, prev, beam_ro,
+                      guess=guess, threshold=self.threshold)
+            _, nf, _ = _virt_struct(self.vp)    # the workspace step() used
+            ok = guided_residual(self.config, prev.x_offt.shape[0], nf) <= self.threshold
+            self.policy.observe(ok)
+            self.guided += 1
+            self.accepted += int(ok)
+        else:
+            st = step(self.config, self.const, self.vp, prev, beam_ro)
+            self.plain += 1
+        self.extrap.push(st)
+        return st
+
+
 # ---------------------------------------------------------------------------
 # Plasma set-up and init (lcode.py:454-573, :936-975); runs once, on the host
 # like the reference's, then uploads

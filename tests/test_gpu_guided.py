@@ -13,6 +13,9 @@ predictor runs when it is too large.
   5.2e-4): an accepted slice differs from the plain step by about the
   threshold, which is the accuracy this hook trades for the skipped
   predictor.
+* GuidedStepper backs off after rejected guesses: with threshold 0 its
+  guided slices follow the doubling waits exactly and its result is the
+  plain step's.
 
 The threshold: with the reference's single corrector pass the fields are
 one fixed-point iteration from the predictor, not converged, so even the
@@ -118,3 +121,29 @@ def test_extrapolated_guess_accepted_and_accurate(lc):
     for name, e in worst.items():
         assert e < 1e-2, (name, e)
     assert ez_err < 5e-3
+
+
+def test_stepper_backs_off_after_rejections(lc):
+    """GuidedStepper with threshold 0 (every guess rejected) and backoff 2
+    (doubling to 8): guesses on slices 3, 6, 11, 20, 29 and 38 of 40 (the
+    first three build the history), plain steps otherwise, and the result
+    is the plain step's to 1e-10 (a rejected guess falls back to it)."""
+    say("guided stepper backoff")
+    cfg = _cfg()
+    xs, ys, const, vp, st_plain = lc.init(cfg)
+    st = st_plain
+    gs = lc.GuidedStepper(cfg, const, vp, threshold=0.0, backoff=2, max_backoff=8)
+    guided_at = []
+    for i in range(40):
+        beam = _beam(lc, cfg, i)
+        st_plain = lc.step(cfg, const, vp, st_plain, beam)
+        before = gs.guided
+        st = gs.step(st, beam)
+        if gs.guided > before:
+            guided_at.append(i)
+    assert guided_at == [3, 6, 11, 20, 29, 38], guided_at
+    assert gs.accepted == 0 and gs.plain == 34
+    for name in STATE:
+        floor = torch.linalg.norm(st_plain.Bx).item() if name == "Bz" else 0.
+        assert rel_l2(host(getattr(st, name)), host(getattr(st_plain, name)),
+                      floor) < 1e-10, name
