@@ -8,6 +8,8 @@ the CPU oracle.
 * 641^2 (grid_step_size 0.025, xi_step_size 0.005, coarseness 3,
   fineness 2): one step from a perturbed state, relative L2 <= 1e-11.
 * Variant B and the Laplace (trick 0) paths, one step each.
+* Particles displaced up to 4.5 cells (beyond the deposition tiles' 3-cell
+  halo: the spill path inside the fused step), two steps.
 Tolerances are relative L2 over each array; Bz uses the Bx norm as its floor
 (Bz is small for a round beam).
 """
@@ -128,3 +130,33 @@ def test_step_accepts_scalar_and_host_beams(lc, oracle):
         assert torch.abs(getattr(a, name)).max().item() < 1e-12, name
     view = lc.GPUArraysView(a)
     assert isinstance(view.Ez, np.ndarray) and view.Ez.shape == (129, 129)
+
+
+def test_step_particles_beyond_the_deposition_halo(lc, oracle):
+    """Offsets up to 4.5 cells: the pushes' displacement bound exceeds the
+    3-cell tile halo, so the fused step's depositions list the particles
+    beyond it and deposit them with global atomics (the spill path), and
+    the bound slots hand over between the steps; two steps against the
+    oracle, relative L2 <= 1e-11 and 1e-10."""
+    say("129^2 two steps, offsets up to 4.5 cells")
+    cfg_o, cfg_g = _cfg(), _cfg()
+    xs, ys, const, vp, st = perturbed_state(oracle, cfg_o, seed=5,
+                                            amp_x=4.5 * cfg_o.grid_step_size)
+    lc.init(cfg_g)
+    gconst = lc.GPUArrays(**{k: dev(getattr(const, k)) for k in (
+        "m", "q", "x_init", "y_init", "ro_initial")})
+    gvp = lc.GPUArrays(**{k: getattr(vp, k) for k in (
+        "influence_prev", "influence_next", "indices_prev", "indices_next",
+        "fine_grid")})
+    beam = .05 * np.exp(-.5 * (xs**2 + ys**2))
+    ref, got = st, to_device_state(lc, st)
+    for k, tol in enumerate((1e-11, 1e-10)):
+        ref = oracle.step(cfg_o, const, vp, ref, beam)
+        got = lc.step(cfg_g, gconst, gvp, got, dev(beam))
+        for name in STATE:
+            err = rel_l2(host(getattr(got, name)), getattr(ref, name), _floor(name, ref))
+            assert err < tol, (k, name, err)
+    disp = max(np.abs(ref.x_offt).max(), np.abs(ref.y_offt).max()) / cfg_o.grid_step_size
+    say(f"  max offset after two steps {disp:.2f} cells")
+    assert disp > 3.5
+    assert lc.fault_count(cfg_g, const.m.shape[0]) == 0
