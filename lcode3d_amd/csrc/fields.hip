@@ -568,33 +568,6 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     const S* taba = static_cast<const S*>(fp.table) + (size_t)la * len;
     const S* tabb = static_cast<const S*>(fp.table) + (size_t)lb * len;
 
-    // the spectral multipliers of this thread's points, before the
-    // transform (they do not depend on it); rows and points clamped in
-    // bounds, unused values dropped
-    S ta[kPts], tb[kPts];
-    if (LCODE_FFT_TABLE_INLINE) {
-        // evaluated as the tables are built on the host, operation for
-        // operation (the same values, none of the table's HBM traffic)
-        const double lla = fp.lamL[va ? la : 0], llb = fp.lamL[vb ? lb : 0];
-#pragma unroll
-        for (int u = 0; u < kPts; ++u) {
-            const int kc = min(t + u * Tp, len - 1);
-            const double lk = fp.lamK[kc];
-            const double a = 1 / (lla + lk + fp.lam_add) / fp.lam_norm;
-            const double c = 1 / (llb + lk + fp.lam_add) / fp.lam_norm;
-            ta[u] = (S)(fp.zero_mode && la == 0 && kc == 0 ? 0. : a);
-            tb[u] = (S)c;
-        }
-    } else {
-        const size_t oa = (size_t)(va ? la : 0) * len, ob = (size_t)(vb ? lb : 0) * len;
-        const S* tab = static_cast<const S*>(fp.table);
-#pragma unroll
-        for (int u = 0; u < kPts; ++u) {
-            const int kc = min(t + u * Tp, len - 1);
-            ta[u] = tab[oa + kc];
-            tb[u] = tab[ob + kc];
-        }
-    }
     if constexpr (L_ != 0) {
         constexpr int kPer = points_per_thread(L_);
         const S* T = static_cast<const S*>(fp.T);
@@ -619,6 +592,34 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     put_ext_zeros(z, xf, len, t);
     __syncthreads();
     fft_any<L_, S>(z, g, tw, t);
+    // the spectral multipliers of this thread's points, all loaded before
+    // their first use (rows and points clamped in bounds, unused values
+    // dropped); loading them before the first transform instead keeps 12
+    // more VGPRs live across it, which spill under the 64-VGPR budget
+    S ta[kPts], tb[kPts];
+    if (LCODE_FFT_TABLE_INLINE) {
+        // evaluated as the tables are built on the host, operation for
+        // operation (the same values, none of the table's HBM traffic)
+        const double lla = fp.lamL[va ? la : 0], llb = fp.lamL[vb ? lb : 0];
+#pragma unroll
+        for (int u = 0; u < kPts; ++u) {
+            const int kc = min(t + u * Tp, len - 1);
+            const double lk = fp.lamK[kc];
+            const double a = 1 / (lla + lk + fp.lam_add) / fp.lam_norm;
+            const double c = 1 / (llb + lk + fp.lam_add) / fp.lam_norm;
+            ta[u] = (S)(fp.zero_mode && la == 0 && kc == 0 ? 0. : a);
+            tb[u] = (S)c;
+        }
+    } else {
+        const size_t oa = (size_t)(va ? la : 0) * len, ob = (size_t)(vb ? lb : 0) * len;
+        const S* tab = static_cast<const S*>(fp.table);
+#pragma unroll
+        for (int u = 0; u < kPts; ++u) {
+            const int kc = min(t + u * Tp, len - 1);
+            ta[u] = tab[oa + kc];
+            tb[u] = tab[ob + kc];
+        }
+    }
 
     S ha[kPts], hb[kPts];
 #pragma unroll
