@@ -354,6 +354,64 @@ def guided_trace_leg(lc, cosine_gaussian, torch, slices):
     return out
 
 
+LARGE_WORKLOAD = dict(grid_steps=1281, grid_step_size=.0125, xi_step_size=.0025,
+                      plasma_coarseness=3, plasma_fineness=3)
+LARGE_BEAM = dict(amplitude=.1, sigma=.7, shift_x=.5)
+
+
+def large_workload_leg(lc, cosine_gaussian, torch, slices):
+    """The large non-axisymmetric case: 1281^2 (h 0.0125, xi step 0.0025,
+    coarseness 3, fineness 3), beam amplitude 0.1, sigma 0.7, offset +0.5
+    in x.  The plain FP64 run and the mixed run (FP32 transforms,
+    GuidedStepper with the quadratic extrapolation) over the same `slices`
+    slices, each timed on its own; Ez(xi, 0, 0) and the full Ez every 500
+    slices compared between them."""
+    def cfg(precision):
+        c = make_config()
+        for k, v in LARGE_WORKLOAD.items():
+            setattr(c, k, v)
+        c.field_solver_precision = precision
+        return c
+    out = {"workload": "1281^2, coarseness 3, fineness 3, beam 0.1 / 0.7 / +0.5 in x",
+           "slices": slices}
+    runs = {}
+    for mode in ("plain", "mixed"):
+        c = cfg("fp64" if mode == "plain" else "fp32")
+        _, _, const, vp, st = lc.init(c)
+        beams = [cosine_gaussian(c, i, **LARGE_BEAM) for i in range(slices)]
+        gs = (lc.GuidedStepper(c, const, vp, GUIDED_THRESHOLD, backoff=GUIDED_BACKOFF)
+              if mode == "mixed" else None)
+        n = c.grid_steps
+        axis, checks = [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(slices):
+            st = gs.step(st, beams[i]) if gs else lc.step(c, const, vp, st, beams[i])
+            axis.append(st.Ez[n // 2, n // 2])
+            if (i + 1) % 500 == 0:
+                checks.append(st.Ez.clone())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        del beams
+        runs[mode] = (torch.stack(axis), checks)
+        out[f"{mode}_xi_steps_per_s"] = slices / dt
+        out[f"{mode}_faults"] = lc.fault_count(c, const.m.shape[0])
+        if gs:
+            out.update({"guided_slices": gs.guided, "accepted": gs.accepted,
+                        "plain_slices": gs.plain})
+        else:
+            out["max_offset_cells"] = max(st.x_offt.abs().max().item(),
+                                          st.y_offt.abs().max().item()) / c.grid_step_size
+        del st
+    (ap, cp), (am, cm) = runs["plain"], runs["mixed"]
+    out["mixed_vs_plain"] = out["mixed_xi_steps_per_s"] / out["plain_xi_steps_per_s"]
+    out["ez_axis_rel_l2"] = (torch.linalg.norm(am - ap) / torch.linalg.norm(ap)).item()
+    out["ez_checkpoint_rel_l2_max"] = max(
+        [(torch.linalg.norm(m - p) / torch.linalg.norm(p)).item() for p, m in zip(cp, cm)],
+        default=0.0)
+    return out
+
+
 def small_workload_leg(lc, cosine_gaussian, torch, with_cpu):
     """The 129^2 reference-runnable case in full (200 slices of the same
     beam shape): the HIP step's throughput and, when asked, the oracle's
@@ -446,6 +504,9 @@ def main():
                     help="skip the 129^2 leg (GPU and oracle)")
     ap.add_argument("--guided-backoff", type=int, default=GUIDED_BACKOFF,
                     help="plain slices after a rejected guess (0: guess every slice)")
+    ap.add_argument("--large", type=int, default=0, metavar="SLICES",
+                    help="also run the 1281^2 fineness-3 off-axis case, plain FP64 and "
+                         "mixed + guided, for SLICES slices (0: skip)")
     ap.add_argument("--guided-trace", type=int, default=0, metavar="SLICES",
                     help="only time plain against guided steps in the developed "
                          "wake (SLICES each) and count accepted guesses")
@@ -587,6 +648,17 @@ def main():
             f"slices); timed: {learned['timed_accepted']} of {learned['timed_guided_slices']} "
             f"accepted, {learned['timed_plain_slices']} plain")
 
+    large = None
+    if args.large:
+        large = large_workload_leg(lc, cosine_gaussian, torch, args.large)
+        log(f"1281^2 x {large['slices']} slices: plain {large['plain_xi_steps_per_s']:.1f}, "
+            f"mixed+guided {large['mixed_xi_steps_per_s']:.1f} xi-steps/s "
+            f"({large['mixed_vs_plain']:.2f}x); Ez axis rel L2 {large['ez_axis_rel_l2']:.2e}, "
+            f"checkpoints max {large['ez_checkpoint_rel_l2_max']:.2e}; "
+            f"{large['accepted']} of {large['guided_slices']} guesses accepted; max offset "
+            f"{large['max_offset_cells']:.2f} cells; faults {large['plain_faults']}/"
+            f"{large['mixed_faults']}")
+
     small = None
     if args.small_workload:
         small = small_workload_leg(lc, cosine_gaussian, torch, not args.no_cpu_baseline)
@@ -631,6 +703,7 @@ def main():
         "mixed_learned": learned,
         "precision_table": ptable,
         "small_workload": small,
+        "large_workload": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

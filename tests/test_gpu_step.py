@@ -6,7 +6,8 @@ the CPU oracle.
   amplitude 0.05): 200 slices from init, per-field relative L2 <= 1e-10
   every 50 slices and Ez(xi, 0, 0) on every slice.
 * 641^2 (grid_step_size 0.025, xi_step_size 0.005, coarseness 3,
-  fineness 2): one step from a perturbed state, relative L2 <= 1e-11.
+  fineness 2) and 1281^2 (0.0125, 0.0025, coarseness 3, fineness 3): one
+  step from a perturbed state, relative L2 <= 1e-11.
 * Variant B and the Laplace (trick 0) paths, one step each.
 * Particles displaced up to 4.5 cells (beyond the deposition tiles' 3-cell
   halo: the spill path inside the fused step), two steps.
@@ -101,6 +102,19 @@ def test_step_641_one_step_perturbed(lc, oracle):
               plasma_coarseness=3, plasma_fineness=2)
     errs = _one_step(lc, oracle, _cfg(**kw), _cfg(**kw), seed=11)
     say("641^2 rel L2:", {k: f"{v:.1e}" for k, v in errs.items()})
+    for name, err in errs.items():
+        assert err < 1e-11, (name, err)
+
+
+def test_step_1281_fineness_3_one_step_perturbed(lc, oracle):
+    """1281^2 (grid_step_size 0.0125, xi_step_size 0.0025, coarseness 3,
+    fineness 3): transforms of 2560 points (one line pair per block,
+    twiddles from global memory) and groups of 9 fine particles."""
+    say("1281^2 fineness 3 one step")
+    kw = dict(grid_steps=1281, grid_step_size=.0125, xi_step_size=.0025,
+              plasma_coarseness=3, plasma_fineness=3)
+    errs = _one_step(lc, oracle, _cfg(**kw), _cfg(**kw), seed=13, beam_scale=.1)
+    say("1281^2 rel L2:", {k: f"{v:.1e}" for k, v in errs.items()})
     for name, err in errs.items():
         assert err < 1e-11, (name, err)
 
