@@ -76,8 +76,11 @@ namespace {
 #endif
 
 // Cells a fine particle may move from its initial cell and still be
-// deposited through the LDS tiles (beyond: the atomic spill path).
-constexpr int kDepositHalo = 3;
+// deposited through the LDS tiles (beyond: the atomic spill path, which the
+// last block of the kernel runs alone).  The adaptive halo is the pushes'
+// displacement bound, so small displacements keep small tiles; 6 covers the
+// nonlinear 1281^2 case (up to ~4.4 cells) within the LDS tables.
+constexpr int kDepositHalo = 6;
 
 // Round-to-nearest-even BF16 bits of a float (finite inputs).
 uint16_t bf16_bits(float f) {

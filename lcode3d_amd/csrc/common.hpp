@@ -94,7 +94,7 @@ __device__ __forceinline__ bool gate_open(const Gate& g) {
 #endif
 constexpr int kDepTile = LCODE_DEP_TILE;
 // LDS sizes of a deposition tile: fine indices per side (at least fineness
-// (kDepTile + 2 halo + 2) for fineness <= 3, halo <= 3) and coarse
+// (kDepTile + 2 halo + 2) for fineness <= 3, halo <= 6) and coarse
 // particles per side of the staged coarse rectangle (a tile whose rectangle
 // is larger takes the unstaged kernel).
 #ifndef LCODE_DEP_FINE_MAX

@@ -120,7 +120,7 @@ def test_move_estimate_and_move_smart(lc, oracle, N, c, f):
                                          (129, 2, 2, 7), (641, 3, 2, 7)])
 def test_deposit(lc, oracle, N, c, f, cells):
     """cells: amplitude of the random particle offsets in cells; at 7 most
-    particles leave the 3-cell tile halo and take the atomic spill path."""
+    particles leave the tile halo (at most 6 cells) and take the atomic spill path."""
     say(f"deposit N={N} c={c} f={f} offsets<={cells} cells")
     cfg = _cfg(N, h=3.2 / N, c=c, f=f)
     const, vp, xo, yo, px, py, pz = _particles(oracle, cfg, N + 3,
