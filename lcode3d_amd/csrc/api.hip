@@ -31,7 +31,6 @@ struct lcode_workspace {
     double* costab = nullptr;      // quarter-wave cosine table
     double2* stage_tw = nullptr;   // static-plan stage twiddles
     float* costab32 = nullptr;
-    float2* stage_tw32 = nullptr;
     int fp32 = 0;                  // LCODE_SOLVER_FFT32
     int solver = 0;                // LCODE_SOLVER_*
     DenseGeom dense{};             // LCODE_SOLVER_DENSE*: matrices, scratch
@@ -290,7 +289,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     int P;
     if (g.static_L) {
         // stage twiddles in LDS when that still leaves two line pairs
-        P = pairs_for(solver_tw_count(L) * cplx);
+        P = pairs_for(solver_tw_count(L) * (int)sizeof(double2));   // FP64 roots
         g.tw_lds = LCODE_TW_LDS_OK && P >= 2;
         if (!g.tw_lds) P = pairs_for(0);
     } else {
@@ -323,13 +322,6 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         }
         if ((rc = upload(ws, &ws->stage_tw, tw))) return fail(rc);
         g.stage_tw = ws->stage_tw;
-        if (ws->fp32) {
-            std::vector<float2> tw32(tw.size());
-            for (size_t k = 0; k < tw.size(); ++k)
-                tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
-            if ((rc = upload(ws, &ws->stage_tw32, tw32))) return fail(rc);
-            g.stage_tw32 = ws->stage_tw32;
-        }
     }
 
     std::vector<double> costab(L / 4 + 1);

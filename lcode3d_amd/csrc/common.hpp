@@ -457,7 +457,6 @@ struct SolverGeom {
     const double* costab;     // cos(2 pi r / L), r in [0, L / 4]
     const double2* stage_tw;  // per-stage twiddles of the static plan
     const float* costab32;    // the same tables in FP32
-    const float2* stage_tw32;
     int fp32;                 // transforms, spectra and tables in FP32
     int static_L;             // L with a compile-time plan, else 0
     int tw_lds;               // static plan: twiddles copied to LDS

@@ -267,7 +267,7 @@ __host__ __device__ constexpr int points_per_thread(int L) {
 }
 
 template <int L, int SPAN, typename V>
-__device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
+__device__ __forceinline__ void fft_static(V* z, const double2* __restrict__ tw,
                                            int t) {
     if constexpr (SPAN < L) {
         constexpr int R = next_radix(L / SPAN);
@@ -281,7 +281,9 @@ __device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
             const int b = t + it * Tp;
             if (b < T) {
                 const int k = b % SPAN;
-                V w[R];
+                // the root in FP64 and its powers formed in FP64, each
+                // rounded once to the transform's precision
+                double2 w[R];
                 if constexpr (SPAN > 1) {
                     w[1] = tw[OFF + k];
                     if constexpr (R > 2) w[2] = cmul(w[1], w[1]);
@@ -294,7 +296,7 @@ __device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     V x = z[zi(b + q * T)];
-                    if (SPAN > 1 && q) x = cmul(x, w[q]);
+                    if (SPAN > 1 && q) x = cmul(x, mk<V>(w[q].x, w[q].y));
                     v[it][q] = x;
                 }
             }
@@ -317,22 +319,21 @@ __device__ __forceinline__ void fft_static(V* z, const V* __restrict__ tw,
 }
 
 // Twiddle source of a block: for L == 0 (runtime plan) the quarter-wave
-// cosine table, copied to LDS; for a static L the stage twiddles, copied to
-// LDS when TW_LDS, else read from global memory.
+// cosine table, copied to LDS; for a static L the stage twiddle roots (FP64
+// in both precisions), copied to LDS when TW_LDS, else read from global
+// memory.
 template <typename S>
 struct Tw {
     const S* ct;
-    const typename Vec<S>::T* tw;
+    const double2* tw;
 };
 
 template <typename S> struct TwSrc;
 template <> struct TwSrc<double> {
     static __device__ const double* ct(const SolverGeom& g) { return g.costab; }
-    static __device__ const double2* tw(const SolverGeom& g) { return g.stage_tw; }
 };
 template <> struct TwSrc<float> {
     static __device__ const float* ct(const SolverGeom& g) { return g.costab32; }
-    static __device__ const float2* tw(const SolverGeom& g) { return g.stage_tw32; }
 };
 
 template <int L, bool TW_LDS, typename S>
@@ -350,10 +351,11 @@ __device__ __forceinline__ Tw<S> load_twiddles(typename Vec<S>::T* lds,
         // most three per thread, all loaded before the LDS stores
         constexpr int n = tw_count(L);
         static_assert(n <= 3 * (L / 4), "twiddle table larger than three per thread");
-        V* d = lds + (size_t)g.P * padded_len(L);
-        const V* src = TwSrc<S>::tw(g);
+        // (the data end is 16-byte aligned: an even number of points)
+        double2* d = reinterpret_cast<double2*>(lds + (size_t)g.P * padded_len(L));
+        const double2* src = g.stage_tw;
         const int nt = blockDim.x;
-        V v[3];
+        double2 v[3];
 #pragma unroll
         for (int u = 0; u < 3; ++u) v[u] = src[min((int)threadIdx.x + u * nt, n - 1)];
 #pragma unroll
@@ -363,7 +365,7 @@ __device__ __forceinline__ Tw<S> load_twiddles(typename Vec<S>::T* lds,
         }
         r.tw = d;
     } else {
-        r.tw = TwSrc<S>::tw(g);
+        r.tw = g.stage_tw;
     }
     return r;
 }
@@ -920,7 +922,7 @@ int solver_lds_bytes(const SolverGeom& g) {
     const size_t cplx = g.fp32 ? sizeof(float2) : sizeof(double2);
     const size_t data = (size_t)g.P * padded_len(g.fft.L) * cplx;
     if (g.static_L == 0) return (int)(data + ((size_t)g.fft.L / 4 + 1) * cplx / 2);
-    if (g.tw_lds) return (int)(data + (size_t)tw_count(g.fft.L) * cplx);
+    if (g.tw_lds) return (int)(data + (size_t)tw_count(g.fft.L) * sizeof(double2));
     return (int)data;
 }
 
