@@ -478,6 +478,19 @@ __device__ __forceinline__ BlockPos xcd_block() {
     return BlockPos{logical / gx, logical % gx};
 }
 
+// Pass A's order: the fields of one line block get consecutive logical
+// ids, so one XCD assembles all six right-hand sides of its rows and reads
+// their shared sources (densities, currents, beam) into its L2 once.
+#ifndef LCODE_PASS_A_FIELDS_INTERLEAVED
+#define LCODE_PASS_A_FIELDS_INTERLEAVED 1
+#endif
+__device__ __forceinline__ BlockPos xcd_block_rows() {
+    if (!LCODE_PASS_A_FIELDS_INTERLEAVED) return xcd_block();
+    const int gx = gridDim.x, nf = gridDim.y, total = gx * nf;
+    const int logical = xcd_logical(blockIdx.y * gx + blockIdx.x, total);
+    return BlockPos{logical % nf, logical / nf};
+}
+
 // ---------------------------------------------------------------------------
 // Passes
 // ---------------------------------------------------------------------------
@@ -489,7 +502,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     V* lds = reinterpret_cast<V*>(lds_raw);
     if (!gate_open(b.gate)) return;
-    const BlockPos bp = xcd_block();
+    const BlockPos bp = xcd_block_rows();
     const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L, N = g.N;
     const int lines = fp.rowsA;
