@@ -52,6 +52,17 @@ estimate_kernel(EstimateArgs a) {
 #define LCODE_PUSH_BLOCK 64
 #endif
 constexpr int kPushBlock = LCODE_PUSH_BLOCK;
+// A launch has about one wave per SIMD, so occupancy buys nothing: with a
+// low waves-per-EU target the compiler issues the 54 field gathers of a
+// particle together instead of one round trip at a time.
+#ifndef LCODE_PUSH_WAVES
+#define LCODE_PUSH_WAVES 2
+#endif
+#if LCODE_PUSH_WAVES > 0
+#define LCODE_PUSH_ATTR __attribute__((amdgpu_waves_per_eu(1, LCODE_PUSH_WAVES)))
+#else
+#define LCODE_PUSH_ATTR
+#endif
 
 // One coarse particle's push; returns max(|x_offt|, |y_offt|) after it.
 __device__ __forceinline__ double push_one(const PushArgs& a, int k) {
@@ -114,7 +125,7 @@ __device__ __forceinline__ double push_one(const PushArgs& a, int k) {
     return fmax(fabs(x_offt), fabs(y_offt));
 }
 
-__global__ void __launch_bounds__(kPushBlock)
+__global__ void __launch_bounds__(kPushBlock) LCODE_PUSH_ATTR
 push_kernel(PushArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.;
