@@ -605,6 +605,18 @@ def main():
                     "avg_us": kernels[dom]["kernel_us"],
                     "avg_us_with_event_pair": kernels[dom]["avg_us"],
                     "event_pair_overhead_us": bracket_us}
+        if "deposit_tile" in kernels:
+            # the reference's deposition rate measure: 4 quantities x 9
+            # stencil cells = 36 atomic adds per fine particle
+            # (lcode.py:434-449, :660-663); the tile kernel sums them in
+            # registers and LDS but carries out the same 36 updates
+            v = kernels["deposit_tile"]
+            updates = 36 * vp.fine_grid.numel() ** 2
+            v["fine_particle_updates_per_launch"] = updates
+            v["fine_particle_updates_per_s"] = updates / (v["kernel_us"] * 1e-6)
+            log(f"  deposit_tile: {updates / 1e6:.1f} M fine-particle stencil updates per launch, "
+                f"{v['fine_particle_updates_per_s'] / 1e12:.2f} T updates/s "
+                f"({vp.fine_grid.numel() ** 2 / (v['kernel_us'] * 1e-6) / 1e9:.1f} G fine particles/s)")
     if args.copy_probe_mib:
         src = torch.ones(args.copy_probe_mib << 17, dtype=torch.float64,
                          device="cuda")
