@@ -277,8 +277,8 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
                   " has a prime factor other than 2, 3, 5");
         return fail(LCODE_E_UNSUPPORTED);
     }
-    g.Tp = ((L + 3) / 4 + 63) / 64 * 64;
     g.static_L = solver_static_length(L) ? L : 0;
+    g.Tp = solver_threads_per_pair(L, ws->fp32, g.static_L != 0);
     const int max_lds = LCODE_SOLVER_MAX_LDS;   // per block, bytes
     auto pairs_for = [&](int fixed_bytes) {
         int P = 1024 / g.Tp;

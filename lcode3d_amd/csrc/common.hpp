@@ -468,6 +468,7 @@ int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
 bool solver_static_length(int L);
 int solver_padded_len(int L);
 int solver_tw_count(int L);   // static plan: twiddle table entries
+int solver_threads_per_pair(int L, bool fp32, bool static_plan);   // threads of one line pair
 int solver_static_radix(int rem);   // static plan: radix of the next stage
 int solver_lds_bytes(const SolverGeom& g);
 

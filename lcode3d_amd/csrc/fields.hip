@@ -23,6 +23,8 @@
 
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace lcode {
 
 namespace {
@@ -256,14 +258,42 @@ __host__ __device__ constexpr int tw_offset(int L, int span) {
 
 __host__ __device__ constexpr int tw_count(int L) { return tw_offset(L, L); }
 
+// Threads of one line pair: enough for the L / 4 butterflies of a radix-4
+// stage (the runtime plans need it: two butterflies per thread at most).
 __host__ __device__ constexpr int threads_per_pair(int L) {
     return ((L + 3) / 4 + 63) / 64 * 64;
 }
 
+// LCODE_FFT_PAIR_WAVES_F32 / _F64 (0: threads_per_pair): waves per line
+// pair of the static plans; each thread then takes up to two butterflies of
+// a stage.  Kept at or above four input points per thread and three
+// twiddle-table entries per thread.  Fewer waves per pair let more pairs
+// share a CU's 32 wave slots (at 641^2, 5 waves per pair leave 966 blocks
+// for 768 slots); measured at 641^2, six-field solve: FP64 92.6 us with
+// 5 waves, 84.2 with 4; FP32 83.6 with 5, 72.0 with 4, 70.9 with 3.
+#ifndef LCODE_FFT_PAIR_WAVES_F32
+#define LCODE_FFT_PAIR_WAVES_F32 3
+#endif
+#ifndef LCODE_FFT_PAIR_WAVES_F64
+#define LCODE_FFT_PAIR_WAVES_F64 4
+#endif
+__host__ __device__ constexpr int static_threads_per_pair(int L, bool fp32) {
+    const int d = threads_per_pair(L);
+    const int w = fp32 ? LCODE_FFT_PAIR_WAVES_F32 : LCODE_FFT_PAIR_WAVES_F64;
+    if (w <= 0) return d;
+    const int lo = ((L / 2 + 1 + 3) / 4 + 63) / 64 * 64;   // <= 4 input points
+    int t = 64 * w;
+    if (t < lo) t = lo;
+    if (t > d) t = d;
+    if (2 * t < (L + 3) / 4 || 3 * t < tw_count(L)) t = d;
+    return t;
+}
+
 // Input points per thread of a pair: a transform input has at most
 // L / 2 + 1 points.
-__host__ __device__ constexpr int points_per_thread(int L) {
-    return (L / 2 + 1 + threads_per_pair(L) - 1) / threads_per_pair(L);
+__host__ __device__ constexpr int points_per_thread(int L, bool fp32) {
+    return (L / 2 + 1 + static_threads_per_pair(L, fp32) - 1) /
+           static_threads_per_pair(L, fp32);
 }
 
 template <int L, int SPAN, typename V>
@@ -272,7 +302,7 @@ __device__ __forceinline__ void fft_static(V* z, const double2* __restrict__ tw,
     if constexpr (SPAN < L) {
         constexpr int R = next_radix(L / SPAN);
         constexpr int T = L / R;
-        constexpr int Tp = threads_per_pair(L);
+        constexpr int Tp = static_threads_per_pair(L, sizeof(V) == sizeof(float2));
         constexpr int ITER = (T + Tp - 1) / Tp;
         constexpr int OFF = tw_offset(L, SPAN);
         V v[ITER][R];
@@ -347,10 +377,11 @@ __device__ __forceinline__ Tw<S> load_twiddles(typename Vec<S>::T* lds,
         for (int k = threadIdx.x; k <= (g.fft.L >> 2); k += blockDim.x) ct[k] = src[k];
         r.ct = ct;
     } else if constexpr (TW_LDS) {
-        // tw_count(L) <= 2 L / 3 entries over P * Tp >= L / 4 threads: at
-        // most three per thread, all loaded before the LDS stores
+        // at most three entries per thread (static_threads_per_pair keeps
+        // P * Tp >= tw_count / 3), all loaded before the LDS stores
         constexpr int n = tw_count(L);
-        static_assert(n <= 3 * (L / 4), "twiddle table larger than three per thread");
+        static_assert(n <= 3 * static_threads_per_pair(L, std::is_same<S, float>::value),
+                      "twiddle table larger than three per thread");
         // (the data end is 16-byte aligned: an even number of points)
         double2* d = reinterpret_cast<double2*>(lds + (size_t)g.P * padded_len(L));
         const double2* src = g.stage_tw;
@@ -514,7 +545,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     const bool va = ra < lines, vb = rb < lines;
     if constexpr (L_ != 0) {
         // rows without data read row rowsA0 (in bounds) and are dropped
-        constexpr int kPer = points_per_thread(L_);
+        constexpr int kPer = points_per_thread(L_, std::is_same<S, float>::value);
         double a[kPer], c[kPer];
         rhs_rows<kPer>(fp.kind, fp.rowsA0 + (va ? ra : 0), fp.rowsA0 + (vb ? rb : 0),
                        fp.colA0, fp.lenA, t, Tp, N, b.in, a, c);
@@ -567,7 +598,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     V* lds = reinterpret_cast<V*>(lds_raw);
     __shared__ double red[2][16];
     if (!gate_open(b.gate)) return;
-    constexpr int kPts = L_ != 0 ? points_per_thread(L_) : kMaxPerThread;
+    constexpr int kPts = L_ != 0 ? points_per_thread(L_, std::is_same<S, float>::value) : kMaxPerThread;
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
     const int Tp = g.Tp, L = g.fft.L;
@@ -584,7 +615,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     const S* tabb = static_cast<const S*>(fp.table) + (size_t)lb * len;
 
     if constexpr (L_ != 0) {
-        constexpr int kPer = points_per_thread(L_);
+        constexpr int kPer = points_per_thread(L_, std::is_same<S, float>::value);
         const S* T = static_cast<const S*>(fp.T);
         const size_t oa = (size_t)(va ? la : 0) * len, ob = (size_t)(vb ? lb : 0) * len;
         S xa[kPer], xb[kPer];
@@ -720,7 +751,7 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     if constexpr (L_ != 0) {
         // every column load of the thread first (addresses clamped in
         // bounds, unused values dropped), then the LDS stores
-        constexpr int kPer = points_per_thread(L_);
+        constexpr int kPer = points_per_thread(L_, std::is_same<S, float>::value);
         const int cac = ha ? ca : 0, cbc = hb ? cb : 0;
         S xa[kPer], xb[kPer];
 #pragma unroll
@@ -792,7 +823,7 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     if constexpr (L_ != 0) {
         // the prev and guess loads of every point of the thread first
         // (rows and columns clamped in bounds), then the stores
-        constexpr int kPer = points_per_thread(L_);
+        constexpr int kPer = points_per_thread(L_, std::is_same<S, float>::value);
         const size_t oa = (size_t)min(ra, N - 1) * N, ob = (size_t)min(rb, N - 1) * N;
         double pa[kPer], pb[kPer], qa[kPer], qb[kPer];
 #pragma unroll
@@ -942,6 +973,10 @@ int solver_lds_bytes(const SolverGeom& g) {
 int solver_padded_len(int L) { return padded_len(L); }
 
 int solver_tw_count(int L) { return tw_count(L); }
+
+int solver_threads_per_pair(int L, bool fp32, bool static_plan) {
+    return static_plan ? static_threads_per_pair(L, fp32) : threads_per_pair(L);
+}
 
 int solver_static_radix(int rem) { return next_radix(rem); }
 
