@@ -526,8 +526,27 @@ __device__ __forceinline__ BlockPos xcd_block_rows() {
 // Passes
 // ---------------------------------------------------------------------------
 
+// LCODE_PASS_A_WAVES, LCODE_PASS_C_WAVES (0: the compiler's choice): waves
+// per SIMD the register allocation of passes A and C must allow.
+#ifndef LCODE_PASS_A_WAVES
+#define LCODE_PASS_A_WAVES 0
+#endif
+#ifndef LCODE_PASS_C_WAVES
+#define LCODE_PASS_C_WAVES 0
+#endif
+#if LCODE_PASS_A_WAVES > 0
+#define LCODE_PASS_A_ATTR __attribute__((amdgpu_waves_per_eu(LCODE_PASS_A_WAVES)))
+#else
+#define LCODE_PASS_A_ATTR
+#endif
+#if LCODE_PASS_C_WAVES > 0
+#define LCODE_PASS_C_ATTR __attribute__((amdgpu_waves_per_eu(LCODE_PASS_C_WAVES)))
+#else
+#define LCODE_PASS_C_ATTR
+#endif
+
 template <int L_, bool TW_LDS, typename S>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) LCODE_PASS_A_ATTR
 pass_a_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -585,9 +604,10 @@ constexpr int kMaxPerThread = 4;
 #endif
 
 // LCODE_PASS_B_WAVES: waves per SIMD the register allocation must allow
-// (8: at most 64 VGPRs, three 640-thread blocks per CU).
+// (6: at most 80 VGPRs; at 641^2 a CU holds three 8-wave FP64 blocks or two
+// 12-wave FP32 blocks, and at 8 the FP64 pass spills).
 #ifndef LCODE_PASS_B_WAVES
-#define LCODE_PASS_B_WAVES 8
+#define LCODE_PASS_B_WAVES 6
 #endif
 template <int L_, bool TW_LDS, typename S>
 __global__ void __launch_bounds__(1024)
@@ -722,7 +742,7 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
 }
 
 template <int L_, bool TW_LDS, typename S>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) LCODE_PASS_C_ATTR
 pass_c_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
     extern __shared__ __align__(16) unsigned char lds_raw[];
