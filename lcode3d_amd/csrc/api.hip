@@ -70,6 +70,22 @@ namespace {
 #ifndef LCODE_TW_LDS_OK
 #define LCODE_TW_LDS_OK 1
 #endif
+// Line pairs per FFT block (at most).  One pair per block measured best at
+// 641^2 (six-field solve FP64 79.6 -> 74.2 us, FP32 66.1 -> 61.3 us against
+// up to four): the same pairs per CU, a quarter or half the waves behind
+// each barrier, and a finer-grained last round of blocks.
+#ifndef LCODE_FFT_MAX_PAIRS_F64
+#define LCODE_FFT_MAX_PAIRS_F64 1
+#endif
+#ifndef LCODE_FFT_MAX_PAIRS_F32
+#define LCODE_FFT_MAX_PAIRS_F32 1
+#endif
+// Static-plan twiddle roots go to LDS when the block has at least this many
+// line pairs (else they are read through the caches).  With one pair per
+// block, LDS roots measured 74.0 -> 72.4 us (FP64) and 62.3 -> 60.2 us (FP32).
+#ifndef LCODE_TW_LDS_MIN_PAIRS
+#define LCODE_TW_LDS_MIN_PAIRS 1
+#endif
 #ifndef LCODE_SOLVER_MAX_LDS
 #define LCODE_SOLVER_MAX_LDS 65536
 #endif
@@ -280,17 +296,20 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     g.static_L = solver_static_length(L) ? L : 0;
     g.Tp = solver_threads_per_pair(L, ws->fp32, g.static_L != 0);
     const int max_lds = LCODE_SOLVER_MAX_LDS;   // per block, bytes
+    // line pairs per block: as many as 1024 threads and the LDS limit allow,
+    // at most LCODE_FFT_MAX_PAIRS_F64 / _F32
+    const int max_pairs = ws->fp32 ? LCODE_FFT_MAX_PAIRS_F32 : LCODE_FFT_MAX_PAIRS_F64;
     auto pairs_for = [&](int fixed_bytes) {
         int P = 1024 / g.Tp;
         const int by_lds = (max_lds - fixed_bytes) / (solver_padded_len(L) * cplx);
         if (by_lds < P) P = by_lds;
-        return P > 4 ? 4 : P;
+        return P > max_pairs ? max_pairs : P;
     };
     int P;
     if (g.static_L) {
-        // stage twiddles in LDS when that still leaves two line pairs
+        // stage twiddles in LDS when the block keeps LCODE_TW_LDS_MIN_PAIRS pairs
         P = pairs_for(solver_tw_count(L) * (int)sizeof(double2));   // FP64 roots
-        g.tw_lds = LCODE_TW_LDS_OK && P >= 2;
+        g.tw_lds = LCODE_TW_LDS_OK && P >= LCODE_TW_LDS_MIN_PAIRS;
         if (!g.tw_lds) P = pairs_for(0);
     } else {
         g.tw_lds = 0;
