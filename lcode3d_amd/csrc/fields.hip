@@ -270,7 +270,9 @@ __host__ __device__ constexpr int threads_per_pair(int L) {
 // twiddle-table entries per thread.  Fewer waves per pair let more pairs
 // share a CU's 32 wave slots (at 641^2, 5 waves per pair leave 966 blocks
 // for 768 slots); measured at 641^2, six-field solve: FP64 92.6 us with
-// 5 waves, 84.2 with 4; FP32 83.6 with 5, 72.0 with 4, 70.9 with 3.
+// 5 waves, 84.2 with 4; FP32 83.6 with 5, 72.0 with 4, 70.9 with 3 (two
+// pairs per block); with one pair per block FP64 76.8 / 71.9 / 77.7 us with
+// 5 / 4 / 3 waves, FP32 60.6 / 59.3 us with 4 / 3.
 #ifndef LCODE_FFT_PAIR_WAVES_F32
 #define LCODE_FFT_PAIR_WAVES_F32 3
 #endif
