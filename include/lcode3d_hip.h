@@ -49,7 +49,8 @@ enum {
     LCODE_OK = 0,
     LCODE_E_INVALID = -1,   /* bad argument or config */
     LCODE_E_HIP = -2,       /* a HIP runtime call failed */
-    LCODE_E_UNSUPPORTED = -3 /* e.g. a transform length with a prime > 7 */
+    LCODE_E_UNSUPPORTED = -3 /* e.g. a transform length 2 (N - 1) with a
+                                prime factor other than 2, 3 and 5 */
 };
 
 /* Transform back-end of the field solves. */
@@ -121,6 +122,18 @@ int lcode_workspace_destroy(lcode_workspace *ws);
  * (docs/overview/window_and_grids.rst).  Synchronises `stream`. */
 int lcode_workspace_fault_count(lcode_workspace *ws, hipStream_t stream,
                                 int64_t *count);
+
+/* Whether a particle with a non-finite (or more than 1e6 cells) offset or
+ * momentum was met since the flag was last reset: 1 or 0 in *flag.  Does
+ * not synchronise: it reads a host-visible flag that the pushes and the
+ * deposition set, so a kernel still running has not reported yet (after a
+ * stream synchronisation every finished kernel has).  When the flag is set
+ * the depositions that met it wrote NaN grids instead of depositing, and
+ * everything computed from them is NaN.  reset != 0 clears it.  The
+ * reference has no such check: a NaN particle there deposits at an
+ * undefined cell or faults. */
+int lcode_workspace_nonfinite(lcode_workspace *ws, int32_t reset,
+                              int32_t *flag);
 
 /* Per-kernel timing (instrumentation; no counterpart in the reference).
  * Between begin and end every kernel the library launches is bracketed by

@@ -99,6 +99,9 @@ _SIGNATURES = {
                            ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "lcode_workspace_fault_count": (ctypes.c_int,
                                     [_vp, _vp, ctypes.POINTER(ctypes.c_int64)]),
+    "lcode_workspace_nonfinite": (ctypes.c_int,
+                                  [_vp, ctypes.c_int32,
+                                   ctypes.POINTER(ctypes.c_int32)]),
     "lcode_move_estimate_wo_fields": (ctypes.c_int,
                                       [_vp, ctypes.POINTER(LcodeConfig)]
                                       + [_vp] * 10 + [_vp]),

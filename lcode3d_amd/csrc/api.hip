@@ -50,10 +50,13 @@ struct lcode_workspace {
     double* favg[6] = {};          // field averages of the current pass
     int* fault = nullptr;
     int* spill = nullptr;          // nf * nf spilled fine particle indices
-    int* spill_ctr = nullptr;      // [0] count, [1] blocks done, [2], [3]
-                                   // the pushes' displacement bound in
-                                   // cells, two slots used in turn
+    int* spill_ctr = nullptr;      // [0], [1] spill counts, [2], [3] the
+                                   // pushes' displacement bound in cells,
+                                   // each pair of slots used in turn
     int bound_slot = 0;            // slot the next push raises
+    int spill_slot = 0;            // count slot of the next deposition
+    int* host_flag = nullptr;      // pinned host int: a non-finite particle
+                                   // was met (written by the kernels)
     double* guess[6] = {};         // guided step: the field guess
     double* resid = nullptr;       // {sum (F - guess)^2, sum F^2}
     double* resid_part = nullptr;  // 2 per block of the last FFT pass
@@ -496,6 +499,12 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     const size_t nfp = (size_t)cfg->nf * cfg->nf;
     if ((rc = dev_alloc(ws, (void**)&ws->spill, (nfp ? nfp : 1) * sizeof(int)))) return fail(rc);
     if ((rc = dev_alloc(ws, (void**)&ws->spill_ctr, 4 * sizeof(int)))) return fail(rc);
+    if (hipHostMalloc((void**)&ws->host_flag, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        ws->host_flag = nullptr;
+        set_error("hipHostMalloc of the non-finite flag failed");
+        return fail(LCODE_E_HIP);
+    }
+    *ws->host_flag = 0;
     if (hipMemset(ws->fault, 0, sizeof(int)) != hipSuccess ||
         hipMemset(ws->spill_ctr, 0, 4 * sizeof(int)) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
@@ -519,6 +528,7 @@ int lcode_workspace_destroy(lcode_workspace* ws) {
     int rc = LCODE_OK;
     for (void* p : ws->allocs)
         if (hipFree(p) != hipSuccess) rc = LCODE_E_HIP;
+    if (ws->host_flag && hipHostFree(ws->host_flag) != hipSuccess) rc = LCODE_E_HIP;
     delete ws;
     if (rc) set_error("hipFree failed in lcode_workspace_destroy");
     return rc;
@@ -582,6 +592,15 @@ int lcode_diag_zn(lcode_workspace* ws, const lcode_config* cfg, const double* ro
                           ws->diag_sum, zn_out, stream, &ws->prof);
 }
 
+int lcode_workspace_nonfinite(lcode_workspace* ws, int32_t reset,
+                              int32_t* flag) {
+    LCODE_REQUIRE(ws && flag, "NULL argument");
+    volatile int* f = ws->host_flag;
+    *flag = *f;
+    if (reset) *f = 0;
+    return LCODE_OK;
+}
+
 int lcode_workspace_fault_count(lcode_workspace* ws, hipStream_t stream,
                                 int64_t* count) {
     LCODE_REQUIRE(ws && count, "NULL argument");
@@ -640,6 +659,7 @@ static int do_push(lcode_workspace* ws, const lcode_config* cfg,
     a.out_px = out[2]; a.out_py = out[3]; a.out_pz = out[4];
     a.fault = ws->fault;
     a.max_cells = ws->spill_ctr + 2 + ws->bound_slot;
+    a.host_flag = ws->host_flag;
     a.gate = gate;
     return launch_push(a, stream, &ws->prof);
 }
@@ -687,8 +707,9 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.ro = out[0]; a.jx = out[1]; a.jy = out[2]; a.jz = out[3];
     a.fault = ws->fault;
     a.spill = ws->spill;
-    a.spill_count = ws->spill_ctr;
-    a.spill_done = ws->spill_ctr + 1;
+    a.host_flag = ws->host_flag;
+    a.spill_count = ws->spill_ctr + ws->spill_slot;
+    a.spill_next = ws->spill_ctr + (1 - ws->spill_slot);
     a.halo = kDepositHalo;
     // after a push of this workspace: a halo of the pushes' displacement
     // bound, read from the slot the pushes raised; the deposition zeroes
@@ -703,6 +724,7 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.gate = gate;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
     const int rc = launch_deposit(a, stream, &ws->prof);
+    if (rc == LCODE_OK) ws->spill_slot = 1 - ws->spill_slot;
     if (rc == LCODE_OK && adaptive_halo) ws->bound_slot = 1 - ws->bound_slot;
     return rc;
 }

@@ -207,8 +207,16 @@ struct PushArgs {
     double *out_x, *out_y, *out_px, *out_py, *out_pz;
     int* fault;
     int* max_cells;                // atomic max of ceil(|offset| / h), or NULL
+    // non-finite (or beyond kFarCells) particle state: flagged in
+    // host-visible memory (NULL: not reported)
+    int* host_flag;
     Gate gate;
 };
+
+// Displacement bound (in cells) that marks a non-finite or absurd particle
+// state; the deposition fills its tiles with NaN instead of depositing.
+constexpr int kNonFinite = 1 << 30;
+constexpr double kFarCells = 1e6;
 
 struct EstimateArgs {
     int n;
@@ -227,11 +235,14 @@ struct DepositArgs {
     const double* ro_initial;      // may be NULL
     double *ro, *jx, *jy, *jz;
     int* fault;
+    int* host_flag;                // host-visible flag of a non-finite
+                                   // particle met (or NULL)
     // particles displaced by more than the tile halo: appended to spill[]
-    // by their home tile, deposited with atomics by a second kernel
+    // by their home tile, deposited with atomics by a second, grid-wide
+    // kernel; the two count slots are used in turn
     int* spill;                    // nf * nf capacity
-    int* spill_count;              // zero between calls
-    int* spill_done;               // zero between calls
+    int* spill_count;              // this call's count (zero on entry)
+    int* spill_next;               // the next call's count: zeroed here
     int halo;                      // tile halo in cells (maximum)
     int* halo_cells;               // NULL, or the pushes' displacement bound
     int* halo_next;                // NULL, or the bound slot of the next

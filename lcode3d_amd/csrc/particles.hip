@@ -8,6 +8,8 @@ namespace lcode {
 namespace {
 
 constexpr int kBlock = 256;
+// Blocks of the grid-wide spill deposition (one per CU).
+constexpr int kSpillBlocks = 256;
 
 __device__ __forceinline__ int clampi(int v, int n) {
     return v < 0 ? 0 : (v >= n ? n - 1 : v);
@@ -122,7 +124,10 @@ __device__ __forceinline__ double push_one(const PushArgs& a, int k) {
     a.out_px[k] = px;
     a.out_py[k] = py;
     a.out_pz[k] = pz;
-    return fmax(fabs(x_offt), fabs(y_offt));
+    // infinity marks a non-finite particle (fmax alone would drop a NaN)
+    const bool finite = isfinite(x_offt) && isfinite(y_offt) && isfinite(px) &&
+                        isfinite(py) && isfinite(pz);
+    return finite ? fmax(fabs(x_offt), fabs(y_offt)) : __builtin_inf();
 }
 
 __global__ void __launch_bounds__(kPushBlock) LCODE_PUSH_ATTR
@@ -132,16 +137,24 @@ push_kernel(PushArgs a) {
     if (k < a.n && gate_open(a.gate)) d = push_one(a, k);
     if (a.max_cells) {
         // cells the particles moved from their initial positions, at most
-        // (the deposition sizes its tile halo by it); NaN counts as far
+        // (the deposition sizes its tile halo by it); a non-finite state,
+        // or one beyond kFarCells cells, is the kNonFinite mark, on which
+        // the deposition writes NaN instead of depositing
         const double c = ceil(d / a.h);
-        int cells = c < 1e6 ? (int)c : (1 << 20);
+        int cells = c <= kFarCells ? (int)c : kNonFinite;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) cells = max(cells, __shfl_xor(cells, off, 64));
         // one atomic per wave, and only when it raises the bound: after the
         // first waves the bound is reached and the rest only read it
-        if ((threadIdx.x & 63) == 0 &&
-            cells > __hip_atomic_load(a.max_cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(a.max_cells, cells);
+        if ((threadIdx.x & 63) == 0) {
+            if (cells > __hip_atomic_load(a.max_cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(a.max_cells, cells);
+            if (cells == kNonFinite) {
+                if (a.host_flag)
+                    __hip_atomic_store(a.host_flag, 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
     }
 }
 
@@ -219,8 +232,10 @@ __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
 // interpolate from are staged in LDS first; then one thread per fine
 // particle adds its 9 x 4 weighted values to the LDS tile.  A particle
 // displaced by more than a.halo cells from its initial cell is appended to
-// the spill list by its home tile (the tile owning its initial cell); the
-// last block to finish deposits the list with global atomics and resets it.
+// the spill list by its home tile (the tile owning its initial cell), and
+// deposit_spill_kernel, launched next, adds the list with device atomics
+// across the whole grid.  A non-finite particle deposits nothing: the tile
+// (or, when the pushes reported one, every tile) is written as NaN.
 constexpr int kFineMax = kDepFineMax;       // fine indices per side
 constexpr int kCoarseMax = kDepCoarseMax;   // coarse particles per side staged in LDS
 constexpr int kExtMax = 48;      // initial cells per side: kTile + 2 halo + 2
@@ -290,7 +305,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     __shared__ FineTab ti, tj;
     __shared__ double cs[7][kCoarseMax * kCoarseMax];
     __shared__ int range[4];
-    __shared__ int is_last;
+    __shared__ int tile_nan;
     const int N = a.N;
     const double inv_h = 1. / a.h;
     const int ci0 = blockIdx.y * kTile, cj0 = blockIdx.x * kTile;
@@ -303,8 +318,12 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     const int bound = a.halo_cells ? max(*a.halo_cells, 0) : a.halo + 1;
     const int halo = min(bound, a.halo);
     const bool may_spill = bound > a.halo;
+    // the pushes met a non-finite (or absurdly far) particle: the whole
+    // deposition is NaN, with no particle visited (already reported)
+    const bool nonfinite = bound >= kNonFinite;
     if (a.halo_next && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
         *a.halo_next = 0;   // for the next pushes; nothing reads it here
+    if (threadIdx.x == 0) tile_nan = 0;
     const int ext = kTile + 2 * halo + 2;
     const int ci_lo = ci0 - 1 - halo, cj_lo = cj0 - 1 - halo;
     for (int k = threadIdx.x; k < 4 * kTile * kTile; k += blockDim.x)
@@ -325,7 +344,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         // denser fine grid than the host-checked fineness allows: flag the
         // tile as faulted and fill it with NaN rather than drop particles
         if (threadIdx.x == 0) atomicAdd(a.fault, 1);
-    } else {
+    } else if (!nonfinite) {
         for (int k = threadIdx.x; k < ni + nj; k += blockDim.x) {
             const bool isj = k >= ni;
             const int f = isj ? fj_a + k - ni : fi_a + k;
@@ -413,6 +432,17 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             const int ci = ti.cell[li], cj = tj.cell[lj];   // initial cell
             const bool home = ci >= ci0 && ci < ci0 + kTile &&
                               cj >= cj0 && cj < cj0 + kTile;
+            if (!(fabs(x) <= kFarCells * a.h && fabs(y) <= kFarCells * a.h)) {
+                // non-finite or absurdly far particle (a deposition called
+                // without the pushes' bound): its home tile becomes NaN
+                if (home) {
+                    tile_nan = 1;
+                    if (a.host_flag)
+                        __hip_atomic_store(a.host_flag, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+                return false;
+            }
             tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
             if (abs(t.i - ci) > halo || abs(t.j - cj) > halo) {
                 // (beyond a bound the pushes reported: cannot happen; counted
@@ -513,7 +543,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         const int i = ci0 + r, j = cj0 + c;
         if (k >= kTile * kTile || i >= N || j >= N) continue;
         const size_t idx = (size_t)i * N + j;
-        if (overfull) {
+        if (overfull || nonfinite || tile_nan) {
             a.ro[idx] = a.jx[idx] = a.jy[idx] = a.jz[idx] = __builtin_nan("");
             continue;
         }
@@ -522,28 +552,20 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         a.jy[idx] = acc[2][r][c];
         a.jz[idx] = acc[3][r][c];
     }
-    if (!may_spill) return;
-    // the last block to finish deposits the spilled particles: the barrier
-    // orders this block's stores before thread 0's device-scope release
-    // fence, which makes them visible before the block counts itself done
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const int total = gridDim.x * gridDim.y;
-        is_last = atomicAdd(a.spill_done, 1) == total - 1;
-    }
-    __syncthreads();
-    if (!is_last) return;
-    if (threadIdx.x == 0) __threadfence();   // acquire the other blocks' stores
-    __syncthreads();
-    const int count = atomicAdd(a.spill_count, 0);
-    for (int p = threadIdx.x; p < count; p += blockDim.x)
-        deposit_spill_one(a, a.spill[p], N);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicExch(a.spill_count, 0);
-        atomicExch(a.spill_done, 0);
-    }
+}
+
+// Deposits the particles the tile kernel listed as displaced beyond its
+// halo, with device-scope atomics over the whole grid (after the tile
+// kernel's stores, which it follows on the stream), and zeroes the count
+// slot of the next call.  An empty list costs one load per block.
+__global__ void __launch_bounds__(kBlock)
+deposit_spill_kernel(DepositArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.spill_next = 0;
+    if (!gate_open(a.gate)) return;
+    const int count = *a.spill_count;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count;
+         p += gridDim.x * blockDim.x)
+        deposit_spill_one(a, a.spill[p], a.N);
 }
 
 template <bool STAGED>
@@ -580,6 +602,11 @@ int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof) {
             deposit_tile_kernel<true><<<grid, kDepBlock, 0, s>>>(a);
         else
             deposit_tile_kernel<false><<<grid, kDepBlock, 0, s>>>(a);
+    }
+    LCODE_HIP(hipGetLastError());
+    {
+        ProfScope ps(prof, "deposit_spill", s);
+        deposit_spill_kernel<<<kSpillBlocks, kBlock, 0, s>>>(a);
     }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;

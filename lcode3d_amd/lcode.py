@@ -380,10 +380,60 @@ def initial_deposition(config, x_offt, y_offt, px, py, pz, m, q, virt_params):
 FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
 
 
+#: Default acceptance threshold of a guess: the largest corrector residual
+#: ||F_corr - guess|| / ||F_corr|| at which the guess stands in for the
+#: predictor pass.  The reference's one-pass corrector leaves a residual of
+#: about 0.1 dxi even for a perfect guess (3e-5 .. 6e-4 over a 641^2 run), so
+#: a threshold of 1e-5 would reject every guess; 2e-3 accepts guesses on the
+#: smooth slices and keeps Ez(xi, 0, 0) within 1e-5 of the plain step.
+GUIDED_THRESHOLD = 2e-3
+
+
+def _check_step_arrays(config, const, prev):
+    """dtype, layout, shape and device of every array the fused step reads
+    through a raw pointer (before anything is launched)."""
+    nc = prev.x_offt.shape[0]
+    n = int(config.grid_steps)
+    coarse = ("x_offt", "y_offt", "px", "py", "pz")
+    checks = [(f"const.{k}", getattr(const, k), (nc, nc))
+              for k in ("m", "q", "x_init", "y_init")]
+    checks.append(("const.ro_initial", const.ro_initial, (n, n)))
+    checks += [(f"prev.{k}", getattr(prev, k),
+                (nc, nc) if k in coarse else (n, n)) for k in _lib.STATE_FIELDS]
+    for name, t, shape in checks:
+        if (not isinstance(t, torch.Tensor) or t.dtype != _F64
+                or not t.is_contiguous() or tuple(t.shape) != shape):
+            raise ValueError(f"{name} must be a contiguous float64 tensor "
+                             f"of shape {shape}")
+    for name, t, shape in checks:
+        if t.device.type != "cuda":
+            raise ValueError(f"{name} must be a device (HIP) tensor")
+    return nc
+
+
+def check_nonfinite(ws, reset=True):
+    """Raise if a kernel on workspace `ws` has met a non-finite particle
+    state.  Reads a host-visible flag without synchronising, so a step
+    whose kernels are still running reports at a later call."""
+    flag = ctypes.c_int32()
+    _lib.check(_lib.load().lcode_workspace_nonfinite(ws, 1 if reset else 0,
+                                                     ctypes.byref(flag)),
+               "lcode_workspace_nonfinite")
+    if flag.value:
+        raise FloatingPointError(
+            "a particle offset or momentum became non-finite (or moved "
+            "more than 1e6 cells) in an earlier step on this workspace; "
+            "the grids deposited from it are NaN")
+
+
 def step(config, const, virt_params, prev, beam_ro, guess=None,
-         threshold=1e-5):
+         threshold=GUIDED_THRESHOLD):
     """One xi step; returns the new state (lcode.py:828-931).  beam_ro is
     an (N, N) array (host or device) or a scalar.
+
+    Raises FloatingPointError when an earlier step on the same workspace
+    met a non-finite particle state (checked without synchronising, so the
+    error surfaces within a step or two of the cause).
 
     guess (not in the reference): the predictor hook of lcode.py:841-881.
     A sequence of (coefficient, fields) terms, fields being an object with
@@ -391,27 +441,17 @@ def step(config, const, virt_params, prev, beam_ro, guess=None,
     arrays; their linear combination replaces the predictor pass's
     deposition and field solve.  The corrector's residual against the guess
     is checked on the device and the plain predictor runs when it exceeds
-    `threshold` (see guided_residual)."""
+    `threshold` (see guided_residual and GUIDED_THRESHOLD)."""
+    nc = _check_step_arrays(config, const, prev)
     lib = _lib.load()
     beam, beam_scalar = _beam_arg(config, beam_ro)
     vp, nf, keep_vp = _virt_struct(virt_params)
-    nc = prev.x_offt.shape[0]
     cs = _config_struct(config, nc, nf)
+    ws = _workspace(cs)
+    check_nonfinite(ws)
     c = _lib.LcodeConst(*(getattr(const, n).data_ptr()
                           for n in ("m", "q", "x_init", "y_init",
                                     "ro_initial")))
-    for name in ("m", "q", "x_init", "y_init"):
-        t = getattr(const, name)
-        if t.dtype != _F64 or not t.is_contiguous() or t.shape != (nc, nc):
-            raise ValueError(f"const.{name} must be contiguous float64 ({nc}, {nc})")
-    n = int(config.grid_steps)
-    for name in _lib.STATE_FIELDS:
-        t = getattr(prev, name)
-        shape = (nc, nc) if name in ("x_offt", "y_offt", "px", "py", "pz") else (n, n)
-        if t.dtype != _F64 or not t.is_contiguous() or tuple(t.shape) != shape:
-            raise ValueError(f"prev.{name} must be contiguous float64 {shape}")
-    if tuple(const.ro_initial.shape) != (n, n):
-        raise ValueError("const.ro_initial has the wrong shape")
     p = _lib.LcodeState(*(getattr(prev, name).data_ptr()
                           for name in _lib.STATE_FIELDS))
     new = {name: torch.empty_like(getattr(prev, name))
@@ -419,13 +459,13 @@ def step(config, const, virt_params, prev, beam_ro, guess=None,
     o = _lib.LcodeState(*(new[name].data_ptr() for name in _lib.STATE_FIELDS))
     beam_ptr = _ptr(beam) if beam is not None else None
     if guess is None:
-        _lib.check(lib.lcode_step(_workspace(cs), ctypes.byref(cs),
+        _lib.check(lib.lcode_step(ws, ctypes.byref(cs),
                                   ctypes.byref(c), ctypes.byref(vp),
                                   ctypes.byref(p), beam_ptr, beam_scalar,
                                   ctypes.byref(o), _stream()), "lcode_step")
     else:
         g, keep_g = _guess_struct(config, guess)
-        _lib.check(lib.lcode_step_guided(_workspace(cs), ctypes.byref(cs),
+        _lib.check(lib.lcode_step_guided(ws, ctypes.byref(cs),
                                          ctypes.byref(c), ctypes.byref(vp),
                                          ctypes.byref(p), beam_ptr,
                                          beam_scalar, ctypes.byref(g),

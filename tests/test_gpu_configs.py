@@ -1,0 +1,155 @@
+"""Runs of the production workloads, through the C ABI:
+
+* 641^2 plain FP64 (grid_step_size 0.025, xi_step_size 0.005, coarseness 3,
+  fineness 2, cosine-Gaussian beam of amplitude 0.05, sigma 1) over 1500
+  slices against the oracle's fixture ``golden/oracle_641_1500.npz``
+  (written by ``golden/make_641_fixture.py``): Ez(xi, 0, 0) on every slice
+  within 1e-10 of its peak, and the centre row and column of all 15 state
+  arrays every 100 slices within relative L2 1e-10 (Bz against the Bx
+  row's norm).
+* 641^2 with the reduced-precision field solvers (FP32 FFT, dense BF16x3)
+  for beam amplitude {0.01, 0.05, 0.1} x sigma {0.5, 1, 2}, 300 slices
+  each, against the plain FP64 HIP step of the same beam (itself pinned by
+  the first test) every 100 slices.  Bounds (relative L2 over the whole
+  array; Bz against ||Bx||): Ez <= 1e-4 in every mode; FP32 FFT: other
+  fields <= 1e-4, momenta <= 1e-5; BF16x3: Ex Ey Bx By <= 2e-3, Bz <= 1e-4,
+  momenta <= 1e-4.  The measured worst values are printed.
+
+The beam is the oracle's expression evaluated the same way: the Gaussian
+profile amplitude * exp(-r^2 / (2 sigma^2)) computed on the host once, then
+multiplied on the device by the slice's scalar (1 - cos(xi sqrt(pi/2))),
+one exactly rounded product, as in config_example.py:21-28.
+"""
+
+import math
+import os
+import types
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_helpers import STATE, dev, host, rel_l2, say
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "oracle_641_1500.npz")
+
+
+@pytest.fixture(scope="module")
+def lc():
+    import lcode3d_amd
+    assert torch.cuda.is_available(), "no HIP device visible"
+    return lcode3d_amd
+
+
+def _cfg641(**kw):
+    base = dict(grid_steps=641, grid_step_size=.025, xi_step_size=.005,
+                field_solver_subtraction_trick=1, field_solver_variant_A=True,
+                reflect_padding_steps=5, plasma_padding_steps=10,
+                plasma_coarseness=3, plasma_fineness=2)
+    base.update(kw)
+    return types.SimpleNamespace(**base)
+
+
+def beam_source(cfg, amplitude, sigma, shift_x=0.0, shift_y=0.0):
+    """xi_i -> the slice's beam (device array, or the scalar 0 behind the
+    bunch), bit for bit the oracle's beam()."""
+    n, h = cfg.grid_steps, cfg.grid_step_size
+    grid = (np.arange(n) - n // 2) * h
+    xs, ys = grid[:, None], grid[None, :]
+    r = np.sqrt((xs - shift_x) ** 2 + (ys - shift_y) ** 2)
+    prof = dev(amplitude * np.exp(-.5 * (r / sigma) ** 2))
+
+    def beam(xi_i):
+        xi = -xi_i * cfg.xi_step_size
+        if xi < -2 * math.sqrt(2 * math.pi):
+            return 0
+        return prof * (1 - math.cos(xi * math.sqrt(math.pi / 2)))
+    return beam
+
+
+@pytest.mark.skipif(not os.path.exists(FIXTURE),
+                    reason="golden/oracle_641_1500.npz not generated")
+def test_plain_641_1500_slices_against_oracle_fixture(lc):
+    fx = np.load(FIXTURE)
+    cfg = _cfg641()
+    _, _, const, vp, st = lc.init(cfg)
+    beam = beam_source(cfg, .05, 1.0)
+    c = cfg.grid_steps // 2
+    checkpoints = list(fx["checkpoints"])
+    ez = torch.zeros(1500, dtype=torch.float64, device="cuda")
+    worst = {}
+    for xi_i in range(1500):
+        st = lc.step(cfg, const, vp, st, beam(xi_i))
+        ez[xi_i] = st.Ez[c, c]
+        if xi_i + 1 in checkpoints:
+            k = checkpoints.index(xi_i + 1)
+            say(f"641^2 plain, slice {xi_i + 1}")
+            for name in STATE:
+                a = host(getattr(st, name))
+                m = a.shape[0] // 2
+                for part, got in (("row", a[m, :]), ("col", a[:, m])):
+                    want = fx[f"{name}_{part}"][k]
+                    floor = (np.linalg.norm(fx[f"Bx_{part}"][k])
+                             if name == "Bz" else 0.0)
+                    err = rel_l2(got, want, floor)
+                    worst[name] = max(worst.get(name, 0.0), err)
+    ez_axis = host(ez)
+    ez_err = np.abs(ez_axis - fx["ez_axis"]).max() / np.abs(fx["ez_axis"]).max()
+    say("641^2 x 1500 worst rel L2 of the centre rows/columns:",
+        {k: f"{v:.1e}" for k, v in worst.items()}, f"Ez(xi,0,0) {ez_err:.1e}")
+    assert ez_err < 1e-10
+    for name, err in worst.items():
+        assert err < 1e-10, (name, err)
+    assert lc.fault_count(cfg, const.m.shape[0]) == 0
+
+
+BOUNDS = {
+    ("fft", "fp32"): dict(Ez=1e-4, fields=1e-4, Bz=1e-4, momenta=1e-5),
+    ("dense", "bf16x3"): dict(Ez=1e-4, fields=2e-3, Bz=1e-4, momenta=1e-4),
+}
+
+
+def _run(lc, cfg, beam, slices, every, on_checkpoint):
+    _, _, const, vp, st = lc.init(cfg)
+    for xi_i in range(slices):
+        st = lc.step(cfg, const, vp, st, beam(xi_i))
+        if (xi_i + 1) % every == 0:
+            on_checkpoint(xi_i + 1, st)
+    return const
+
+
+@pytest.mark.parametrize("transform,precision", sorted(BOUNDS))
+def test_reduced_precision_641_beam_grid(lc, transform, precision):
+    bounds = BOUNDS[(transform, precision)]
+    slices, every = 300, 100
+    worst = {}
+    for amp in (.01, .05, .1):
+        for sigma in (.5, 1.0, 2.0):
+            say(f"641^2 {transform}/{precision}, amplitude {amp}, sigma {sigma}")
+            ref = {}
+            cfg64 = _cfg641()
+            beam = beam_source(cfg64, amp, sigma)
+            _run(lc, cfg64, beam, slices, every,
+                 lambda k, s: ref.__setitem__(k, {n: host(getattr(s, n)) for n in STATE}))
+            cfg = _cfg641(field_solver_transform=transform,
+                          field_solver_precision=precision)
+
+            def compare(k, s):
+                r = ref[k]
+                for name in ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "px", "py", "pz"):
+                    floor = np.linalg.norm(r["Bx"]) if name == "Bz" else 0.0
+                    err = rel_l2(host(getattr(s, name)), r[name], floor)
+                    worst[name] = max(worst.get(name, 0.0), err)
+            const = _run(lc, cfg, beam, slices, every, compare)
+            assert lc.fault_count(cfg, const.m.shape[0]) == 0
+    say(f"641^2 {transform}/{precision} worst rel L2 over 9 beams x {slices} slices:",
+        {k: f"{v:.1e}" for k, v in worst.items()})
+    assert worst["Ez"] < bounds["Ez"]
+    for name in ("Ex", "Ey", "Bx", "By"):
+        assert worst[name] < bounds["fields"], (name, worst[name])
+    assert worst["Bz"] < bounds["Bz"]
+    for name in ("px", "py", "pz"):
+        assert worst[name] < bounds["momenta"], (name, worst[name])

@@ -174,3 +174,56 @@ def test_step_particles_beyond_the_deposition_halo(lc, oracle):
     say(f"  max offset after two steps {disp:.2f} cells")
     assert disp > 6.5
     assert lc.fault_count(cfg_g, const.m.shape[0]) == 0
+
+
+def test_step_641_particles_ten_cells_out(lc, oracle):
+    """641^2: offsets up to 10 cells, beyond the 6-cell largest tile halo,
+    so every deposition of the step lists the far particles and the
+    grid-wide spill kernel deposits them; one step against the oracle,
+    relative L2 <= 1e-11."""
+    say("641^2 one step, offsets up to 10 cells (spill kernel)")
+    kw = dict(grid_steps=641, grid_step_size=.025, xi_step_size=.005,
+              plasma_coarseness=3, plasma_fineness=2)
+    cfg_o, cfg_g = _cfg(**kw), _cfg(**kw)
+    xs, ys, const, vp, st = perturbed_state(oracle, cfg_o, seed=17,
+                                            amp_x=10 * cfg_o.grid_step_size)
+    lc.init(cfg_g)
+    gconst = lc.GPUArrays(**{k: dev(getattr(const, k)) for k in (
+        "m", "q", "x_init", "y_init", "ro_initial")})
+    gvp = lc.GPUArrays(**{k: getattr(vp, k) for k in (
+        "influence_prev", "influence_next", "indices_prev", "indices_next",
+        "fine_grid")})
+    beam = .05 * np.exp(-.5 * (xs**2 + ys**2))
+    ref = oracle.step(cfg_o, const, vp, st, beam)
+    got = lc.step(cfg_g, gconst, gvp, to_device_state(lc, st), dev(beam))
+    torch.cuda.synchronize()
+    errs = {name: rel_l2(host(getattr(got, name)), getattr(ref, name),
+                         _floor(name, ref)) for name in STATE}
+    say("  rel L2:", {k: f"{v:.1e}" for k, v in errs.items()})
+    for name, err in errs.items():
+        assert err < 1e-11, (name, err)
+    disp = max(np.abs(st.x_offt).max(), np.abs(st.y_offt).max()) / cfg_o.grid_step_size
+    assert disp > 9
+    assert lc.fault_count(cfg_g, const.m.shape[0]) == 0
+
+
+def test_step_nonfinite_particle_is_reported(lc, oracle):
+    """A NaN momentum makes the step's depositions write NaN grids (no
+    particle is deposited, so nothing grinds on a NaN cell index) and the
+    next step() call raises FloatingPointError; a reset workspace then
+    steps normally again."""
+    say("129^2 NaN momentum")
+    cfg = _cfg()
+    xs, ys, gconst, gvp, gst = lc.init(cfg)
+    bad = lc.GPUArrays(**{k: getattr(gst, k).clone() for k in STATE})
+    bad.px[3, 5] = float("nan")
+    out = lc.step(cfg, gconst, gvp, bad, 0)
+    torch.cuda.synchronize()
+    assert torch.isnan(out.ro).all() and torch.isnan(out.jz).all()
+    with pytest.raises(FloatingPointError):
+        lc.step(cfg, gconst, gvp, gst, 0)
+    # the flag was reset by the raise: a clean state steps normally
+    ok = lc.step(cfg, gconst, gvp, gst, 0)
+    torch.cuda.synchronize()
+    assert torch.isfinite(ok.ro).all()
+    lc.step(cfg, gconst, gvp, ok, 0)
