@@ -1,12 +1,16 @@
 """Builds liblcode3d_hip.so in-tree with hipcc for gfx950.
 
 Usage: ``python -m lcode3d_amd.build`` (or ``build()`` from Python).
+Each source is compiled to an object in parallel (one code object per
+source: every kernel is launched from the file that defines it), then the
+objects are linked into the shared library.
 """
 
 import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -15,7 +19,7 @@ OUTPUT = os.path.join(HERE, "liblcode3d_hip.so")
 ARCH = "gfx950"
 
 FLAGS = [
-    "-O3", "-std=c++17", "-shared", "-fPIC",
+    "-O3", "-std=c++17", "-fPIC",
     f"--offload-arch={ARCH}",
     # packed FP32 instructions stay off in all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -30,18 +34,36 @@ def hipcc():
     return exe
 
 
+def _run(cmds, verbose):
+    procs = []
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                      stderr=subprocess.STDOUT, text=True))
+    failed = False
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed")
+
+
 def build(verbose=False):
     """Compile every source into OUTPUT (always recompiles)."""
-    tmp = OUTPUT + ".tmp"
-    cmd = [hipcc(), *FLAGS, *(os.path.join(CSRC, s) for s in SOURCES),
-           "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode})")
-    os.replace(tmp, OUTPUT)
+    with tempfile.TemporaryDirectory(prefix="lcode3d_build_") as tmpdir:
+        objs, cmds = [], []
+        for s in SOURCES:
+            obj = os.path.join(tmpdir, s.replace(".hip", ".o"))
+            objs.append(obj)
+            cmds.append([hipcc(), *FLAGS, "-c", os.path.join(CSRC, s), "-o", obj])
+        _run(cmds, verbose)
+        tmp = OUTPUT + ".tmp"
+        _run([[hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs,
+               "-o", tmp]], verbose)
+        os.replace(tmp, OUTPUT)
     return OUTPUT
 
 

@@ -527,6 +527,88 @@ def step(config, const, virt_params, prev, beam_ro):
                   ro=ro, jx=jx, jy=jy, jz=jz)
 
 
+FIELD_NAMES = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
+
+
+def step_guided(config, const, virt_params, prev, beam_ro, guess, threshold):
+    """The xi step with its predictor pass (lcode.py:841-881) replaced by a
+    field guess - the hook of the learned and extrapolated predictors.  Not
+    in the reference; stated here from the reference's own passes:
+
+    1. the predictor push with prev fields from the field-free estimate
+       (lcode.py:843-854), as in step();
+    2. the guess G (six fields) stands for the predictor's fields: the
+       corrector pushes with (G + prev)/2 from the predictor positions,
+       deposits and solves with (G + prev)/2 in the trick term
+       (lcode.py:884-911);
+    3. residual r = ||F_corr - G|| / ||F_corr||, the six fields summed;
+    4. r > threshold (or not a number): the plain predictor and corrector
+       passes run after all (the fields of step());
+    5. the final push with the averages of the accepted pass and the final
+       deposition (lcode.py:914-921).
+
+    Returns (state, r, accepted).  With G equal to the plain predictor's
+    fields the result is step()'s, bit for bit."""
+    beam_ro = np.asarray(beam_ro, dtype=np.float64)
+    prev_fields = tuple(getattr(prev, k) for k in FIELD_NAMES)
+    guess = tuple(np.asarray(g, dtype=np.float64) for g in guess)
+    x_e, y_e = move_estimate_wo_fields(
+        config, const.m, const.x_init, const.y_init,
+        prev.x_offt, prev.y_offt, prev.px, prev.py, prev.pz)
+    p1 = move_smart(config, const.m, const.q, const.x_init, const.y_init,
+                    prev.x_offt, prev.y_offt, x_e, y_e,
+                    prev.px, prev.py, prev.pz, *prev_fields)
+
+    def corrector(first_avg):
+        p2 = move_smart(config, const.m, const.q, const.x_init,
+                        const.y_init, prev.x_offt, prev.y_offt, p1[0], p1[1],
+                        prev.px, prev.py, prev.pz, *first_avg)
+        dens = deposit(config, const.ro_initial, p2[0], p2[1], const.m,
+                       const.q, p2[2], p2[3], p2[4], virt_params)
+        fields, avg = _solve_fields(config, prev, first_avg, beam_ro, *dens)
+        return p2, fields, avg
+
+    g_avg = tuple((g + o) / 2 for g, o in zip(guess, prev_fields))
+    p2, fields, avg = corrector(g_avg)
+    num = sum(float(np.sum((f - g) ** 2)) for f, g in zip(fields, guess))
+    den = sum(float(np.sum(f ** 2)) for f in fields)
+    r = math.sqrt(num / den) if den > 0 else (math.inf if num > 0 else 0.)
+    accepted = r <= threshold
+    if not accepted:
+        dens = deposit(config, const.ro_initial, p1[0], p1[1], const.m,
+                       const.q, p1[2], p1[3], p1[4], virt_params)
+        _, avg1 = _solve_fields(config, prev, prev_fields, beam_ro, *dens)
+        p2, fields, avg = corrector(avg1)
+    x_offt, y_offt, px, py, pz = move_smart(
+        config, const.m, const.q, const.x_init, const.y_init,
+        prev.x_offt, prev.y_offt, p2[0], p2[1],
+        prev.px, prev.py, prev.pz, *avg)
+    ro, jx, jy, jz = deposit(config, const.ro_initial, x_offt, y_offt,
+                             const.m, const.q, px, py, pz, virt_params)
+    Ex, Ey, Ez, Bx, By, Bz = (f.copy() for f in fields)
+    st = Arrays(x_offt=x_offt, y_offt=y_offt, px=px, py=py, pz=pz,
+                Ex=Ex, Ey=Ey, Ez=Ez, Bx=Bx, By=By, Bz=Bz,
+                ro=ro, jx=jx, jy=jy, jz=jz)
+    return st, r, accepted
+
+
+def predictor_fields(config, const, virt_params, prev, beam_ro):
+    """The predictor pass's six fields (lcode.py:841-874): what a perfect
+    guess of step_guided reproduces."""
+    beam_ro = np.asarray(beam_ro, dtype=np.float64)
+    prev_fields = tuple(getattr(prev, k) for k in FIELD_NAMES)
+    x_e, y_e = move_estimate_wo_fields(
+        config, const.m, const.x_init, const.y_init,
+        prev.x_offt, prev.y_offt, prev.px, prev.py, prev.pz)
+    p1 = move_smart(config, const.m, const.q, const.x_init, const.y_init,
+                    prev.x_offt, prev.y_offt, x_e, y_e,
+                    prev.px, prev.py, prev.pz, *prev_fields)
+    dens = deposit(config, const.ro_initial, p1[0], p1[1], const.m,
+                   const.q, p1[2], p1[3], p1[4], virt_params)
+    fields, _ = _solve_fields(config, prev, prev_fields, beam_ro, *dens)
+    return fields
+
+
 def init(config):
     """Validate the config, write reflect_boundary into it, build const,
     virt_params and the zero state (lcode.py:936-975)."""

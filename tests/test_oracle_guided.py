@@ -1,0 +1,79 @@
+"""Pins the oracle's guided step (the predictor hook of lcode.py:841-881)
+against the oracle's step, itself pinned in test_oracle_step.py:
+
+* a perfect guess - the plain predictor's own fields - makes the guided step
+  equal to step() bit for bit, whatever the threshold: this fixes which field
+  set feeds the corrector push, its trick term and the final push;
+* a rejected guess (threshold below the residual) gives step() bit for bit:
+  the fallback is the plain predictor and corrector;
+* the residual is ||F_corr - G|| / ||F_corr|| over the six fields, and with
+  the perfect guess it is the plain step's own predictor-corrector gap;
+* an accepted imperfect guess changes the result (the guess is really used).
+"""
+
+import numpy as np
+
+NAMES = ("x_offt", "y_offt", "px", "py", "pz", "Ex", "Ey", "Ez", "Bx", "By",
+         "Bz", "ro", "jx", "jy", "jz")
+FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
+
+
+def _developed(oracle, nsteps=12):
+    cfg = oracle.make_config(grid_steps=41, grid_step_size=.1,
+                             xi_step_size=.1, coarseness=2, fineness=2,
+                             plasma_padding_steps=6)
+    xs, ys, const, vp, st = oracle.init(cfg)
+    for i in range(nsteps):
+        st = oracle.step(cfg, const, vp, st,
+                         oracle.beam(cfg, i, xs, ys, amplitude=.1, sigma=.6,
+                                     shift_x=.15))
+    beam = oracle.beam(cfg, nsteps, xs, ys, amplitude=.1, sigma=.6,
+                       shift_x=.15)
+    return cfg, const, vp, st, beam
+
+
+def _same(a, b):
+    for k in NAMES:
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+def test_perfect_guess_reproduces_the_step(oracle):
+    cfg, const, vp, st, beam = _developed(oracle)
+    plain = oracle.step(cfg, const, vp, st, beam)
+    f1 = oracle.predictor_fields(cfg, const, vp, st, beam)
+    for threshold in (np.inf, 0.0):
+        got, r, acc = oracle.step_guided(cfg, const, vp, st, beam, f1,
+                                         threshold)
+        _same(got, plain)
+        assert acc == (r <= threshold)
+    # the residual of the perfect guess is the predictor-corrector gap
+    num = sum(np.sum((getattr(plain, k) - f) ** 2) for k, f in zip(FIELDS, f1))
+    den = sum(np.sum(getattr(plain, k) ** 2) for k in FIELDS)
+    assert r > 0
+    assert abs(r - np.sqrt(num / den)) <= 1e-12 * r
+
+
+def test_rejected_guess_falls_back_to_the_step(oracle):
+    cfg, const, vp, st, beam = _developed(oracle)
+    plain = oracle.step(cfg, const, vp, st, beam)
+    prev_fields = [getattr(st, k) for k in FIELDS]
+    got, r, acc = oracle.step_guided(cfg, const, vp, st, beam, prev_fields,
+                                     -1.0)
+    assert not acc and r > 0
+    _same(got, plain)
+
+
+def test_accepted_imperfect_guess_is_used(oracle):
+    cfg, const, vp, st, beam = _developed(oracle)
+    plain = oracle.step(cfg, const, vp, st, beam)
+    f1 = oracle.predictor_fields(cfg, const, vp, st, beam)
+    bent = [f * 1.01 for f in f1]
+    got, r, acc = oracle.step_guided(cfg, const, vp, st, beam, bent, np.inf)
+    assert acc
+    assert np.abs(got.Ez - plain.Ez).max() > 1e-9 * np.abs(plain.Ez).max()
+    # a 1 % error in the guess enters the corrector through one push, so
+    # the corrector fields move by far less than 1 %
+    for k in FIELDS:
+        ref = np.linalg.norm(getattr(plain, k))
+        if ref > 0:
+            assert np.linalg.norm(getattr(got, k) - getattr(plain, k)) < 1e-2 * ref, k
