@@ -186,6 +186,28 @@ int lcode_deposit(lcode_workspace *ws, const lcode_config *cfg,
                   const lcode_virt_params *virt_params,
                   double *const out[4], hipStream_t stream);
 
+/* lcode_deposit without a workspace and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results as
+ * lcode_deposit, except that clamped lookups are not counted and a
+ * non-finite particle is not flagged to the host (its tile is still written
+ * as NaN).  Its scratch - the spill list of particles displaced beyond the
+ * deposition tiles' halo and its counters - is a fixed device array of
+ * 2048^2 + 64 ints (16 MiB) per device, held until the process ends, so
+ * cfg->nf must be at most 2048 (LCODE_E_INVALID otherwise).  Every call on
+ * a device shares that array: calls that overlap on one device (two streams
+ * or two host threads) race and give wrong results.  Each call clears the
+ * counters it needs with a small kernel (no memset node).  Launches a
+ * zero-fill kernel, the tile kernel and the spill kernel on `stream`. */
+int lcode_deposit_noalloc(const lcode_config *cfg, const double *ro_initial,
+                          const double *x_offt, const double *y_offt,
+                          const double *m, const double *q,
+                          const double *px, const double *py,
+                          const double *pz,
+                          const lcode_virt_params *virt_params,
+                          double *const out[4], hipStream_t stream);
+
 /* lcode.py:246-290.  beam_ro is an N*N array or NULL, in which case the
  * scalar beam_scalar is used everywhere (the reference's beam returns a
  * scalar 0 outside the bunch, config_example.py:24-25). out = {Ex,Ey,Bx,By}.*/

@@ -112,6 +112,10 @@ _SIGNATURES = {
                       [_vp, ctypes.POINTER(LcodeConfig)] + [_vp] * 8
                       + [ctypes.POINTER(LcodeVirtParams),
                          ctypes.POINTER(_vp), _vp]),
+    "lcode_deposit_noalloc": (ctypes.c_int,
+                              [ctypes.POINTER(LcodeConfig)] + [_vp] * 8
+                              + [ctypes.POINTER(LcodeVirtParams),
+                                 ctypes.POINTER(_vp), _vp]),
     "lcode_calculate_Ex_Ey_Bx_By": (ctypes.c_int,
                                     [_vp, ctypes.POINTER(LcodeConfig)]
                                     + [_vp] * 5 + [ctypes.c_double]

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -685,13 +686,15 @@ int lcode_move_smart(lcode_workspace* ws, const lcode_config* cfg,
                    pz_prev, fields, out, stream);
 }
 
-static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
-                      const double* ro_initial, const double* xo,
-                      const double* yo, const double* m, const double* q,
-                      const double* px, const double* py, const double* pz,
-                      const lcode_virt_params* vp, double* const out[4],
-                      hipStream_t stream, Gate gate = Gate{nullptr, 0.},
-                      bool adaptive_halo = false) {
+// Deposition arguments that do not depend on where the scratch lives
+// (spill list, counters and flags are set by the caller).
+static DepositArgs deposit_args(const lcode_config* cfg,
+                                const double* ro_initial, const double* xo,
+                                const double* yo, const double* m,
+                                const double* q, const double* px,
+                                const double* py, const double* pz,
+                                const lcode_virt_params* vp,
+                                double* const out[4]) {
     DepositArgs a{};
     a.nf = cfg->nf;
     a.nc = cfg->nc;
@@ -705,12 +708,27 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.P = vp->indices_prev; a.Nn = vp->indices_next;
     a.ro_initial = ro_initial;
     a.ro = out[0]; a.jx = out[1]; a.jy = out[2]; a.jz = out[3];
+    a.halo = kDepositHalo;
+    // the tile's coarse rectangle has at most ext / coarseness + 3 rows
+    a.staged = (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= kDepCoarseMax;
+    a.group = cfg->plasma_fineness;
+    return a;
+}
+
+static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
+                      const double* ro_initial, const double* xo,
+                      const double* yo, const double* m, const double* q,
+                      const double* px, const double* py, const double* pz,
+                      const lcode_virt_params* vp, double* const out[4],
+                      hipStream_t stream, Gate gate = Gate{nullptr, 0.},
+                      bool adaptive_halo = false) {
+    DepositArgs a = deposit_args(cfg, ro_initial, xo, yo, m, q, px, py, pz,
+                                 vp, out);
     a.fault = ws->fault;
     a.spill = ws->spill;
     a.host_flag = ws->host_flag;
     a.spill_count = ws->spill_ctr + ws->spill_slot;
     a.spill_next = ws->spill_ctr + (1 - ws->spill_slot);
-    a.halo = kDepositHalo;
     // after a push of this workspace: a halo of the pushes' displacement
     // bound, read from the slot the pushes raised; the deposition zeroes
     // the other slot, which the next pushes raise
@@ -718,9 +736,6 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
         a.halo_cells = ws->spill_ctr + 2 + ws->bound_slot;
         a.halo_next = ws->spill_ctr + 2 + (1 - ws->bound_slot);
     }
-    // the tile's coarse rectangle has at most ext / coarseness + 3 rows
-    a.staged = (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= kDepCoarseMax;
-    a.group = cfg->plasma_fineness;
     a.gate = gate;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");
     const int rc = launch_deposit(a, stream, &ws->prof);
@@ -741,6 +756,82 @@ int lcode_deposit(lcode_workspace* ws, const lcode_config* cfg,
     for (int k = 0; k < 4; ++k) LCODE_REQUIRE(out[k], "NULL output");
     return do_deposit(ws, cfg, ro_initial, x_offt, y_offt, m, q, px, py, pz,
                       virt_params, out, stream);
+}
+
+}  // extern "C"
+
+// Scratch of lcode_deposit_noalloc: the spill list (kNoAllocMaxFine^2
+// fine-particle indices) followed by its counters.  One copy per device,
+// held until the process ends; in host code `scratch` names a host-side
+// stand-in, so only the address hipGetSymbolAddress returns for the
+// current device is ever used.
+namespace {
+constexpr int kNoAllocMaxFine = 2048;
+constexpr size_t kNoAllocSpillCap = (size_t)kNoAllocMaxFine * kNoAllocMaxFine;
+constexpr int kNoAllocCounters = 64;  // [0] spill count, [1] the spill
+                                      // kernel's next-count slot, [2] faults
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) int scratch[kNoAllocSpillCap + kNoAllocCounters];
+
+static std::mutex g_scratch_mu;
+static int* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+static int scratch_on_current_device(int** out) {
+    int dev = 0;
+    LCODE_HIP(hipGetDevice(&dev));
+    LCODE_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress(scratch) failed: ") +
+                      hipGetErrorString(e));
+            return LCODE_E_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return LCODE_OK;
+}
+
+extern "C" {
+
+int lcode_deposit_noalloc(const lcode_config* cfg, const double* ro_initial,
+                          const double* x_offt, const double* y_offt,
+                          const double* m, const double* q, const double* px,
+                          const double* py, const double* pz,
+                          const lcode_virt_params* virt_params,
+                          double* const out[4], hipStream_t stream) {
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    LCODE_REQUIRE(virt_params && out, "NULL virt_params/out");
+    LCODE_REQUIRE(x_offt && y_offt && m && q && px && py && pz, "NULL particle array");
+    LCODE_REQUIRE(virt_params->influence_prev && virt_params->influence_next &&
+                      virt_params->fine_grid && virt_params->indices_prev &&
+                      virt_params->indices_next,
+                  "NULL virt_params array");
+    for (int k = 0; k < 4; ++k) LCODE_REQUIRE(out[k], "NULL output");
+    LCODE_REQUIRE(cfg->nf <= kNoAllocMaxFine,
+                  "lcode_deposit_noalloc: more than 2048 fine particles per axis "
+                  "(the spill list of its fixed scratch holds 2048^2)");
+    int* s = nullptr;
+    if ((rc = scratch_on_current_device(&s))) return rc;
+    DepositArgs a = deposit_args(cfg, ro_initial, x_offt, y_offt, m, q, px, py,
+                                 pz, virt_params, out);
+    int* ctr = s + kNoAllocSpillCap;
+    a.spill = s;
+    a.spill_count = ctr;
+    a.spill_next = ctr + 1;
+    a.fault = ctr + 2;
+    // the previous call left its count in ctr[0]: clear it with a kernel
+    // (no memset node, so the call can be captured in a graph)
+    if ((rc = launch_zero_ints(ctr, 3, stream))) return rc;
+    return launch_deposit(a, stream, nullptr);
 }
 
 int lcode_calculate_Ex_Ey_Bx_By(lcode_workspace* ws, const lcode_config* cfg,

@@ -255,6 +255,8 @@ struct DepositArgs {
 int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof);
 int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof);
 int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof);
+// Sets n ints at p to 0 with a one-block kernel (no memset node).
+int launch_zero_ints(int* p, int n, hipStream_t s);
 
 // Field solves ---------------------------------------------------------------
 

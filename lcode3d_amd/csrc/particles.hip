@@ -574,6 +574,10 @@ deposit_tile_kernel(DepositArgs a) {
     deposit_tile_body<STAGED>(a);
 }
 
+__global__ void __launch_bounds__(64) zero_ints_kernel(int* p, int n) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0;
+}
+
 inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -590,6 +594,13 @@ int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof) {
     if (a.n == 0) return LCODE_OK;
     ProfScope ps(prof, "push", s);
     push_kernel<<<(a.n + kPushBlock - 1) / kPushBlock, kPushBlock, 0, s>>>(a);
+    LCODE_HIP(hipGetLastError());
+    return LCODE_OK;
+}
+
+int launch_zero_ints(int* p, int n, hipStream_t s) {
+    if (n <= 0) return LCODE_OK;
+    zero_ints_kernel<<<1, 64, 0, s>>>(p, n);
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
