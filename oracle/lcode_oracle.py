@@ -41,6 +41,11 @@ Parity pinning (what checks each function, all under ``tests/``):
   returned) is pinned by mutation: six plausible mis-compositions, each
   built from these operators, drift by 13-58 % and fail that pin
   (tests/test_oracle_step.py).
+* ``step_guided``/``predictor_fields`` - with the predictor's own fields as
+  the guess the guided step equals ``step`` bit for bit at any threshold; a
+  rejected guess gives ``step`` bit for bit; the residual of the perfect
+  guess is the predictor-corrector gap computed directly; an accepted
+  imperfect guess changes the result (tests/test_oracle_guided.py).
 """
 
 import math

@@ -147,3 +147,62 @@ def test_stepper_backs_off_after_rejections(lc):
         floor = torch.linalg.norm(st_plain.Bx).item() if name == "Bz" else 0.
         assert rel_l2(host(getattr(st, name)), host(getattr(st_plain, name)),
                       floor) < 1e-10, name
+
+
+def test_guided_step_matches_oracle_statement(lc, oracle):
+    """The HIP guided step against the oracle's statement of it
+    (oracle.step_guided) at 129^2, xi_step_size 0.01, threshold 5e-3: each
+    runs its own trajectory with the quadratic extrapolation of its own
+    last three states as the guess; every slice's accept/reject decision
+    must agree, the residuals to 1e-9 relative, and all 15 arrays after 60
+    slices to relative L2 1e-10 (Bz against ||Bx||)."""
+    from gpu_helpers import dev
+    say("guided step vs oracle.step_guided, 129^2, 60 slices")
+    cfg = _cfg(dxi=.01)
+    ocfg = oracle.make_config(grid_steps=129, grid_step_size=.05,
+                              xi_step_size=.01, coarseness=2, fineness=2)
+    xs, ys, oconst, ovp, ost = oracle.init(ocfg)
+    _, _, const, vp, st = lc.init(cfg)
+    nc, nf = const.m.shape[0], vp.fine_grid.numel()
+    ohist, ex = [], lc.FieldExtrapolator(order=2)
+    decisions, worst_r = [], 0.0
+    for i in range(60):
+        obeam = oracle.beam(ocfg, i, xs, ys)
+        g = ex.guess()
+        if len(ohist) < 3:
+            ost = oracle.step(ocfg, oconst, ovp, ost, obeam)
+            st = lc.step(cfg, const, vp, st, _beam(lc, cfg, i))
+        else:
+            og = [3 * getattr(ohist[0], k) - 3 * getattr(ohist[1], k)
+                  + getattr(ohist[2], k) for k in ("Ex", "Ey", "Ez", "Bx", "By", "Bz")]
+            ost, orr, oacc = oracle.step_guided(ocfg, oconst, ovp, ost, obeam, og, 5e-3)
+            st = lc.step(cfg, const, vp, st, _beam(lc, cfg, i), guess=g, threshold=5e-3)
+            r = lc.guided_residual(cfg, nc, nf)
+            decisions.append((oacc, r <= 5e-3))
+            worst_r = max(worst_r, abs(r - orr) / orr)
+        ohist.insert(0, ost)
+        del ohist[3:]
+        ex.push(st)
+    n_acc = sum(o for o, _ in decisions)
+    say(f"guided vs oracle: {n_acc} of {len(decisions)} accepted, "
+        f"residual agreement {worst_r:.1e}")
+    assert all(o == g for o, g in decisions), decisions
+    assert 0 < n_acc < len(decisions)
+    assert worst_r < 1e-9
+    for name in STATE:
+        want = getattr(ost, name)
+        floor = np.linalg.norm(ost.Bx) if name == "Bz" else 0.
+        assert rel_l2(host(getattr(st, name)), want, floor) < 1e-10, name
+    # a perfect guess (the oracle's own predictor fields) gives the plain
+    # step of the oracle (the step tests' bar: ro, a small difference of
+    # two unit densities, carries the largest relative error)
+    obeam = oracle.beam(ocfg, 60, xs, ys)
+    f1 = oracle.predictor_fields(ocfg, oconst, ovp, ost, obeam)
+    oplain = oracle.step(ocfg, oconst, ovp, ost, obeam)
+    prev = lc.GPUArrays(**{k: dev(getattr(ost, k)) for k in STATE})
+    got = lc.step(cfg, const, vp, prev, _beam(lc, cfg, 60),
+                  guess=[(1.0, tuple(dev(f) for f in f1))], threshold=1.0)
+    for name in STATE:
+        want = getattr(oplain, name)
+        floor = np.linalg.norm(oplain.Bx) if name == "Bz" else 0.
+        assert rel_l2(host(getattr(got, name)), want, floor) < 1e-10, name
