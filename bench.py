@@ -102,20 +102,21 @@ SOLVE_KERNELS = ("solve_pass_a", "solve_pass_b", "solve_pass_c", "dense_rhs",
 MFMA_PEAK_TFS = {"fp64": 78.6, "bf16x3": 2500.0}
 
 
-def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps,
-                bracket_us=0.0):
+def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps):
     """The same step with each field-solver back-end from the same developed
     state: whole-step throughput over `steps` slices and the six-field solve
-    time from HIP events over `prof_steps` slices (less `bracket_us`, the
-    event-pair overhead per launch)."""
+    time from HIP events around every launch over `prof_steps` slices."""
     import copy
     legs = []
-    for transform, precision in (("fft", "fp64"), ("fft", "fp32"),
+    for transform, precision in (("fft", "fp64"), ("fft", "fp32"), ("fft", "mixed"),
                                  ("dense", "bf16x3"), ("dense", "fp64")):
         c = copy.copy(cfg)
         c.field_solver_transform = transform
         c.field_solver_precision = precision
         st = state
+        if precision == "mixed":   # the same state with FP32 grids
+            st = lc.GPUArrays(**{k: (v.float() if v.shape == state.Ex.shape else v)
+                                 for k, v in vars(state).items()})
         for k in range(10):
             st = lc.step(c, const, vp, st, beams[k])
         torch.cuda.synchronize()
@@ -128,7 +129,7 @@ def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps,
         for k in range(prof_steps):
             st = lc.step(c, const, vp, st, beams[k % len(beams)])
         prof = lc.profile_end()
-        kus = {k: v[1] / v[0] * 1e3 - bracket_us for k, v in prof.items()
+        kus = {k: v[1] / v[0] * 1e3 for k, v in prof.items()
                if k in SOLVE_KERNELS}
         calls = {k: v[0] / (2 * prof_steps) for k, v in prof.items() if k in SOLVE_KERNELS}
         solve_us = sum(kus[k] * calls[k] for k in kus)
@@ -152,121 +153,134 @@ def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps,
 GUIDED_THRESHOLD = 2e-3   # above the one-pass corrector's own residual
 GUIDED_BACKOFF = 8        # plain slices after a rejected guess (doubling to 64)
                           # (about 0.1 xi_step_size = 5.5e-4, p90 1.1e-3)
+RUN_SLICES = 1500         # the 641^2 run of the headline workload
 
 
-def _guided_runs(lc, cosine_gaussian, torch, slices, timed, precision,
-                 predictor=None):
-    """The guided step (GuidedStepper: the guess on the slices the backoff
-    policy selects, GUIDED_THRESHOLD, GUIDED_BACKOFF) against the plain FP64
-    step over `slices` slices from init: on-axis Ez(xi) relative L2 and the
-    worst full-Ez relative L2 of the 100-slice checkpoints; then its
-    throughput over `timed` slices of the bunch (beam slices 103 on), guess
-    inference and the residual read-backs included."""
-    cfg_p = make_config()
-    cfg_g = make_config()
-    cfg_g.field_solver_precision = precision
-    _, _, const, vp, sp = lc.init(cfg_p)
-    lc.init(cfg_g)
-    sg = sp
-    n = cfg_p.grid_steps
-    gs = lc.GuidedStepper(cfg_g, const, vp, GUIDED_THRESHOLD, predictor=predictor,
-                          backoff=GUIDED_BACKOFF)
-    ez_p, ez_g, full = [], [], 0.0
-    for i in range(slices):
-        beam = cosine_gaussian(cfg_p, i)
-        sp = lc.step(cfg_p, const, vp, sp, beam)
-        sg = gs.step(sg, beam)
-        ez_p.append(sp.Ez[n // 2, n // 2])
-        ez_g.append(sg.Ez[n // 2, n // 2])
-        if (i + 1) % 100 == 0:
-            e = (torch.linalg.norm(sg.Ez - sp.Ez) / torch.linalg.norm(sp.Ez)).item()
-            full = max(full, e)
-    ez_p, ez_g = torch.stack(ez_p), torch.stack(ez_g)
-    axis = (torch.linalg.norm(ez_g - ez_p) / torch.linalg.norm(ez_p)).item()
-    out = {"precision": precision, "slices_compared": slices,
-           "ez_axis_rel_l2": axis, "ez_checkpoint_rel_l2_max": full,
-           "guided_slices": gs.guided, "accepted": gs.accepted, "plain_slices": gs.plain,
-           "guess_accept_rate": gs.accepted / max(gs.guided, 1),
-           "backoff": GUIDED_BACKOFF}
-    beams = [cosine_gaussian(cfg_g, 100 + k) for k in range(timed + 3)]
-    _, _, const, vp, st = lc.init(cfg_g)
-    gs = lc.GuidedStepper(cfg_g, const, vp, GUIDED_THRESHOLD, predictor=predictor,
-                          backoff=GUIDED_BACKOFF)
-    for k in range(3):
-        st = lc.step(cfg_g, const, vp, st, beams[k])
-        gs.push(st)
+def run_beams(cfg, cosine_gaussian, slices, **beam):
+    """Beam slices 0 .. slices-1 of a run, generated on the device before
+    any clock starts (scalar 0 behind the bunch)."""
+    return [cosine_gaussian(cfg, i, **beam) for i in range(slices)]
+
+
+def continuous_run(lc, torch, cfg, beams, window, stepper=None, record=False):
+    """One run from init over len(beams) slices, plain steps or
+    `stepper(config, const, virt_params)`'s steps.  Times the slice window
+    [w0, w1) (the run's own history leads into it: developed wake) and the
+    whole run.  record: also the on-axis Ez of every slice and the six
+    fields, densities and momenta every 100 slices (device copies)."""
+    _, _, const, vp, st = lc.init(cfg)
+    gs = stepper(cfg, const, vp) if stepper else None
+    n, c = len(beams), cfg.grid_steps // 2
+    w0, w1 = window
+    ez = torch.zeros(n, dtype=torch.float64, device="cuda") if record else None
+    checks = []
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(timed):
-        st = gs.step(st, beams[3 + k])
+    t_all = time.perf_counter()
+    t_w = dt_w = 0.0
+    for i in range(n):
+        if i == w0:
+            torch.cuda.synchronize()
+            t_w = time.perf_counter()
+        st = gs.step(st, beams[i]) if gs else lc.step(cfg, const, vp, st, beams[i])
+        if i + 1 == w1:
+            torch.cuda.synchronize()
+            dt_w = time.perf_counter() - t_w
+        if record:
+            ez[i] = st.Ez[c, c]
+            if (i + 1) % 100 == 0:
+                checks.append({k: getattr(st, k).clone() for k in (
+                    "Ex", "Ey", "Ez", "Bx", "By", "Bz", "ro", "px", "py", "pz")})
     torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    out.update({"value": timed / dt, "unit": "xi-steps/s",
-                "timed_guided_slices": gs.guided, "timed_accepted": gs.accepted,
-                "timed_plain_slices": gs.plain})
-    return out
+    dt_all = time.perf_counter() - t_all
+    if gs is not None:
+        gs.drain(wait=True)
+    out = {"window_xi_steps_per_s": (w1 - w0) / dt_w, "window": [w0, w1],
+           "run_xi_steps_per_s": n / dt_all, "run_slices": n,
+           "faults": lc.fault_count(cfg, const.m.shape[0])}
+    if gs is not None:
+        out.update({"guided_slices": gs.guided, "accepted": gs.accepted,
+                    "plain_slices": gs.plain})
+    return out, ez, checks
 
 
-def mixed_guided_leg(lc, cosine_gaussian, torch, slices, timed, precision):
-    """Mixed precision (FP32 transforms) + the guided step with the
-    quadratic extrapolation of the last three slices' fields as the guess."""
-    out = _guided_runs(lc, cosine_gaussian, torch, slices, timed, precision)
-    out["predictor"] = ("quadratic extrapolation of the last 3 slices' fields, "
-                        f"corrector residual threshold {GUIDED_THRESHOLD:g}, "
-                        f"backoff {GUIDED_BACKOFF} slices after a rejection")
-    return out
+def compare_runs(torch, ez_ref, checks_ref, ez, checks):
+    """Ez(xi, 0, 0) relative L2 over the run and the worst relative L2 of
+    each recorded array over the 100-slice checkpoints (Bz against |Bx|)."""
+    axis = (torch.linalg.norm(ez - ez_ref) / torch.linalg.norm(ez_ref)).item()
+    worst = {}
+    for r, g in zip(checks_ref, checks):
+        for k, v in r.items():
+            den = torch.linalg.norm(r["Bx"] if k == "Bz" else v).item()
+            e = torch.linalg.norm(g[k].double() - v.double()).item() / max(den, 1e-300)
+            worst[k] = max(worst.get(k, 0.0), e)
+    return {"ez_axis_rel_l2": axis, "checkpoint_rel_l2_max": worst}
 
 
-def learned_leg(lc, cosine_gaussian, torch, slices, timed, precision,
-                 train_runs=2, train_slices=220, epochs=3, hidden=(8, 8)):
-    """Mixed precision + the learned predictor: a ConvPredictor trained on
-    `train_runs` plain 641^2 runs (amplitude ~ U[0.01, 0.1], sigma ~
-    U[0.5, 2] from default_rng(0)), its guess in place of the predictor
-    pass with residual threshold GUIDED_THRESHOLD; accuracy against the
-    plain FP64 step over `slices` slices of the bench beam, throughput over
-    `timed` slices (guess inference included)."""
-    import collections
+def stepper_factory(lc, predictor=None):
+    def make(cfg, const, vp):
+        return lc.GuidedStepper(cfg, const, vp, GUIDED_THRESHOLD,
+                                predictor=predictor, backoff=GUIDED_BACKOFF)
+    return make
+
+
+def mode_legs(lc, torch, cosine_gaussian, window, plain_ref, modes):
+    """Each (name, precision, stepper) mode as a continuous 641^2 run
+    against the plain FP64 run `plain_ref` = (ez, checks, stats)."""
+    ez_p, ch_p, stats_p = plain_ref
+    legs = {}
+    for name, precision, stepper in modes:
+        cfg = make_config()
+        cfg.field_solver_precision = precision
+        beams = run_beams(cfg, cosine_gaussian, RUN_SLICES)
+        timing, _, _ = continuous_run(lc, torch, cfg, beams, window, stepper)
+        acc, ez, checks = continuous_run(lc, torch, cfg, beams, window, stepper,
+                                         record=True)
+        del beams
+        leg = {"precision": precision, **timing}
+        for k in ("guided_slices", "accepted", "plain_slices"):
+            if k in acc:
+                leg[k] = acc[k]
+        leg.update(compare_runs(torch, ez_p, ch_p, ez, checks))
+        leg["window_vs_plain"] = leg["window_xi_steps_per_s"] / stats_p["window_xi_steps_per_s"]
+        leg["run_vs_plain"] = leg["run_xi_steps_per_s"] / stats_p["run_xi_steps_per_s"]
+        legs[name] = leg
+        log(f"{name:14s}: window {leg['window_xi_steps_per_s']:7.1f} xi-steps/s "
+            f"({leg['window_vs_plain']:.2f}x plain), run of {RUN_SLICES} "
+            f"{leg['run_xi_steps_per_s']:7.1f} ({leg['run_vs_plain']:.2f}x); Ez axis rel L2 "
+            f"{leg['ez_axis_rel_l2']:.1e}, checkpoints "
+            + " ".join(f"{k} {v:.1e}" for k, v in leg["checkpoint_rel_l2_max"].items())
+            + (f"; {leg['accepted']} of {leg['guided_slices']} guesses accepted, "
+               f"{leg['plain_slices']} plain" if "guided_slices" in leg else ""))
+    return legs
+
+
+def train_learned(lc, cosine_gaussian, torch, runs=4, slices=RUN_SLICES, every=2,
+                  epochs=4):
+    """The U-Net predictor trained on the GPU on ~3000 slices of `runs` plain
+    FP64 641^2 runs (amplitude ~ U[0.01, 0.1], sigma ~ U[0.5, 2] from
+    default_rng(0)), every `every`-th slice; target F1 - Q (the predictor
+    pass's fields minus the quadratic extrapolation)."""
     import numpy as np
-    from lcode3d_amd.predictor import ConvPredictor
-    cfg_p = make_config()
+    from lcode3d_amd import unet
+    cfg = make_config()
     rng = np.random.default_rng(0)
-    params = [(rng.uniform(.01, .1), rng.uniform(.5, 2)) for _ in range(train_runs)]
-    net = ConvPredictor(cfg_p.grid_steps, hidden=hidden, seed=0, lr=2e-3)
+    params = [(float(rng.uniform(.01, .1)), float(rng.uniform(.5, 2))) for _ in range(runs)]
+    net = unet.UNetPredictor(cfg.grid_steps, seed=0, lr=1e-3)
     t0 = time.perf_counter()
-
-    def windows(amp, sig):
-        _, _, const, vp, st = lc.init(cfg_p)
-        last = collections.deque(maxlen=4)
-        for i in range(train_slices):
-            beam = cosine_gaussian(cfg_p, i, amplitude=amp, sigma=sig)
-            st = lc.step(cfg_p, const, vp, st, beam)
-            last.append(st)
-            if i >= 23 and len(last) == 4:
-                yield last[3], last[2], last[1], last[0], beam
-    acc = net.scale_accumulator()
-    for s0, s1, s2, s3, _ in windows(*params[0]):
-        acc.add(s0, s1, s2, s3)
-    acc.finish()
-    samples = []
-    for amp, sig in params:
-        for s0, s1, s2, s3, beam in windows(amp, sig):
-            samples.append((net.inputs(s1, s2, s3, beam).clone(),
-                            net.targets(s1, s2, s3, s0).clone()))
-    hist = net.fit(samples, epochs=epochs)
-    del samples
+    samples, sx, st = unet.collect_training_data(lc, net, cfg, params, slices, every=every)
     torch.cuda.synchronize()
-    train_s = time.perf_counter() - t0
-    log(f"learned predictor: {len(params)} runs x {train_slices} slices, {epochs} epochs, "
-        f"loss {hist[0]:.3g} -> {hist[-1]:.3g}, {train_s:.1f} s")
-
-    out = _guided_runs(lc, cosine_gaussian, torch, slices, timed, precision,
-                       predictor=net.guess)
-    out["predictor"] = (f"ConvPredictor hidden {list(hidden)}, trained on {len(params)} "
-                        f"plain runs x {train_slices} slices, {epochs} epochs; corrector "
-                        f"residual threshold {GUIDED_THRESHOLD:g}, backoff "
-                        f"{GUIDED_BACKOFF} slices after a rejection")
-    out.update({"train_seconds": train_s, "train_loss": hist})
-    return out
+    t1 = time.perf_counter()
+    hist = unet.train(net, samples, (sx, st), epochs)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    n = len(samples)
+    del samples
+    log(f"learned predictor: {n} samples from {runs} runs ({t1 - t0:.1f} s), "
+        f"{epochs} epochs ({t2 - t1:.1f} s), loss {hist[0]:.3g} -> {hist[-1]:.3g}")
+    info = {"predictor": "U-Net, 4 levels x 32 channels, BF16 MFMA, target F1 - Q",
+            "train_runs": params, "train_samples": n, "epochs": epochs,
+            "collect_seconds": t1 - t0, "train_seconds": t2 - t1, "train_loss": hist}
+    return net, info
 
 
 def precision_table(lc, cosine_gaussian, torch, slices):
@@ -314,101 +328,42 @@ SMALL_WORKLOAD = dict(grid_steps=129, grid_step_size=.05, xi_step_size=.05,
                       reflect_padding_steps=5, plasma_padding_steps=10)
 
 
-def guided_trace_leg(lc, cosine_gaussian, torch, slices):
-    """The guided step in the developed wake: `slices` plain FP32-transform
-    steps (beam slices 103 on), then `slices` guided steps (quadratic
-    extrapolation, threshold GUIDED_THRESHOLD), each block timed without a
-    synchronisation inside; then `slices` more guided steps with the
-    residual read back after each, for the acceptance count."""
-    cfg = make_config()
-    cfg.field_solver_precision = "fp32"
-    _, _, const, vp, st = lc.init(cfg)
-    nc, nf = const.m.shape[0], vp.fine_grid.numel()
-    ex = lc.FieldExtrapolator(order=2)
-    beams = [cosine_gaussian(cfg, 100 + k) for k in range(3 + 3 * slices)]
-    for k in range(3):
-        st = lc.step(cfg, const, vp, st, beams[k])
-        ex.push(st)
-    out = {"workload": "641^2, FP32 transforms, slices 103 on", "slices": slices,
-           "threshold": GUIDED_THRESHOLD}
-    for b, mode in enumerate(("plain", "guided")):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(slices):
-            beam = beams[3 + b * slices + k]
-            if mode == "plain":
-                st = lc.step(cfg, const, vp, st, beam)
-            else:
-                st = lc.step(cfg, const, vp, st, beam, guess=ex.guess(),
-                             threshold=GUIDED_THRESHOLD)
-            ex.push(st)
-        torch.cuda.synchronize()
-        out[f"{mode}_us_per_slice"] = (time.perf_counter() - t0) / slices * 1e6
-    accepted = 0
-    for k in range(slices):
-        st = lc.step(cfg, const, vp, st, beams[3 + 2 * slices + k],
-                     guess=ex.guess(), threshold=GUIDED_THRESHOLD)
-        ex.push(st)
-        accepted += lc.guided_residual(cfg, nc, nf) <= GUIDED_THRESHOLD
-    out["accepted_after"] = accepted
-    return out
-
-
 LARGE_WORKLOAD = dict(grid_steps=1281, grid_step_size=.0125, xi_step_size=.0025,
                       plasma_coarseness=3, plasma_fineness=3)
 LARGE_BEAM = dict(amplitude=.1, sigma=.7, shift_x=.5)
 
 
-def large_workload_leg(lc, cosine_gaussian, torch, slices):
+def large_workload_leg(lc, torch, cosine_gaussian, slices):
     """The large non-axisymmetric case: 1281^2 (h 0.0125, xi step 0.0025,
     coarseness 3, fineness 3), beam amplitude 0.1, sigma 0.7, offset +0.5
-    in x.  The plain FP64 run and the mixed run (FP32 transforms,
-    GuidedStepper with the quadratic extrapolation) over the same `slices`
-    slices, each timed on its own; Ez(xi, 0, 0) and the full Ez every 500
-    slices compared between them."""
+    in x.  The plain FP64 run and the mixed-precision run with the guided
+    step (quadratic extrapolation) over the same `slices` slices, each a
+    continuous run timed whole and over slices [slices/3, slices/3 + 300);
+    Ez(xi, 0, 0) and the fields every 100 slices compared between them."""
     def cfg(precision):
         c = make_config()
         for k, v in LARGE_WORKLOAD.items():
             setattr(c, k, v)
         c.field_solver_precision = precision
         return c
+    w0 = slices // 3
+    window = (w0, min(w0 + 300, slices))
     out = {"workload": "1281^2, coarseness 3, fineness 3, beam 0.1 / 0.7 / +0.5 in x",
            "slices": slices}
-    runs = {}
-    for mode in ("plain", "mixed"):
-        c = cfg("fp64" if mode == "plain" else "fp32")
-        _, _, const, vp, st = lc.init(c)
-        beams = [cosine_gaussian(c, i, **LARGE_BEAM) for i in range(slices)]
-        gs = (lc.GuidedStepper(c, const, vp, GUIDED_THRESHOLD, backoff=GUIDED_BACKOFF)
-              if mode == "mixed" else None)
-        n = c.grid_steps
-        axis, checks = [], []
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(slices):
-            st = gs.step(st, beams[i]) if gs else lc.step(c, const, vp, st, beams[i])
-            axis.append(st.Ez[n // 2, n // 2])
-            if (i + 1) % 500 == 0:
-                checks.append(st.Ez.clone())
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        del beams
-        runs[mode] = (torch.stack(axis), checks)
-        out[f"{mode}_xi_steps_per_s"] = slices / dt
-        out[f"{mode}_faults"] = lc.fault_count(c, const.m.shape[0])
-        if gs:
-            out.update({"guided_slices": gs.guided, "accepted": gs.accepted,
-                        "plain_slices": gs.plain})
-        else:
-            out["max_offset_cells"] = max(st.x_offt.abs().max().item(),
-                                          st.y_offt.abs().max().item()) / c.grid_step_size
-        del st
-    (ap, cp), (am, cm) = runs["plain"], runs["mixed"]
-    out["mixed_vs_plain"] = out["mixed_xi_steps_per_s"] / out["plain_xi_steps_per_s"]
-    out["ez_axis_rel_l2"] = (torch.linalg.norm(am - ap) / torch.linalg.norm(ap)).item()
-    out["ez_checkpoint_rel_l2_max"] = max(
-        [(torch.linalg.norm(m - p) / torch.linalg.norm(p)).item() for p, m in zip(cp, cm)],
-        default=0.0)
+    c = cfg("fp64")
+    beams = run_beams(c, cosine_gaussian, slices, **LARGE_BEAM)
+    plain, ez_p, ch_p = continuous_run(lc, torch, c, beams, window, record=True)
+    out["plain"] = plain
+    for name, prec, stepper in (("mixed", "mixed", None),
+                                ("mixed_guided", "mixed", stepper_factory(lc))):
+        r, ez, ch = continuous_run(lc, torch, cfg(prec), beams, window, stepper,
+                                   record=True)
+        r.update(compare_runs(torch, ez_p, ch_p, ez, ch))
+        r["run_vs_plain"] = r["run_xi_steps_per_s"] / plain["run_xi_steps_per_s"]
+        r["window_vs_plain"] = r["window_xi_steps_per_s"] / plain["window_xi_steps_per_s"]
+        out[name] = r
+        del ez, ch
+    del beams, ch_p
     return out
 
 
@@ -484,32 +439,29 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", choices=("fp64", "fp32"), default="fp64",
-                    help="field-solve transform precision (fp32: mixed)")
     ap.add_argument("--solver-steps", type=int, default=50,
                     help="slices per field-solver back-end comparison leg (0: skip)")
-    ap.add_argument("--settle", type=int, default=100,
-                    help="untimed slices run before warm-up (plasma develops)")
+    ap.add_argument("--settle", type=int, default=400,
+                    help="untimed slices run before warm-up: the timed window "
+                         "starts at slice settle + warmup, in the developed wake")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="steps timed per kernel with HIP events")
-    ap.add_argument("--mixed-slices", type=int, default=1500,
-                    help="slices of the mixed+guided vs plain comparison (0: skip)")
+    ap.add_argument("--no-modes", dest="modes", action="store_false",
+                    help="skip the 1500-slice runs of the mixed / guided / learned "
+                         "modes against the plain FP64 run")
     ap.add_argument("--guided-threshold", type=float, default=GUIDED_THRESHOLD,
                     help="corrector residual threshold of the guided legs")
     ap.add_argument("--precision-table", type=int, default=0, metavar="SLICES",
                     help="mixed-precision error table over SLICES slices (0: skip)")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
-                    help="skip the mixed+learned leg")
+                    help="skip the learned-predictor leg")
     ap.add_argument("--no-small-workload", dest="small_workload", action="store_false",
                     help="skip the 129^2 leg (GPU and oracle)")
     ap.add_argument("--guided-backoff", type=int, default=GUIDED_BACKOFF,
                     help="plain slices after a rejected guess (0: guess every slice)")
     ap.add_argument("--large", type=int, default=0, metavar="SLICES",
-                    help="also run the 1281^2 fineness-3 off-axis case, plain FP64 and "
-                         "mixed + guided, for SLICES slices (0: skip)")
-    ap.add_argument("--guided-trace", type=int, default=0, metavar="SLICES",
-                    help="only time plain against guided steps in the developed "
-                         "wake (SLICES each) and count accepted guesses")
+                    help="also run the 1281^2 fineness-3 off-axis case, plain FP64 "
+                         "against mixed and mixed + guided, for SLICES slices (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -526,33 +478,26 @@ def main():
     from lcode3d_amd.beams import cosine_gaussian
 
     torch.cuda.set_device(0)
-    if args.guided_trace:
-        r = guided_trace_leg(lc, cosine_gaussian, torch, args.guided_trace)
-        log(f"plain {r['plain_us_per_slice']:.1f} us/slice, guided "
-            f"{r['guided_us_per_slice']:.1f} us/slice, accepted "
-            f"{r['accepted_after']} of {r['slices']} after")
-        print(json.dumps({"guided_trace": r}))
-        return
     cfg = make_config()
-    cfg.field_solver_precision = args.precision
     xs, ys, const, vp, state = lc.init(cfg)
     nc = const.m.shape[0]
     n = cfg.grid_steps
     total = args.warmup + args.steps
+    w0 = args.settle + args.warmup
     # beam slices of the run, resident in HBM before the clock starts
-    first = args.settle
-    beams = [cosine_gaussian(cfg, first + k) for k in range(total)]
-    for k in range(first):  # bring the plasma into a developed state
-        state = lc.step(cfg, const, vp, state, cosine_gaussian(cfg, k))
+    beams = run_beams(cfg, cosine_gaussian, max(RUN_SLICES, w0 + args.steps
+                                                + args.profile_steps))
+    for k in range(args.settle):  # bring the plasma into the developed wake
+        state = lc.step(cfg, const, vp, state, beams[k])
     torch.cuda.synchronize()
     log(f"workload 641^2, nc={nc}, nf={vp.fine_grid.numel()}, "
-        f"{args.warmup} warm-up + {args.steps} timed steps")
+        f"{args.warmup} warm-up + {args.steps} timed steps (slices {w0}-{w0 + args.steps - 1})")
     for k in range(args.warmup):
-        state = lc.step(cfg, const, vp, state, beams[k])
+        state = lc.step(cfg, const, vp, state, beams[args.settle + k])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        state = lc.step(cfg, const, vp, state, beams[args.warmup + k])
+        state = lc.step(cfg, const, vp, state, beams[w0 + k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     finite = bool(torch.isfinite(state.Ez).all().item())
@@ -567,33 +512,30 @@ def main():
     log(f"{steps_per_s:.1f} xi-steps/s, {ms:.3f} ms/step, "
         f"step-level {step_bytes / (dt / args.steps) / 1e9:.1f} GB/s algorithmic")
 
-    # per-kernel times over a separate region, HIP events on the step stream
-    roofline, kernels, bracket_us = None, {}, 0.0
+    # per-kernel times over the next slices: HIP events around every
+    # launch on the step's stream (an event pair also times the launch's
+    # dispatch, so these durations are upper bounds of the kernels' own;
+    # the rocprofv3 kernel trace in profiles/ gives the kernels alone)
+    roofline, kernels = None, {}
     if args.profile_steps:
         lc.profile_begin(64 * args.profile_steps)
         for k in range(args.profile_steps):
-            state = lc.step(cfg, const, vp, state, beams[k % total])
+            state = lc.step(cfg, const, vp, state, beams[w0 + args.steps + k])
         prof = lc.profile_end()
+        event_sum = 0.0
         for name, (calls, kms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
             avg_us = kms / calls * 1e3
+            b = kernel_bytes(name, nc, n)
             kernels[name] = {"launches_per_step": calls / args.profile_steps,
                              "avg_us": avg_us,
-                             "share": kms / sum(v[1] for v in prof.values())}
-        # An event pair around a launch also times the launch's dispatch:
-        # the per-launch excess of the bracketed kernel times over the
-        # unbracketed timed loop is that overhead; subtracting it gives the
-        # kernel's own duration (what a rocprofv3 kernel trace reports).
-        launches = sum(v["launches_per_step"] for v in kernels.values())
-        bracket_sum = sum(v["launches_per_step"] * v["avg_us"] for v in kernels.values())
-        bracket_us = max(0.0, (bracket_sum - ms * 1e3) / launches)
-        for name, v in kernels.items():
-            v["kernel_us"] = v["avg_us"] - bracket_us
-            b = kernel_bytes(name, nc, n)
-            v["GBps"] = (b / (v["kernel_us"] * 1e-6) / 1e9) if b else None
-        log(f"  event-pair overhead {bracket_us:.2f} us per launch; kernel durations:")
+                             "share": kms / sum(v[1] for v in prof.values()),
+                             "GBps": (b / (avg_us * 1e-6) / 1e9) if b else None}
+            event_sum += calls / args.profile_steps * avg_us
+        log(f"  kernel durations (HIP events; their sum {event_sum:.1f} us per step "
+            f"against {ms * 1e3:.1f} us measured):")
         for name, v in kernels.items():
             log(f"  {name:14s} {v['launches_per_step']:4.1f}/step "
-                f"{v['kernel_us']:8.2f} us  {v['share'] * 100:5.1f}%"
+                f"{v['avg_us']:8.2f} us  {v['share'] * 100:5.1f}%"
                 + (f"  {v['GBps']:7.0f} GB/s" if v["GBps"] else ""))
         dom = max((k for k in kernels if kernels[k]["GBps"]),
                   key=lambda k: kernels[k]["share"])
@@ -602,9 +544,7 @@ def main():
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                     "bytes_per_launch": kernel_bytes(dom, nc, n),
-                    "avg_us": kernels[dom]["kernel_us"],
-                    "avg_us_with_event_pair": kernels[dom]["avg_us"],
-                    "event_pair_overhead_us": bracket_us}
+                    "avg_us": kernels[dom]["avg_us"]}
         if "deposit_tile" in kernels:
             # the reference's deposition rate measure: 4 quantities x 9
             # stencil cells = 36 atomic adds per fine particle
@@ -613,10 +553,9 @@ def main():
             v = kernels["deposit_tile"]
             updates = 36 * vp.fine_grid.numel() ** 2
             v["fine_particle_updates_per_launch"] = updates
-            v["fine_particle_updates_per_s"] = updates / (v["kernel_us"] * 1e-6)
+            v["fine_particle_updates_per_s"] = updates / (v["avg_us"] * 1e-6)
             log(f"  deposit_tile: {updates / 1e6:.1f} M fine-particle stencil updates per launch, "
-                f"{v['fine_particle_updates_per_s'] / 1e12:.2f} T updates/s "
-                f"({vp.fine_grid.numel() ** 2 / (v['kernel_us'] * 1e-6) / 1e9:.1f} G fine particles/s)")
+                f"{v['fine_particle_updates_per_s'] / 1e12:.2f} T updates/s")
     if args.copy_probe_mib:
         src = torch.ones(args.copy_probe_mib << 17, dtype=torch.float64,
                          device="cuda")
@@ -628,48 +567,42 @@ def main():
 
     solvers = None
     if args.solver_steps:
-        solvers = solver_legs(lc, torch, cfg, const, vp, state, beams,
-                              args.solver_steps, min(args.profile_steps or 10, 10),
-                              bracket_us)
+        solvers = solver_legs(lc, torch, cfg, const, vp, state, beams[w0:w0 + total],
+                              args.solver_steps, min(args.profile_steps or 10, 10))
+    del state
 
-    mixed = None
-    if args.mixed_slices:
-        mixed = mixed_guided_leg(lc, cosine_gaussian, torch, args.mixed_slices,
-                                 args.steps, "fp32")
-        mixed["vs_plain"] = mixed["value"] / steps_per_s
-        log(f"mixed+guided: {mixed['value']:.1f} xi-steps/s "
-            f"({mixed['vs_plain']:.2f}x plain), Ez axis rel L2 "
-            f"{mixed['ez_axis_rel_l2']:.2e}, checkpoints max "
-            f"{mixed['ez_checkpoint_rel_l2_max']:.2e} over "
-            f"{mixed['slices_compared']} slices ({mixed['accepted']} of "
-            f"{mixed['guided_slices']} guesses accepted, {mixed['plain_slices']} plain "
-            f"slices); timed: {mixed['timed_accepted']} of {mixed['timed_guided_slices']} "
-            f"accepted, {mixed['timed_plain_slices']} plain")
-
-    learned = None
-    if args.mixed_slices and args.learned:
-        learned = learned_leg(lc, cosine_gaussian, torch, args.mixed_slices,
-                              args.steps, "fp32")
-        learned["vs_plain"] = learned["value"] / steps_per_s
-        log(f"mixed+learned: {learned['value']:.1f} xi-steps/s "
-            f"({learned['vs_plain']:.2f}x plain), Ez axis rel L2 "
-            f"{learned['ez_axis_rel_l2']:.2e}, checkpoints max "
-            f"{learned['ez_checkpoint_rel_l2_max']:.2e} over "
-            f"{learned['slices_compared']} slices ({learned['accepted']} of "
-            f"{learned['guided_slices']} guesses accepted, {learned['plain_slices']} plain "
-            f"slices); timed: {learned['timed_accepted']} of {learned['timed_guided_slices']} "
-            f"accepted, {learned['timed_plain_slices']} plain")
+    # the whole 1500-slice run, plain FP64, and the other modes against it
+    window = (w0, w0 + args.steps)
+    run_beams_plain = beams[:RUN_SLICES]
+    plain_run, _, _ = continuous_run(lc, torch, cfg, run_beams_plain, window)
+    log(f"plain FP64 run of {RUN_SLICES} slices: {plain_run['run_xi_steps_per_s']:.1f} "
+        f"xi-steps/s (window {plain_run['window_xi_steps_per_s']:.1f})")
+    modes = None
+    if args.modes:
+        _, ez_p, ch_p = continuous_run(lc, torch, cfg, run_beams_plain, window, record=True)
+        mode_list = [("mixed", "mixed", None),
+                     ("mixed_guided", "mixed", stepper_factory(lc)),
+                     ("fp64_guided", "fp64", stepper_factory(lc))]
+        learned_info = None
+        if args.learned:
+            net, learned_info = train_learned(lc, cosine_gaussian, torch)
+            mode_list += [("mixed_learned", "mixed", stepper_factory(lc, net.guess)),
+                          ("fp64_learned", "fp64", stepper_factory(lc, net.guess))]
+        modes = mode_legs(lc, torch, cosine_gaussian, window, (ez_p, ch_p, plain_run),
+                          mode_list)
+        if learned_info:
+            modes["mixed_learned"].update(learned_info)
+        del ch_p
+    del beams, run_beams_plain
 
     large = None
     if args.large:
-        large = large_workload_leg(lc, cosine_gaussian, torch, args.large)
-        log(f"1281^2 x {large['slices']} slices: plain {large['plain_xi_steps_per_s']:.1f}, "
-            f"mixed+guided {large['mixed_xi_steps_per_s']:.1f} xi-steps/s "
-            f"({large['mixed_vs_plain']:.2f}x); Ez axis rel L2 {large['ez_axis_rel_l2']:.2e}, "
-            f"checkpoints max {large['ez_checkpoint_rel_l2_max']:.2e}; "
-            f"{large['accepted']} of {large['guided_slices']} guesses accepted; max offset "
-            f"{large['max_offset_cells']:.2f} cells; faults {large['plain_faults']}/"
-            f"{large['mixed_faults']}")
+        large = large_workload_leg(lc, torch, cosine_gaussian, args.large)
+        for k in ("mixed", "mixed_guided"):
+            r = large[k]
+            log(f"1281^2 x {args.large} {k}: {r['run_xi_steps_per_s']:.1f} xi-steps/s "
+                f"({r['run_vs_plain']:.2f}x plain {large['plain']['run_xi_steps_per_s']:.1f}); "
+                f"Ez axis rel L2 {r['ez_axis_rel_l2']:.1e}")
 
     small = None
     if args.small_workload:
@@ -689,9 +622,7 @@ def main():
         log(f"cpu baseline {cpu['value']:.3f} xi-steps/s on {cpu['cores']} cores")
 
     out = {
-        "metric": "xi-steps/s at 641^2 grid (plain FP64 HIP step)"
-                  if args.precision == "fp64" else
-                  "xi-steps/s at 641^2 grid (FP32-transform mixed HIP step)",
+        "metric": "xi-steps/s at 641^2 grid (plain FP64 HIP step)",
         "value": steps_per_s,
         "unit": "xi-steps/s",
         "n_gpus": 1,
@@ -701,18 +632,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64+f32 transforms",
-        "data": "synthetic cosine-Gaussian beam (amplitude 0.05, sigma 1), "
-                f"slices {args.settle}+ of the bunch",
+        "dtype": "f64",
+        "data": "synthetic cosine-Gaussian beam (amplitude 0.05, sigma 1); timed "
+                f"slices {w0}-{w0 + args.steps - 1} of one run (developed wake)",
         "config": {"workload": "lcode3d xi step, 641^2 grid, coarseness 3, "
-                               f"fineness 2, {'plain FP64' if args.precision == 'fp64' else 'FP32 transforms'}",
+                               "fineness 2, plain FP64",
                    "grid_steps": n, "coarse_particles": nc * nc,
                    "fine_particles": int(vp.fine_grid.numel()) ** 2},
         "roofline": roofline,
         "kernels": kernels,
+        "full_run": plain_run,
         "solver_backends": solvers,
-        "mixed_guided": mixed,
-        "mixed_learned": learned,
+        "modes": modes,
         "precision_table": ptable,
         "small_workload": small,
         "large_workload": large,

@@ -19,10 +19,12 @@
  *                                      (lcode.py:796, :704)
  *   lcode_deposit                   - deposit + deposit_kernel
  *                                      (lcode.py:666, :626)
+ *   lcode_plasma_sizes/make_plasma  - make_plasma (lcode.py:490)
  *   lcode_calculate_Ex_Ey_Bx_By     - calculate_Ex_Ey_Bx_By (lcode.py:246)
  *   lcode_calculate_Ez              - calculate_Ez (lcode.py:160)
  *   lcode_calculate_Bz              - calculate_Bz (lcode.py:333)
  *   lcode_step                      - step (lcode.py:828)
+ *   lcode_step_mixed                - the same step in mixed precision
  *   lcode_step_guided(_residual)    - step with its predictor pass
  *                                      (lcode.py:841-881) replaced by a
  *                                      supplied field guess
@@ -62,11 +64,24 @@ enum {
                                averages stay FP64 */
     LCODE_SOLVER_DENSE64 = 2,      /* dense N x N sine/cosine matrices,
                                       FP64 MFMA (v_mfma_f64_16x16x4) */
-    LCODE_SOLVER_DENSE_BF16X3 = 3  /* the same matrices on BF16 MFMA with
+    LCODE_SOLVER_DENSE_BF16X3 = 3, /* the same matrices on BF16 MFMA with
                                       hi/lo splits of both operands, three
                                       products, FP32 accumulation; RHS,
                                       tables, Bz mean, Variant A and
                                       averages stay FP64 */
+    LCODE_SOLVER_MIXED = 4         /* the mixed-precision step
+                                      (lcode_step_mixed): the ten N x N
+                                      state grids (fields, densities,
+                                      currents, averages, guess) stored in
+                                      FP32 and the FFT32 transforms; FP64
+                                      stays in the particle positions and
+                                      momenta and their updates, the
+                                      deposition arithmetic, the RHS
+                                      arithmetic (trick term included), the
+                                      Bz zero mode and mean, and the ion
+                                      background ro_initial.  The standalone
+                                      operators treat it as FFT32 on FP64
+                                      grids. */
 };
 
 /* The config attributes the step reads (config_example.py:1-17), with
@@ -102,6 +117,14 @@ typedef struct {
     double *Ex, *Ey, *Ez, *Bx, *By, *Bz;          /* N*N */
     double *ro, *jx, *jy, *jz;                    /* N*N */
 } lcode_state;
+
+/* The state of the mixed-precision step (LCODE_SOLVER_MIXED): FP64
+ * particles, FP32 grids, the same 15 arrays as lcode_state. */
+typedef struct {
+    double *x_offt, *y_offt, *px, *py, *pz;      /* nc*nc, FP64 */
+    float *Ex, *Ey, *Ez, *Bx, *By, *Bz;           /* N*N, FP32 */
+    float *ro, *jx, *jy, *jz;                     /* N*N, FP32 */
+} lcode_state_mixed;
 
 typedef struct lcode_workspace lcode_workspace;
 
@@ -175,6 +198,21 @@ int lcode_move_smart(lcode_workspace *ws, const lcode_config *cfg,
                      const double *const fields[6],
                      double *const out[5],
                      hipStream_t stream);
+
+/* Sizes of make_plasma's arrays (lcode.py:452-487) for a plasma of `steps`
+ * cells: nc coarse particles per axis, nf fine ones. */
+int lcode_plasma_sizes(int32_t steps, int32_t coarseness, int32_t fineness,
+                       int32_t *nc, int32_t *nf);
+
+/* make_plasma (lcode.py:490-573) on the device: x_init, y_init, m, q of
+ * the coarse particles (nc*nc each; offsets and momenta start at zero) and
+ * virt_params (nf each): the same FP64 values as the reference's NumPy. */
+int lcode_make_plasma(int32_t steps, double cell_size, int32_t coarseness,
+                      int32_t fineness, double *x_init, double *y_init,
+                      double *m, double *q, double *influence_prev,
+                      double *influence_next, double *fine_grid,
+                      int64_t *indices_prev, int64_t *indices_next,
+                      hipStream_t stream);
 
 /* lcode.py:666-689.  ro_initial may be NULL (the reference's scalar 0 in
  * initial_deposition, lcode.py:697).  out = {ro, jx, jy, jz}. */
@@ -265,9 +303,43 @@ int lcode_step_guided(lcode_workspace *ws, const lcode_config *cfg,
                       double threshold, const lcode_state *out,
                       hipStream_t stream);
 
+/* lcode_step in the mixed precision of LCODE_SOLVER_MIXED (cfg->solver
+ * must be LCODE_SOLVER_MIXED; the const arrays, ro_initial included, stay
+ * FP64): the same passes, with the state's grids, the deposition and the
+ * transforms in FP32. */
+int lcode_step_mixed(lcode_workspace *ws, const lcode_config *cfg,
+                     const lcode_const *cnst,
+                     const lcode_virt_params *virt_params,
+                     const lcode_state_mixed *prev, const double *beam_ro,
+                     double beam_scalar, const lcode_state_mixed *out,
+                     hipStream_t stream);
+
+/* A field guess over FP32 field sets (the mixed step's fields). */
+typedef struct {
+    const float *f[3][6];
+    double coef[3];
+    int32_t nterms;
+} lcode_guess_mixed;
+
+/* lcode_step_guided in the mixed precision of LCODE_SOLVER_MIXED. */
+int lcode_step_guided_mixed(lcode_workspace *ws, const lcode_config *cfg,
+                            const lcode_const *cnst,
+                            const lcode_virt_params *virt_params,
+                            const lcode_state_mixed *prev,
+                            const double *beam_ro, double beam_scalar,
+                            const lcode_guess_mixed *guess, double threshold,
+                            const lcode_state_mixed *out, hipStream_t stream);
+
 /* Residual of the last guided step on this workspace (synchronises). */
 int lcode_step_guided_residual(lcode_workspace *ws, hipStream_t stream,
                                double *residual);
+
+/* The same without synchronising: enqueues on `stream` a copy of the last
+ * guided step's two sums {sum (F_corr - guess)^2, sum F_corr^2} to dst
+ * (2 doubles, device or pinned host memory); the residual is
+ * sqrt(dst[0] / dst[1]) once the stream has reached the copy. */
+int lcode_step_guided_residual_async(lcode_workspace *ws, double *dst,
+                                     hipStream_t stream);
 
 /* Plasma-noise diagnostic of one density grid (lcode.py:980-989,
  * diags_ro_zn before its running maximum): zn = mean |ro - G ro| /
@@ -279,69 +351,89 @@ int lcode_step_guided_residual(lcode_workspace *ws, hipStream_t stream,
 int lcode_diag_zn(lcode_workspace *ws, const lcode_config *cfg,
                   const double *ro, double *zn_out, hipStream_t stream);
 
+
+
 /* ---------------------------------------------------------------------------
- * Learned predictor kernels.  No counterpart in the reference: they train
- * and run a small convolutional network whose output is a field guess for
+ * Learned predictor: a 4-level convolutional encoder-decoder with 32
+ * channels per level.  No counterpart in the reference: it trains on slices
+ * the plain step generates and its output is a field guess for
  * lcode_step_guided (the learned form of the predictor pass,
- * lcode.py:841-881).  FP32 planes [C][H][W]; 3 x 3 convolutions with zero
- * padding; weights w[cout][cin][3][3], biases b[cout]; 1 <= cin, cout <= 16.
+ * lcode.py:841-881).  Level 0 has S = (N + 3) / 4 pixels per side (the
+ * input planes are 4 x 4 means of the grids), levels 1..3 have
+ * S_{l+1} = (S_l + 1) / 2:
+ *   e0 = relu(C0(x))  e1 = relu(C1(pool e0))  e2 = relu(C2(pool e1))
+ *   e3 = relu(C3(pool e2))  d2 = relu(C4[up e3, e2])  d1 = relu(C5[up d2, e1])
+ *   d0 = relu(C6[up d1, e0])  y = C7(d0)
+ * Ck: 3 x 3 convolutions, zero padding, bias; channels 14->32, 32->32 (x3),
+ * 64->32 (x3), 32->6.  pool: 2 x 2 mean (zeros beyond the edge); up:
+ * nearest neighbour.  BF16 weights (packed from FP32 master copies) and
+ * activations on v_mfma_f32_16x16x32_bf16, FP32 accumulation.  The caller
+ * owns every buffer (sizes from lcode_unet_sizes).
  * -------------------------------------------------------------------------*/
+typedef struct {
+    int32_t S;                 /* level-0 size */
+    float *params;             /* FP32 master parameters: per layer
+                                  w[cout][cin][3][3] then b[cout] (sizes[0]) */
+    float *grad;               /* their gradient (sizes[0]) */
+    float *adam_m, *adam_v;    /* Adam moments (sizes[0] each) */
+    uint16_t *wpack;           /* packed bfloat16 weights (sizes[1]) */
+    uint16_t *act;             /* activations and gradients, bfloat16
+                                  [s][s][32] tensors (sizes[2]) */
+    float *y;                  /* network output [S][S][16], channels 0..5 */
+    float *partial;            /* weight-gradient partials (sizes[3]) */
+    float *loss_part;          /* loss partial sums (sizes[4]) */
+    float *loss;               /* 1 float: the last training loss */
+} lcode_unet;
 
-/* out = conv(in, w) + b, then max(., 0) when relu != 0. */
-int lcode_pred_conv3x3_forward(const float *in, int32_t cin, const float *w,
-                               const float *b, float *out, int32_t cout,
-                               int32_t H, int32_t W, int32_t relu,
-                               hipStream_t stream);
+/* Element counts of the buffers: params, wpack, act, partial, loss_part. */
+int lcode_unet_sizes(int32_t S, int64_t sizes[5]);
+/* Element offset of activation/gradient tensor `index` in act, and of
+ * layer `layer`'s parameters in params (index/layer == count: the total). */
+int64_t lcode_unet_tensor_offset(int32_t S, int32_t index);
+int64_t lcode_unet_param_offset(int32_t layer);
 
-/* Gradient with respect to the layer input; when relu_input is not NULL
- * (the layer's input came out of a ReLU) it is zeroed where relu_input <= 0. */
-int lcode_pred_conv3x3_backward_input(const float *grad_out, int32_t cout,
-                                      const float *w, float *grad_in,
-                                      int32_t cin, const float *relu_input,
-                                      int32_t H, int32_t W, hipStream_t stream);
-
-/* Gradients of w and b, written as grad_wb[cout*cin*9 + cout] (weights,
- * then biases); `scratch` holds lcode_pred_backward_weight_scratch floats. */
-int lcode_pred_conv3x3_backward_weight(const float *in, int32_t cin,
-                                       const float *grad_out, int32_t cout,
-                                       float *scratch, float *grad_wb,
-                                       int32_t H, int32_t W, hipStream_t stream);
-int64_t lcode_pred_backward_weight_scratch(int32_t cin, int32_t cout,
-                                           int32_t H, int32_t W);
-
-/* g = 2 (y - t) / n and *loss_sum = sum (y - t)^2 (device float);
- * `scratch` holds (n + 255) / 256 floats. */
-int lcode_pred_mse_grad(const float *y, const float *t, int64_t n, float *g,
-                        float *scratch, float *loss_sum, hipStream_t stream);
-
-/* One Adam step (step number t >= 1) on n parameters. */
-int lcode_pred_adam(float *p, const float *g, float *m, float *v, int32_t n,
-                    float lr, float beta1, float beta2, float eps, int32_t t,
+/* params -> wpack (after the caller writes params). */
+int lcode_unet_pack(const lcode_unet *net, hipStream_t stream);
+/* y = net(x); x = input planes [S][S][32] bfloat16 (lcode_unet_io 0). */
+int lcode_unet_forward(const lcode_unet *net, const uint16_t *x,
+                       hipStream_t stream);
+/* The same y without keeping the pooled and upsampled copies that
+ * lcode_unet_backward needs (each convolution pools or upsamples its input
+ * while staging it): the inference form. */
+int lcode_unet_infer(const lcode_unet *net, const uint16_t *x,
+                     hipStream_t stream);
+/* After lcode_unet_forward(x): the mean squared error against targets
+ * [S][S][6] (FP32, into *loss) and its gradient into grad. */
+int lcode_unet_backward(const lcode_unet *net, const uint16_t *x,
+                        const float *targets, hipStream_t stream);
+/* One Adam step t >= 1 on params from grad, then lcode_unet_pack. */
+int lcode_unet_adam(const lcode_unet *net, int32_t t, float lr,
                     hipStream_t stream);
 
-/* FP64 fields <-> the network's FP32 planes.  With Q = 3 F(n-1) - 3 F(n-2)
- * + F(n-3) (the quadratic extrapolation of slice n) and D = F(n-1) - F(n-2):
- * inputs: planes[14][N][N] = Q/s (6), D/s_d (6), beam/s_b, ro(n-1)/s_r;
- * targets: planes[6][N][N] = (F(n) - Q) / t; guess: guess[f] = Q + t_f pred[f]. */
+/* Planes of the network from the step's fields (FP64, or FP32 with
+ * f32 != 0).  which = 0: input planes[S][S][32] = 4 x 4 means of
+ * F(n-1)/scale (6), (F(n-1) - F(n-2))/scale (6), beam(n)/scale,
+ * ro(n-1)/scale, zeros; 1: targets[S][S][6] = 4 x 4 means of
+ * (next - Q)/tscale with Q = 3 F(n-1) - 3 F(n-2) + F(n-3); 2: guess =
+ * Q + tscale y(i / 4, j / 4) on the grid (after lcode_unet_forward). */
 typedef struct {
-    const double *f1[6];      /* fields of slice n-1 (Ex Ey Ez Bx By Bz) */
-    const double *f2[6];      /* fields of slice n-2 */
-    const double *f3[6];      /* fields of slice n-3 */
-    const double *beam;       /* beam of slice n (NULL: beam_scalar) */
+    const void *f1[6], *f2[6], *f3[6];   /* slices n-1, n-2, n-3 */
+    const void *next[6];                 /* targets: the fields to learn */
+    const void *ro;                      /* plasma density of slice n-1 */
+    const double *beam;                  /* beam of slice n, or NULL */
     double beam_scalar;
-    const double *ro;         /* plasma density of slice n-1 */
-    const double *next[6];    /* targets only: fields of slice n */
-    float scale[14];          /* input scales */
-    float tscale[6];          /* target scales */
-    float *planes;            /* inputs / targets output */
-    const float *pred;        /* guess only: network output [6][N][N] */
-    double *guess[6];         /* guess only: output fields */
-    int32_t n;                /* N * N */
-} lcode_pred_io;
+    int32_t f32;                         /* grids are float arrays */
+    int32_t grid_steps;                  /* N; S = (N + 3) / 4 */
+    float scale[14], tscale[6];
+    uint16_t *planes;                    /* which 0 */
+    float *planes_f32;                   /* which 0 instead of planes: FP32
+                                            [S][S][16] (14 used) */
+    float *targets;                      /* which 1 */
+    void *guess[6];                      /* which 2 (the grids' type) */
+} lcode_unet_io_args;
 
-int lcode_pred_inputs(const lcode_pred_io *io, hipStream_t stream);
-int lcode_pred_targets(const lcode_pred_io *io, hipStream_t stream);
-int lcode_pred_guess(const lcode_pred_io *io, hipStream_t stream);
+int lcode_unet_io(const lcode_unet *net, const lcode_unet_io_args *io,
+                  int32_t which, hipStream_t stream);
 
 #ifdef __cplusplus
 }

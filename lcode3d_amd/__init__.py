@@ -8,4 +8,5 @@ from .lcode import (  # noqa: F401
     init, initial_deposition, make_coarse_plasma_grid, make_fine_plasma_grid,
     make_plasma, move_estimate_wo_fields, move_smart, profile_begin,
     profile_end, step, guided_residual, FieldExtrapolator, GuessBackoff, GuidedStepper,
+    predictor_fields, is_mixed, grid_dtype,
 )

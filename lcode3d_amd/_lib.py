@@ -22,6 +22,7 @@ LCODE_SOLVER_FFT64 = 0
 LCODE_SOLVER_FFT32 = 1
 LCODE_SOLVER_DENSE64 = 2
 LCODE_SOLVER_DENSE_BF16X3 = 3
+LCODE_SOLVER_MIXED = 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -60,23 +61,32 @@ STATE_FIELDS = ("x_offt", "y_offt", "px", "py", "pz",
 
 
 class LcodeState(ctypes.Structure):
+    """lcode_state, and lcode_state_mixed (the same layout of pointers)."""
     _fields_ = [(n, ctypes.c_void_p) for n in STATE_FIELDS]
 
 
 class LcodeGuess(ctypes.Structure):
+    """lcode_guess, and lcode_guess_mixed (the same layout)."""
     _fields_ = [("f", (ctypes.c_void_p * 6) * 3),
                 ("coef", ctypes.c_double * 3),
                 ("nterms", ctypes.c_int32)]
 
 
-class LcodePredIo(ctypes.Structure):
+class LcodeUnet(ctypes.Structure):
+    _fields_ = [("S", ctypes.c_int32)] + [
+        (n, ctypes.c_void_p) for n in ("params", "grad", "adam_m", "adam_v", "wpack",
+                                       "act", "y", "partial", "loss_part", "loss")]
+
+
+class LcodeUnetIo(ctypes.Structure):
     _fields_ = [("f1", ctypes.c_void_p * 6), ("f2", ctypes.c_void_p * 6),
-                ("f3", ctypes.c_void_p * 6),
-                ("beam", ctypes.c_void_p), ("beam_scalar", ctypes.c_double),
-                ("ro", ctypes.c_void_p), ("next", ctypes.c_void_p * 6),
+                ("f3", ctypes.c_void_p * 6), ("next", ctypes.c_void_p * 6),
+                ("ro", ctypes.c_void_p), ("beam", ctypes.c_void_p),
+                ("beam_scalar", ctypes.c_double), ("f32", ctypes.c_int32),
+                ("grid_steps", ctypes.c_int32),
                 ("scale", ctypes.c_float * 14), ("tscale", ctypes.c_float * 6),
-                ("planes", ctypes.c_void_p), ("pred", ctypes.c_void_p),
-                ("guess", ctypes.c_void_p * 6), ("n", ctypes.c_int32)]
+                ("planes", ctypes.c_void_p), ("planes_f32", ctypes.c_void_p),
+                ("targets", ctypes.c_void_p), ("guess", ctypes.c_void_p * 6)]
 
 
 class LcodeKernelTime(ctypes.Structure):
@@ -112,6 +122,11 @@ _SIGNATURES = {
                       [_vp, ctypes.POINTER(LcodeConfig)] + [_vp] * 8
                       + [ctypes.POINTER(LcodeVirtParams),
                          ctypes.POINTER(_vp), _vp]),
+    "lcode_plasma_sizes": (ctypes.c_int,
+                           [ctypes.c_int32] * 3 + [ctypes.POINTER(ctypes.c_int32)] * 2),
+    "lcode_make_plasma": (ctypes.c_int,
+                          [ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32]
+                          + [_vp] * 9 + [_vp]),
     "lcode_deposit_noalloc": (ctypes.c_int,
                               [ctypes.POINTER(LcodeConfig)] + [_vp] * 8
                               + [ctypes.POINTER(LcodeVirtParams),
@@ -138,31 +153,35 @@ _SIGNATURES = {
                            ctypes.POINTER(LcodeState), _vp, ctypes.c_double,
                            ctypes.POINTER(LcodeGuess), ctypes.c_double,
                            ctypes.POINTER(LcodeState), _vp]),
+    "lcode_step_mixed": (ctypes.c_int,
+                         [_vp, ctypes.POINTER(LcodeConfig),
+                          ctypes.POINTER(LcodeConst),
+                          ctypes.POINTER(LcodeVirtParams),
+                          ctypes.POINTER(LcodeState), _vp, ctypes.c_double,
+                          ctypes.POINTER(LcodeState), _vp]),
+    "lcode_step_guided_mixed": (ctypes.c_int,
+                                [_vp, ctypes.POINTER(LcodeConfig),
+                                 ctypes.POINTER(LcodeConst),
+                                 ctypes.POINTER(LcodeVirtParams),
+                                 ctypes.POINTER(LcodeState), _vp, ctypes.c_double,
+                                 ctypes.POINTER(LcodeGuess), ctypes.c_double,
+                                 ctypes.POINTER(LcodeState), _vp]),
     "lcode_step_guided_residual": (ctypes.c_int,
                                    [_vp, _vp, ctypes.POINTER(ctypes.c_double)]),
+    "lcode_step_guided_residual_async": (ctypes.c_int, [_vp, _vp, _vp]),
     "lcode_diag_zn": (ctypes.c_int,
                       [_vp, ctypes.POINTER(LcodeConfig), _vp, _vp, _vp]),
-    "lcode_pred_conv3x3_forward": (ctypes.c_int,
-                                   [_vp, ctypes.c_int32, _vp, _vp, _vp, ctypes.c_int32,
-                                    ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp]),
-    "lcode_pred_conv3x3_backward_input": (ctypes.c_int,
-                                          [_vp, ctypes.c_int32, _vp, _vp, ctypes.c_int32,
-                                           _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
-    "lcode_pred_conv3x3_backward_weight": (ctypes.c_int,
-                                           [_vp, ctypes.c_int32, _vp, ctypes.c_int32,
-                                            _vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
-    "lcode_pred_backward_weight_scratch": (ctypes.c_int64,
-                                           [ctypes.c_int32, ctypes.c_int32,
-                                            ctypes.c_int32, ctypes.c_int32]),
-    "lcode_pred_mse_grad": (ctypes.c_int,
-                            [_vp, _vp, ctypes.c_int64, _vp, _vp, _vp, _vp]),
-    "lcode_pred_adam": (ctypes.c_int,
-                        [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_float,
-                         ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                         ctypes.c_int32, _vp]),
-    "lcode_pred_inputs": (ctypes.c_int, [ctypes.POINTER(LcodePredIo), _vp]),
-    "lcode_pred_targets": (ctypes.c_int, [ctypes.POINTER(LcodePredIo), _vp]),
-    "lcode_pred_guess": (ctypes.c_int, [ctypes.POINTER(LcodePredIo), _vp]),
+    "lcode_unet_sizes": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "lcode_unet_tensor_offset": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32]),
+    "lcode_unet_param_offset": (ctypes.c_int64, [ctypes.c_int32]),
+    "lcode_unet_pack": (ctypes.c_int, [ctypes.POINTER(LcodeUnet), _vp]),
+    "lcode_unet_forward": (ctypes.c_int, [ctypes.POINTER(LcodeUnet), _vp, _vp]),
+    "lcode_unet_infer": (ctypes.c_int, [ctypes.POINTER(LcodeUnet), _vp, _vp]),
+    "lcode_unet_backward": (ctypes.c_int, [ctypes.POINTER(LcodeUnet), _vp, _vp, _vp]),
+    "lcode_unet_adam": (ctypes.c_int, [ctypes.POINTER(LcodeUnet), ctypes.c_int32,
+                                       ctypes.c_float, _vp]),
+    "lcode_unet_io": (ctypes.c_int, [ctypes.POINTER(LcodeUnet),
+                                     ctypes.POINTER(LcodeUnetIo), ctypes.c_int32, _vp]),
 }
 
 SYMBOLS = tuple(_SIGNATURES)

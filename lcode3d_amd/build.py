@@ -14,7 +14,7 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ("particles.hip", "fields.hip", "dense.hip", "diag.hip", "predictor.hip", "api.hip")
+SOURCES = ("particles.hip", "fields.hip", "dense.hip", "diag.hip", "unet.hip", "api.hip")
 OUTPUT = os.path.join(HERE, "liblcode3d_hip.so")
 ARCH = "gfx950"
 

@@ -32,8 +32,12 @@ struct lcode_workspace {
     double* costab = nullptr;      // quarter-wave cosine table
     double2* stage_tw = nullptr;   // static-plan stage twiddles
     float* costab32 = nullptr;
-    int fp32 = 0;                  // LCODE_SOLVER_FFT32
+    int fp32 = 0;                  // FP32 transforms (FFT32, MIXED)
     int solver = 0;                // LCODE_SOLVER_*
+    int grid32 = 0;                // set for the duration of a mixed step:
+                                   // the state grids are float arrays
+    double* zero64 = nullptr;      // N * N FP64 zeros (MIXED: the RHS's
+                                   // stand-in for a missing beam array)
     DenseGeom dense{};             // LCODE_SOLVER_DENSE*: matrices, scratch
     void* tab_mixed = nullptr;     // [M][N]   (DST index, DCT index)
     void* tab_mixed_t = nullptr;   // [N][M]
@@ -181,7 +185,8 @@ int check_cfg(const lcode_config* c) {
                   "plasma_fineness too large for this build's deposition tiles");
     LCODE_REQUIRE(c->solver == LCODE_SOLVER_FFT64 || c->solver == LCODE_SOLVER_FFT32 ||
                       c->solver == LCODE_SOLVER_DENSE64 ||
-                      c->solver == LCODE_SOLVER_DENSE_BF16X3,
+                      c->solver == LCODE_SOLVER_DENSE_BF16X3 ||
+                      c->solver == LCODE_SOLVER_MIXED,
                   "unknown solver");
     return LCODE_OK;
 }
@@ -288,7 +293,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     SolverGeom& g = ws->geom;
     g.N = N;
     ws->solver = cfg->solver;
-    ws->fp32 = cfg->solver == LCODE_SOLVER_FFT32;
+    ws->fp32 = cfg->solver == LCODE_SOLVER_FFT32 || cfg->solver == LCODE_SOLVER_MIXED;
     g.fp32 = ws->fp32;
     const int cplx = ws->fp32 ? (int)sizeof(float2) : (int)sizeof(double2);
     const int L = 2 * (N - 1);
@@ -489,6 +494,10 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         if ((rc = dev_alloc(ws, (void**)&ws->guess[k], n2 * sizeof(double)))) return fail(rc);
     }
     if ((rc = dev_alloc(ws, (void**)&ws->resid, 2 * sizeof(double)))) return fail(rc);
+    if (cfg->solver == LCODE_SOLVER_MIXED) {
+        if ((rc = dev_alloc(ws, (void**)&ws->zero64, n2 * sizeof(double)))) return fail(rc);
+        LCODE_HIP(hipMemset(ws->zero64, 0, n2 * sizeof(double)));
+    }
     {
         // last FFT pass: ceil(N / (2 P)) blocks per field, six fields
         const int per_block = 2 * ws->geom.P;
@@ -661,6 +670,7 @@ static int do_push(lcode_workspace* ws, const lcode_config* cfg,
     a.fault = ws->fault;
     a.max_cells = ws->spill_ctr + 2 + ws->bound_slot;
     a.host_flag = ws->host_flag;
+    a.grid32 = ws->grid32;
     a.gate = gate;
     return launch_push(a, stream, &ws->prof);
 }
@@ -724,6 +734,7 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
                       bool adaptive_halo = false) {
     DepositArgs a = deposit_args(cfg, ro_initial, xo, yo, m, q, px, py, pz,
                                  vp, out);
+    a.grid32 = ws->grid32;
     a.fault = ws->fault;
     a.spill = ws->spill;
     a.host_flag = ws->host_flag;
@@ -742,6 +753,45 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     if (rc == LCODE_OK) ws->spill_slot = 1 - ws->spill_slot;
     if (rc == LCODE_OK && adaptive_halo) ws->bound_slot = 1 - ws->bound_slot;
     return rc;
+}
+
+int lcode_plasma_sizes(int32_t steps, int32_t coarseness, int32_t fineness,
+                       int32_t* nc, int32_t* nf) {
+    LCODE_REQUIRE(nc && nf, "NULL output");
+    LCODE_REQUIRE(steps > 0 && coarseness > 0 && fineness > 0,
+                  "steps, coarseness and fineness must be positive");
+    const int hc = steps / (coarseness * 2), hf = steps / 2 * fineness;
+    LCODE_REQUIRE(hc >= 1 && hf >= 1, "too few plasma cells for this coarseness");
+    *nc = 2 * hc - 1;
+    *nf = fineness % 2 ? 2 * hf - 1 : 2 * hf;
+    return LCODE_OK;
+}
+
+int lcode_make_plasma(int32_t steps, double cell_size, int32_t coarseness,
+                      int32_t fineness, double* x_init, double* y_init,
+                      double* m, double* q, double* influence_prev,
+                      double* influence_next, double* fine_grid,
+                      int64_t* indices_prev, int64_t* indices_next,
+                      hipStream_t stream) {
+    int nc = 0, nf = 0;
+    int rc = lcode_plasma_sizes(steps, coarseness, fineness, &nc, &nf);
+    if (rc) return rc;
+    LCODE_REQUIRE(cell_size > 0, "cell_size must be positive");
+    LCODE_REQUIRE(x_init && y_init && m && q && influence_prev && influence_next &&
+                      fine_grid && indices_prev && indices_next,
+                  "NULL output array");
+    PlasmaArgs a{};
+    a.coarseness = coarseness;
+    a.fineness = fineness;
+    a.hc = steps / (coarseness * 2);
+    a.hf = steps / 2 * fineness;
+    a.nc = nc;
+    a.nf = nf;
+    a.cell = cell_size;
+    a.x_init = x_init; a.y_init = y_init; a.m = m; a.q = q;
+    a.ip = influence_prev; a.in = influence_next; a.fg = fine_grid;
+    a.P = indices_prev; a.Nn = indices_next;
+    return launch_make_plasma(a, stream);
 }
 
 int lcode_deposit(lcode_workspace* ws, const lcode_config* cfg,
@@ -931,15 +981,21 @@ static int field_pass(lcode_workspace* ws, const lcode_config* cfg,
     for (int k = 0; k < 4; ++k) b.in.Ftrick[k] = trick[k];
     b.in.beam = beam_ro;
     b.in.beam_scalar = beam_scalar;
+    b.in.zero64 = ws->zero64;
     b.variantA = cfg->variant_A ? 1 : 0;
+    b.grid32 = ws->grid32;
     return run_solve(ws, b, stream);
 }
 
-int lcode_step(lcode_workspace* ws, const lcode_config* cfg,
-               const lcode_const* c, const lcode_virt_params* vp,
-               const lcode_state* prev, const double* beam_ro,
-               double beam_scalar, const lcode_state* out,
-               hipStream_t stream) {
+}  // extern "C"
+
+// The composition of the xi step shared by lcode_step (FP64 grids) and
+// lcode_step_mixed (float grids, ws->grid32 set by the caller).
+static int step_impl(lcode_workspace* ws, const lcode_config* cfg,
+                     const lcode_const* c, const lcode_virt_params* vp,
+                     const lcode_state* prev, const double* beam_ro,
+                     double beam_scalar, const lcode_state* out,
+                     hipStream_t stream) {
     int rc = check_ws(ws, cfg);
     if (rc) return rc;
     LCODE_REQUIRE(c && vp && prev && out, "NULL struct");
@@ -1004,12 +1060,12 @@ int lcode_step(lcode_workspace* ws, const lcode_config* cfg,
                       Gate{nullptr, 0.}, true);
 }
 
-int lcode_step_guided(lcode_workspace* ws, const lcode_config* cfg,
-                      const lcode_const* c, const lcode_virt_params* vp,
-                      const lcode_state* prev, const double* beam_ro,
-                      double beam_scalar, const lcode_guess* guess,
-                      double threshold, const lcode_state* out,
-                      hipStream_t stream) {
+static int guided_impl(lcode_workspace* ws, const lcode_config* cfg,
+                       const lcode_const* c, const lcode_virt_params* vp,
+                       const lcode_state* prev, const double* beam_ro,
+                       double beam_scalar, const lcode_guess* guess,
+                       double threshold, const lcode_state* out,
+                       hipStream_t stream) {
     int rc = check_ws(ws, cfg);
     if (rc) return rc;
     LCODE_REQUIRE(c && vp && prev && out && guess, "NULL struct");
@@ -1051,6 +1107,7 @@ int lcode_step_guided(lcode_workspace* ws, const lcode_config* cfg,
     }
     ga.acc_zero = ws->resid;
     ga.n = cfg->grid_steps * cfg->grid_steps;
+    ga.grid32 = ws->grid32;
     if ((rc = launch_guess(ga, stream, &ws->prof))) return rc;
     // corrector (lcode.py:884-911) on the guess, residual accumulated
     if ((rc = do_push(ws, cfg, c->m, c->q, c->x_init, c->y_init,
@@ -1101,6 +1158,88 @@ int lcode_step_guided(lcode_workspace* ws, const lcode_config* cfg,
                       Gate{nullptr, 0.}, true);
 }
 
+// ws->grid32 for the duration of one call
+struct Grid32Scope {
+    lcode_workspace* ws;
+    Grid32Scope(lcode_workspace* w, int on) : ws(w) { if (ws) ws->grid32 = on; }
+    ~Grid32Scope() { if (ws) ws->grid32 = 0; }
+};
+
+// A mixed state as the FP64-typed struct the shared composition takes
+// (the grid pointers keep pointing to floats; ws->grid32 says so).
+static lcode_state as_state(const lcode_state_mixed* m) {
+    lcode_state s{};
+    s.x_offt = m->x_offt; s.y_offt = m->y_offt;
+    s.px = m->px; s.py = m->py; s.pz = m->pz;
+    s.Ex = reinterpret_cast<double*>(m->Ex); s.Ey = reinterpret_cast<double*>(m->Ey);
+    s.Ez = reinterpret_cast<double*>(m->Ez); s.Bx = reinterpret_cast<double*>(m->Bx);
+    s.By = reinterpret_cast<double*>(m->By); s.Bz = reinterpret_cast<double*>(m->Bz);
+    s.ro = reinterpret_cast<double*>(m->ro); s.jx = reinterpret_cast<double*>(m->jx);
+    s.jy = reinterpret_cast<double*>(m->jy); s.jz = reinterpret_cast<double*>(m->jz);
+    return s;
+}
+
+extern "C" {
+
+int lcode_step(lcode_workspace* ws, const lcode_config* cfg,
+               const lcode_const* c, const lcode_virt_params* vp,
+               const lcode_state* prev, const double* beam_ro,
+               double beam_scalar, const lcode_state* out,
+               hipStream_t stream) {
+    LCODE_REQUIRE(cfg && cfg->solver != LCODE_SOLVER_MIXED,
+                  "LCODE_SOLVER_MIXED takes FP32 grids: use lcode_step_mixed");
+    Grid32Scope g(ws, 0);
+    return step_impl(ws, cfg, c, vp, prev, beam_ro, beam_scalar, out, stream);
+}
+
+int lcode_step_guided(lcode_workspace* ws, const lcode_config* cfg,
+                      const lcode_const* c, const lcode_virt_params* vp,
+                      const lcode_state* prev, const double* beam_ro,
+                      double beam_scalar, const lcode_guess* guess,
+                      double threshold, const lcode_state* out,
+                      hipStream_t stream) {
+    LCODE_REQUIRE(cfg && cfg->solver != LCODE_SOLVER_MIXED,
+                  "LCODE_SOLVER_MIXED takes FP32 grids: use lcode_step_guided_mixed");
+    Grid32Scope g(ws, 0);
+    return guided_impl(ws, cfg, c, vp, prev, beam_ro, beam_scalar, guess,
+                       threshold, out, stream);
+}
+
+int lcode_step_mixed(lcode_workspace* ws, const lcode_config* cfg,
+                     const lcode_const* c, const lcode_virt_params* vp,
+                     const lcode_state_mixed* prev, const double* beam_ro,
+                     double beam_scalar, const lcode_state_mixed* out,
+                     hipStream_t stream) {
+    LCODE_REQUIRE(cfg && cfg->solver == LCODE_SOLVER_MIXED,
+                  "lcode_step_mixed needs cfg->solver == LCODE_SOLVER_MIXED");
+    LCODE_REQUIRE(prev && out, "NULL state");
+    const lcode_state p = as_state(prev), o = as_state(out);
+    Grid32Scope g(ws, 1);
+    return step_impl(ws, cfg, c, vp, &p, beam_ro, beam_scalar, &o, stream);
+}
+
+int lcode_step_guided_mixed(lcode_workspace* ws, const lcode_config* cfg,
+                            const lcode_const* c, const lcode_virt_params* vp,
+                            const lcode_state_mixed* prev, const double* beam_ro,
+                            double beam_scalar, const lcode_guess_mixed* guess,
+                            double threshold, const lcode_state_mixed* out,
+                            hipStream_t stream) {
+    LCODE_REQUIRE(cfg && cfg->solver == LCODE_SOLVER_MIXED,
+                  "lcode_step_guided_mixed needs cfg->solver == LCODE_SOLVER_MIXED");
+    LCODE_REQUIRE(prev && out && guess, "NULL state or guess");
+    const lcode_state p = as_state(prev), o = as_state(out);
+    lcode_guess gd{};
+    gd.nterms = guess->nterms;
+    for (int t = 0; t < 3; ++t) {
+        gd.coef[t] = guess->coef[t];
+        for (int f = 0; f < 6; ++f)
+            gd.f[t][f] = reinterpret_cast<const double*>(guess->f[t][f]);
+    }
+    Grid32Scope g(ws, 1);
+    return guided_impl(ws, cfg, c, vp, &p, beam_ro, beam_scalar, &gd, threshold,
+                       &o, stream);
+}
+
 int lcode_step_guided_residual(lcode_workspace* ws, hipStream_t stream,
                                double* residual) {
     LCODE_REQUIRE(ws && residual, "NULL argument");
@@ -1110,6 +1249,14 @@ int lcode_step_guided_residual(lcode_workspace* ws, hipStream_t stream,
     LCODE_HIP(hipStreamSynchronize(stream));
     *residual = host[1] > 0 ? std::sqrt(host[0] / host[1])
                             : (host[0] > 0 ? INFINITY : 0.);
+    return LCODE_OK;
+}
+
+int lcode_step_guided_residual_async(lcode_workspace* ws, double* dst,
+                                     hipStream_t stream) {
+    LCODE_REQUIRE(ws && dst, "NULL argument");
+    LCODE_HIP(hipMemcpyAsync(dst, ws->resid, 2 * sizeof(double),
+                             hipMemcpyDefault, stream));
     return LCODE_OK;
 }
 

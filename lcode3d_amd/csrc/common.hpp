@@ -4,6 +4,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <cstdint>
 #include <string>
 
@@ -132,10 +134,12 @@ __device__ __forceinline__ int xcd_logical(int lin, int total) {
 __device__ __forceinline__ int stencil_di(int k) { return (k % 3) - 1; }
 __device__ __forceinline__ int stencil_dj(int k) { return 1 - (k / 3); }
 
-struct Tsc {
+template <typename W>
+struct TscT {
     int i, j;      // centre cell, clamped to [1, N-2]
-    double w[9];   // wMP w0P wPP wM0 w00 wP0 wMM w0M wPM
+    W w[9];        // wMP w0P wPP wM0 w00 wP0 wMM w0M wPM
 };
+using Tsc = TscT<double>;
 
 // Cell index of the coordinate x: floor(x / h + .5) + N / 2.
 __device__ __forceinline__ int cell_of(double x, int N, double h) {
@@ -150,19 +154,24 @@ __device__ __forceinline__ int cell_of(double x, int N, double h) {
 // deposition; a 1-ulp change of x_h moves no weight measurably: the TSC
 // weights are continuous across cell edges).
 // The nine weights from the in-cell offsets (x_loc, y_loc) in [-.5, .5).
-__device__ __forceinline__ void tsc_w(double x_loc, double y_loc, double* w) {
-    const double wx0 = .75 - x_loc * x_loc, wy0 = .75 - y_loc * y_loc;
-    const double wxP = (.5 + x_loc) * (.5 + x_loc) / 2;
-    const double wyP = (.5 + y_loc) * (.5 + y_loc) / 2;
-    const double wxM = (.5 - x_loc) * (.5 - x_loc) / 2;
-    const double wyM = (.5 - y_loc) * (.5 - y_loc) / 2;
+template <typename W>
+__device__ __forceinline__ void tsc_w(W x_loc, W y_loc, W* w) {
+    const W h = W(.5);
+    const W wx0 = W(.75) - x_loc * x_loc, wy0 = W(.75) - y_loc * y_loc;
+    const W wxP = (h + x_loc) * (h + x_loc) / 2;
+    const W wyP = (h + y_loc) * (h + y_loc) / 2;
+    const W wxM = (h - x_loc) * (h - x_loc) / 2;
+    const W wyM = (h - y_loc) * (h - y_loc) / 2;
     w[0] = wxM * wyP; w[1] = wx0 * wyP; w[2] = wxP * wyP;
     w[3] = wxM * wy0; w[4] = wx0 * wy0; w[5] = wxP * wy0;
     w[6] = wxM * wyM; w[7] = wx0 * wyM; w[8] = wxP * wyM;
 }
 
+// The cell and the in-cell offsets are found in FP64 in every precision
+// (W = float only rounds the offsets before the weights are formed).
+template <typename W>
 __device__ __forceinline__ void tsc_weights(double x, double y, int N,
-                                            double h, Tsc& t, int* fault,
+                                            double h, TscT<W>& t, int* fault,
                                             double inv_h = 0.,
                                             double* xl = nullptr,
                                             double* yl = nullptr) {
@@ -178,18 +187,33 @@ __device__ __forceinline__ void tsc_weights(double x, double y, int N,
     }
     t.i = i;
     t.j = j;
-    tsc_w(x_loc, y_loc, t.w);
+    tsc_w<W>((W)x_loc, (W)y_loc, t.w);
     if (xl) *xl = x_loc;
     if (yl) *yl = y_loc;
 }
 
 // 9-point gather in the reference's summation order (lcode.py:421-430).
-__device__ __forceinline__ double interp9(const double* __restrict__ a,
-                                          int N, const Tsc& t) {
-    const double* c = a + (size_t)t.i * N + t.j;
-    return c[-N + 1] * t.w[0] + c[1] * t.w[1] + c[N + 1] * t.w[2] +
-           c[-N] * t.w[3] + c[0] * t.w[4] + c[N] * t.w[5] +
-           c[-N - 1] * t.w[6] + c[-1] * t.w[7] + c[N - 1] * t.w[8];
+// G is the grid's storage type (float grids: each value converts exactly
+// to double; the sum is FP64 in both).
+template <typename G>
+__device__ __forceinline__ double interp9(const G* __restrict__ a, int N,
+                                          const Tsc& t) {
+    const G* c = a + (size_t)t.i * N + t.j;
+    return (double)c[-N + 1] * t.w[0] + (double)c[1] * t.w[1] +
+           (double)c[N + 1] * t.w[2] + (double)c[-N] * t.w[3] +
+           (double)c[0] * t.w[4] + (double)c[N] * t.w[5] +
+           (double)c[-N - 1] * t.w[6] + (double)c[-1] * t.w[7] +
+           (double)c[N - 1] * t.w[8];
+}
+
+// A grid pointer of the argument structs (declared double) as the grid's
+// storage type: the FP32-grid form of the step passes float grids through
+// the same members.
+template <typename G>
+__device__ __forceinline__ G* gp(double* p) { return reinterpret_cast<G*>(p); }
+template <typename G>
+__device__ __forceinline__ const G* gp(const double* p) {
+    return reinterpret_cast<const G*>(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -210,6 +234,7 @@ struct PushArgs {
     // non-finite (or beyond kFarCells) particle state: flagged in
     // host-visible memory (NULL: not reported)
     int* host_flag;
+    int grid32;                    // the six field grids are float arrays
     Gate gate;
 };
 
@@ -249,12 +274,28 @@ struct DepositArgs {
                                    // pushes: set to 0 by the deposition
     int staged;                    // coarse rectangle of a tile fits LDS
     int group;                     // fine particles per group side (fineness)
+    int grid32;                    // ro jx jy jz are float arrays (the
+                                   // arithmetic and ro_initial stay FP64)
     Gate gate;
 };
 
 int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof);
 int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof);
 int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof);
+// make_plasma (lcode.py:452-573) on the device.
+constexpr double kElectronMass = 1.;      // lcode.py:47
+constexpr double kElectronCharge = -1.;   // lcode.py:46
+struct PlasmaArgs {
+    int coarseness, fineness;
+    int hc, hf;                    // half-grid counts of the two grids
+    int nc, nf;                    // coarse and fine points per axis
+    double cell;                   // cell size
+    double *x_init, *y_init, *m, *q;          // nc * nc
+    double *ip, *in, *fg;                     // nf each
+    int64_t *P, *Nn;                          // nf each
+};
+int launch_make_plasma(const PlasmaArgs& a, hipStream_t s);
+
 // Sets n ints at p to 0 with a one-block kernel (no memset node).
 int launch_zero_ints(int* p, int n, hipStream_t s);
 
@@ -280,6 +321,9 @@ struct RhsInputs {
     const double* beam;                   // NULL: beam_scalar everywhere
     double beam_scalar;
     double trick, h, dxi;
+    // FP32-grid form: an FP64 grid of N * N zeros read in place of a
+    // missing beam array (the grid members above then point to floats)
+    const double* zero64;
 };
 
 // Right-hand sides (lcode.py:166-168, :234-268, :339-343), FP64.
@@ -287,69 +331,80 @@ struct RhsInputs {
 // Without averaging a source is averaged with itself, (r + r) / 2 == r
 // exactly, and without a beam array the beam term is x * 0 + beam_scalar:
 // the same values as the conditional forms, with no branch between loads.
+// G: storage type of the grid members (float in the FP32-grid form: the
+// values convert exactly and the arithmetic stays FP64).  The beam is FP64.
+template <typename G = double>
 __device__ __forceinline__ double beam_at(const RhsInputs& in, size_t idx) {
-    const double* b = in.beam ? in.beam : in.ro;
+    const double* dummy = std::is_same<G, double>::value ? in.ro : in.zero64;
+    const double* b = in.beam ? in.beam : dummy;
     return b[idx] * (in.beam ? 1. : 0.) + (in.beam ? 0. : in.beam_scalar);
 }
 
+template <typename G = double>
 __device__ __forceinline__ double rho_src(const RhsInputs& in, size_t idx) {
-    const double* prev = in.ro_prev ? in.ro_prev : in.ro;
-    const double r = (in.ro[idx] + prev[idx]) / 2;        // lcode.py:862
-    return r + beam_at(in, idx);
+    const G* ro = gp<G>(in.ro);
+    const G* prev = in.ro_prev ? gp<G>(in.ro_prev) : ro;
+    const double r = ((double)ro[idx] + (double)prev[idx]) / 2;   // lcode.py:862
+    return r + beam_at<G>(in, idx);
 }
 
+template <typename G = double>
 __device__ __forceinline__ double jz_src(const RhsInputs& in, size_t idx) {
-    const double* prev = in.jz_prev ? in.jz_prev : in.jz;
-    const double r = (in.jz[idx] + prev[idx]) / 2;        // lcode.py:863
-    return r + beam_at(in, idx);
+    const G* jz = gp<G>(in.jz);
+    const G* prev = in.jz_prev ? gp<G>(in.jz_prev) : jz;
+    const double r = ((double)jz[idx] + (double)prev[idx]) / 2;   // lcode.py:863
+    return r + beam_at<G>(in, idx);
 }
 
 // The right-hand side of field `kind` at (i, j).  Outside the interior the
 // derivative terms are zero; their neighbour offsets then collapse to the
 // point itself, so every load stays in bounds for any (i, j) of the grid.
-template <int KIND>
+template <int KIND, typename G = double>
 __device__ __forceinline__ double rhs_value_k(int i, int j, int N,
                                               const RhsInputs& in) {
     const size_t idx = (size_t)i * N + j;
     const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
     const size_t oi = inner ? (size_t)N : 0, oj = inner ? 1 : 0;
     const double h2 = in.h * 2;
+    const G *jx = gp<G>(in.jx), *jy = gp<G>(in.jy);
+    const G *jxp = gp<G>(in.jx_prev), *jyp = gp<G>(in.jy_prev);
     if constexpr (KIND == F_EX) {
-        const double d = (rho_src(in, idx + oi) - rho_src(in, idx - oi)) / h2;
-        const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
-        return -(((inner ? d : 0.) - dj) - in.Ftrick[0][idx] * in.trick);
+        const double d = (rho_src<G>(in, idx + oi) - rho_src<G>(in, idx - oi)) / h2;
+        const double dj = ((double)jxp[idx] - (double)jx[idx]) / in.dxi;
+        return -(((inner ? d : 0.) - dj) - (double)gp<G>(in.Ftrick[0])[idx] * in.trick);
     } else if constexpr (KIND == F_EY) {
-        const double d = (rho_src(in, idx + oj) - rho_src(in, idx - oj)) / h2;
-        const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
-        return -(((inner ? d : 0.) - dj) - in.Ftrick[1][idx] * in.trick);
+        const double d = (rho_src<G>(in, idx + oj) - rho_src<G>(in, idx - oj)) / h2;
+        const double dj = ((double)jyp[idx] - (double)jy[idx]) / in.dxi;
+        return -(((inner ? d : 0.) - dj) - (double)gp<G>(in.Ftrick[1])[idx] * in.trick);
     } else if constexpr (KIND == F_BX) {
-        const double d = (jz_src(in, idx + oj) - jz_src(in, idx - oj)) / h2;
-        const double dj = (in.jy_prev[idx] - in.jy[idx]) / in.dxi;
-        return +(((inner ? d : 0.) - dj) + in.Ftrick[2][idx] * in.trick);
+        const double d = (jz_src<G>(in, idx + oj) - jz_src<G>(in, idx - oj)) / h2;
+        const double dj = ((double)jyp[idx] - (double)jy[idx]) / in.dxi;
+        return +(((inner ? d : 0.) - dj) + (double)gp<G>(in.Ftrick[2])[idx] * in.trick);
     } else if constexpr (KIND == F_BY) {
-        const double d = (jz_src(in, idx + oi) - jz_src(in, idx - oi)) / h2;
-        const double dj = (in.jx_prev[idx] - in.jx[idx]) / in.dxi;
-        return -(((inner ? d : 0.) - dj) - in.Ftrick[3][idx] * in.trick);
+        const double d = (jz_src<G>(in, idx + oi) - jz_src<G>(in, idx - oi)) / h2;
+        const double dj = ((double)jxp[idx] - (double)jx[idx]) / in.dxi;
+        return -(((inner ? d : 0.) - dj) - (double)gp<G>(in.Ftrick[3])[idx] * in.trick);
     } else if constexpr (KIND == F_EZ) {
         // the interior only carries this right-hand side (zero elsewhere)
-        return -((in.jx[idx + oi] - in.jx[idx - oi]) +
-                 (in.jy[idx + oj] - in.jy[idx - oj])) / h2;
+        return -(((double)jx[idx + oi] - (double)jx[idx - oi]) +
+                 ((double)jy[idx + oj] - (double)jy[idx - oj])) / h2;
     } else {  // F_BZ
-        const double djx_dy = in.jx[idx + oj] - in.jx[idx - oj];
-        const double djy_dx = in.jy[idx + oi] - in.jy[idx - oi];
+        const double djx_dy = (double)jx[idx + oj] - (double)jx[idx - oj];
+        const double djy_dx = (double)jy[idx + oi] - (double)jy[idx - oi];
         return -((inner ? djx_dy : 0.) - (inner ? djy_dx : 0.)) / h2;
     }
 }
 
+template <typename G = double>
 __device__ __forceinline__ double rhs_value_dev(int kind, int i, int j, int N,
                                                 const RhsInputs& in) {
     switch (kind) {
-        case F_EX: return rhs_value_k<F_EX>(i, j, N, in);
-        case F_EY: return rhs_value_k<F_EY>(i, j, N, in);
-        case F_BX: return rhs_value_k<F_BX>(i, j, N, in);
-        case F_BY: return rhs_value_k<F_BY>(i, j, N, in);
-        case F_EZ: return rhs_value_k<F_EZ>(i, j, N, in);
-        default: return rhs_value_k<F_BZ>(i, j, N, in);
+        case F_EX: return rhs_value_k<F_EX, G>(i, j, N, in);
+        case F_EY: return rhs_value_k<F_EY, G>(i, j, N, in);
+        case F_BX: return rhs_value_k<F_BX, G>(i, j, N, in);
+        case F_BY: return rhs_value_k<F_BY, G>(i, j, N, in);
+        case F_EZ: return rhs_value_k<F_EZ, G>(i, j, N, in);
+        default: return rhs_value_k<F_BZ, G>(i, j, N, in);
     }
 }
 
@@ -382,6 +437,8 @@ struct SolveBatch {
     const double* guess[6];   // guided step: the caller's field guess ...
     double* resid;            // ... and {sum (F - guess)^2, sum F^2}
     double* resid_part;       // FFT solver: 2 partial sums per last-pass block
+    int grid32;               // every N x N grid above (sources, prev, out,
+                              // averages, guess) is a float array
 };
 
 // guess[f] = sum_t c[t] h[t][f]; avg[f] = (guess[f] + prev[f]) / 2 over n
@@ -395,6 +452,7 @@ struct GuessArgs {
     double* avg[6];
     double* acc_zero;
     int n;
+    int grid32;                // the field arrays are float arrays
 };
 
 int launch_guess(const GuessArgs& p, hipStream_t s, Profiler* prof);
@@ -484,5 +542,71 @@ int solver_tw_count(int L);   // static plan: twiddle table entries
 int solver_threads_per_pair(int L, bool fp32, bool static_plan);   // threads of one line pair
 int solver_static_radix(int rem);   // static plan: radix of the next stage
 int solver_lds_bytes(const SolverGeom& g);
+
+
+// Learned predictor (unet.hip) ----------------------------------------------
+
+// 3 x 3 convolution over bfloat16 [H][W][32] tensors: nin input chunks of
+// 32 channels, cout_pad = 16, 32 or 64 output channels (cout real).
+struct UnetConvArgs {
+    const uint16_t* in[2];    // [H][W][32] each (in[1] with nin == 2)
+    const uint16_t* w;        // packed [9][cout_pad][32 nin] bfloat16
+    const float* bias;        // [cout] or NULL
+    int H, W, nin, cout, cout_pad;
+    int relu;
+    const uint16_t* mask;     // zero where mask <= 0 ([H][W][32]) or NULL
+    uint16_t* out[2];         // channels 0..31, 32..63 ([H][W][32] bf16)
+    float* out_f32;           // or FP32 [H][W][16] (cout_pad == 16)
+    // how input chunk k is read: 0 as is; 1 the 2 x 2 mean of a finer
+    // [in_h][in_w][32] tensor; 2 nearest upsampling of a coarser one
+    int in_mode[2], in_h[2], in_w[2];
+};
+
+// Weight gradient of one convolution: per-block partials of
+// sum_p g[p][co] x[p + off][ci] and sum_p g[p][co].
+struct UnetWgradArgs {
+    const uint16_t* in[2];    // the layer input, nin chunks
+    const uint16_t* g;        // output gradient [H][W][32] (cout_pad used)
+    int H, W, nin, cout_pad;
+    float* partial;           // [tiles][9 cout_pad 32 nin + cout_pad]
+};
+
+// Planes of the network from the step's fields (FP64 or FP32 grids).
+struct UnetIoArgs {
+    const void *f1[6], *f2[6], *f3[6];   // fields of slices n-1, n-2, n-3
+    const void* next[6];                 // targets: the fields to predict
+    const void* ro;                      // plasma density of slice n-1
+    const double* beam;                  // beam of slice n, or NULL
+    double beam_scalar;
+    int f32;                             // the grids are float arrays
+    int N, S;                            // grid and level-0 sizes
+    float scale[14], tscale[6];
+    uint16_t* planes;                    // inputs [S][S][32] bf16
+    float* planes_f32;                   // or FP32 [S][S][16] (14 used)
+    float* targets;                      // targets [S][S][6]
+    const float* y;                      // network output [S][S][16]
+    void* guess[6];                      // guess output (grid type)
+};
+
+int unet_conv_lds_bytes(int nin, int mt);
+int unet_tiles(int H, int W);
+int launch_unet_conv(const UnetConvArgs& a, hipStream_t s);
+int launch_unet_wgrad(const UnetWgradArgs& a, int cout, int cin, float* grad,
+                      hipStream_t s);
+int launch_unet_pack(const float* w, int cout, int cin, int cout_pad, int cinp,
+                     uint16_t* fwd, uint16_t* bwd, hipStream_t s);
+int launch_unet_adam(float* p, const float* g, float* m, float* v, int n,
+                     float lr, int t, hipStream_t s);
+int launch_unet_pool(const uint16_t* x, int H, int W, uint16_t* y, hipStream_t s);
+int launch_unet_up(const uint16_t* x, int h, int w, uint16_t* y, int H, int W,
+                   hipStream_t s);
+int launch_unet_up_t(const uint16_t* g, int H, int W, const uint16_t* mask,
+                     uint16_t* out, int h, int w, hipStream_t s);
+int launch_unet_pool_t(const uint16_t* g, int h, int w, const uint16_t* skip,
+                       const uint16_t* mask, uint16_t* out, int H, int W,
+                       hipStream_t s);
+int launch_unet_mse(const float* y, const float* t, int P, uint16_t* g,
+                    float* part, float* loss, hipStream_t s);
+int launch_unet_io(const UnetIoArgs& a, int which, hipStream_t s);
 
 }  // namespace lcode

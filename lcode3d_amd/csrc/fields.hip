@@ -455,15 +455,16 @@ __device__ __forceinline__ void get_ext(const V* z, int xf, int k, double& a,
     }
 }
 
+template <typename G>
 __device__ __forceinline__ double rhs_value(int kind, int i, int j, int N,
                                             const RhsInputs& in) {
-    return rhs_value_dev(kind, i, j, N, in);
+    return rhs_value_dev<G>(kind, i, j, N, in);
 }
 
 // Right-hand sides of rows ia and ib at columns j0 + t + u Tp, u < K, as
 // straight-line code: columns clamped to [j0, j0 + len), so all loads of
 // the thread are issued before their values are used.
-template <int KIND, int K>
+template <int KIND, int K, typename G>
 __device__ __forceinline__ void rhs_rows_k(int ia, int ib, int j0, int len,
                                            int t, int Tp, int N,
                                            const RhsInputs& in, double* a,
@@ -471,23 +472,23 @@ __device__ __forceinline__ void rhs_rows_k(int ia, int ib, int j0, int len,
 #pragma unroll
     for (int u = 0; u < K; ++u) {
         const int j = j0 + min(t + u * Tp, len - 1);
-        a[u] = rhs_value_k<KIND>(ia, j, N, in);
-        c[u] = rhs_value_k<KIND>(ib, j, N, in);
+        a[u] = rhs_value_k<KIND, G>(ia, j, N, in);
+        c[u] = rhs_value_k<KIND, G>(ib, j, N, in);
     }
 }
 
-template <int K>
+template <int K, typename G>
 __device__ __forceinline__ void rhs_rows(int kind, int ia, int ib, int j0,
                                          int len, int t, int Tp, int N,
                                          const RhsInputs& in, double* a,
                                          double* c) {
     switch (kind) {
-        case F_EX: rhs_rows_k<F_EX, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        case F_EY: rhs_rows_k<F_EY, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        case F_BX: rhs_rows_k<F_BX, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        case F_BY: rhs_rows_k<F_BY, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        case F_EZ: rhs_rows_k<F_EZ, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        default: rhs_rows_k<F_BZ, K>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_EX: rhs_rows_k<F_EX, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_EY: rhs_rows_k<F_EY, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_BX: rhs_rows_k<F_BX, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_BY: rhs_rows_k<F_BY, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_EZ: rhs_rows_k<F_EZ, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        default: rhs_rows_k<F_BZ, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
     }
 }
 
@@ -547,7 +548,8 @@ __device__ __forceinline__ BlockPos xcd_block_rows() {
 #define LCODE_PASS_C_ATTR
 #endif
 
-template <int L_, bool TW_LDS, typename S>
+// G: storage type of the source grids (FP64 arithmetic in both).
+template <int L_, bool TW_LDS, typename S, typename G>
 __global__ void __launch_bounds__(1024) LCODE_PASS_A_ATTR
 pass_a_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
@@ -568,7 +570,7 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
         // rows without data read row rowsA0 (in bounds) and are dropped
         constexpr int kPer = points_per_thread(L_, std::is_same<S, float>::value);
         double a[kPer], c[kPer];
-        rhs_rows<kPer>(fp.kind, fp.rowsA0 + (va ? ra : 0), fp.rowsA0 + (vb ? rb : 0),
+        rhs_rows<kPer, G>(fp.kind, fp.rowsA0 + (va ? ra : 0), fp.rowsA0 + (vb ? rb : 0),
                        fp.colA0, fp.lenA, t, Tp, N, b.in, a, c);
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
@@ -578,8 +580,8 @@ pass_a_kernel(SolverGeom g, SolveBatch b) {
     } else {
         for (int n = t; n < fp.lenA; n += Tp) {
             const int j = fp.colA0 + n;
-            const double a = va ? rhs_value(fp.kind, fp.rowsA0 + ra, j, N, b.in) : 0.;
-            const double c = vb ? rhs_value(fp.kind, fp.rowsA0 + rb, j, N, b.in) : 0.;
+            const double a = va ? rhs_value<G>(fp.kind, fp.rowsA0 + ra, j, N, b.in) : 0.;
+            const double c = vb ? rhs_value<G>(fp.kind, fp.rowsA0 + rb, j, N, b.in) : 0.;
             put_ext(z, fp.xfA, n, fp.lenA, L, a, c);
         }
     }
@@ -633,8 +635,6 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     const bool va = la < lines, vb = lb < lines;
     S* rowa = static_cast<S*>(fp.T) + (size_t)la * len;
     S* rowb = static_cast<S*>(fp.T) + (size_t)lb * len;
-    const S* taba = static_cast<const S*>(fp.table) + (size_t)la * len;
-    const S* tabb = static_cast<const S*>(fp.table) + (size_t)lb * len;
 
     if constexpr (L_ != 0) {
         constexpr int kPer = points_per_thread(L_, std::is_same<S, float>::value);
@@ -743,7 +743,9 @@ pass_b_kernel(SolverGeom g, SolveBatch b) {
     }
 }
 
-template <int L_, bool TW_LDS, typename S>
+// G: storage type of the fields, averages and guess (FP64 epilogue
+// arithmetic in both).
+template <int L_, bool TW_LDS, typename S, typename G>
 __global__ void __launch_bounds__(1024) LCODE_PASS_C_ATTR
 pass_c_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
@@ -816,7 +818,10 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
     const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
                             fp.kind == F_BX || fp.kind == F_BY;
     const bool varA = b.variantA && transverse;
-    const double* guess = b.resid ? b.guess[fp.kind] : nullptr;
+    const G* guess = b.resid ? gp<G>(b.guess[fp.kind]) : nullptr;
+    const G* prevF = gp<G>(fp.prevF);
+    G* const outF = gp<G>(fp.outF);
+    G* const outAvg = gp<G>(fp.outAvg);
     double r_num = 0., r_den = 0.;
     // one output point (i, j) of this pair: F from the spectrum value v,
     // prev and guess values p, q of that point
@@ -825,8 +830,8 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
         double F = v;
         if (fp.partial) F -= mean;                         // lcode.py:362
         if (varA) F = 2 * F - p;                           // lcode.py:869-871
-        if (fp.outF) fp.outF[idx] = F;
-        if (fp.outAvg) fp.outAvg[idx] = (F + p) / 2;       // lcode.py:876-881
+        if (outF) outF[idx] = (G)F;
+        if (outAvg) outAvg[idx] = (G)((F + p) / 2);        // lcode.py:876-881
         if (guess) {
             const double d = F - q;
             r_num += d * d;
@@ -850,12 +855,12 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
         double pa[kPer], pb[kPer], qa[kPer], qb[kPer];
 #pragma unroll
         for (int u = 0; u < kPer; ++u) pa[u] = pb[u] = qa[u] = qb[u] = 0.;
-        if (fp.prevF) {
+        if (prevF) {
 #pragma unroll
             for (int u = 0; u < kPer; ++u) {
                 const int jc = min(t + u * Tp, N - 1);
-                pa[u] = fp.prevF[oa + jc];
-                pb[u] = fp.prevF[ob + jc];
+                pa[u] = prevF[oa + jc];
+                pb[u] = prevF[ob + jc];
             }
         }
         if (guess) {
@@ -881,9 +886,9 @@ pass_c_kernel(SolverGeom g, SolveBatch b) {
             spec(j, va, vb);
             const size_t ia = (size_t)ra * N + j, ib = (size_t)rb * N + j;
             if (ra < N)
-                emit(ra, j, va, fp.prevF ? fp.prevF[ia] : 0., guess ? guess[ia] : 0.);
+                emit(ra, j, va, prevF ? (double)prevF[ia] : 0., guess ? (double)guess[ia] : 0.);
             if (rb < N)
-                emit(rb, j, vb, fp.prevF ? fp.prevF[ib] : 0., guess ? guess[ib] : 0.);
+                emit(rb, j, vb, prevF ? (double)prevF[ib] : 0., guess ? (double)guess[ib] : 0.);
         }
     }
     if (b.resid) {
@@ -940,6 +945,7 @@ resid_reduce_kernel(const double* part, int nblocks, double* resid) {
 // Guided step set-up: guess = sum_t c[t] hist[t] for the six fields, the
 // predictor-pass averages (guess + prev) / 2 (lcode.py:876-881 with the
 // guess in place of the predicted fields), and zeroed residual sums.
+template <typename G>
 __global__ void __launch_bounds__(256)
 guess_kernel(GuessArgs p) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -948,22 +954,23 @@ guess_kernel(GuessArgs p) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
         double gsum = 0.;
-        for (int t = 0; t < p.nt; ++t) gsum += p.c[t] * p.h[t][f][k];
-        p.guess[f][k] = gsum;
-        p.avg[f][k] = (gsum + p.prev[f][k]) / 2;
+        for (int t = 0; t < p.nt; ++t) gsum += p.c[t] * (double)gp<G>(p.h[t][f])[k];
+        const G gs = (G)gsum;
+        gp<G>(p.guess[f])[k] = gs;
+        gp<G>(p.avg[f])[k] = (G)(((double)gs + (double)gp<G>(p.prev[f])[k]) / 2);
     }
 }
 
 }  // namespace
 
-template <int L_, bool TW_LDS, typename S>
+template <int L_, bool TW_LDS, typename S, typename G>
 static int launch_passes(const SolverGeom& g, const SolveBatch& b,
                          hipStream_t s, Profiler* prof, size_t lds,
                          dim3 ga, dim3 gb, dim3 gc) {
     const dim3 block(g.P * g.Tp);
     {
         ProfScope ps(prof, "solve_pass_a", s);
-        pass_a_kernel<L_, TW_LDS, S><<<ga, block, lds, s>>>(g, b);
+        pass_a_kernel<L_, TW_LDS, S, G><<<ga, block, lds, s>>>(g, b);
     }
     LCODE_HIP(hipGetLastError());
     {
@@ -973,7 +980,7 @@ static int launch_passes(const SolverGeom& g, const SolveBatch& b,
     LCODE_HIP(hipGetLastError());
     {
         ProfScope ps(prof, "solve_pass_c", s);
-        pass_c_kernel<L_, TW_LDS, S><<<gc, block, lds, s>>>(g, b);
+        pass_c_kernel<L_, TW_LDS, S, G><<<gc, block, lds, s>>>(g, b);
     }
     LCODE_HIP(hipGetLastError());
     if (b.resid) {
@@ -1017,7 +1024,7 @@ bool solver_static_length(int L) {
     }
 }
 
-template <typename S>
+template <typename S, typename G>
 static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
                           hipStream_t s, Profiler* prof) {
     const size_t lds = (size_t)solver_lds_bytes(g);
@@ -1033,14 +1040,14 @@ static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
 #define LCODE_CASE(LL)                                                          \
     case LL:                                                                    \
         return g.tw_lds                                                         \
-                   ? launch_passes<LL, true, S>(g, b, s, prof, lds, ga, gb, gc) \
-                   : launch_passes<LL, false, S>(g, b, s, prof, lds, ga, gb, gc);
+                   ? launch_passes<LL, true, S, G>(g, b, s, prof, lds, ga, gb, gc) \
+                   : launch_passes<LL, false, S, G>(g, b, s, prof, lds, ga, gb, gc);
     switch (g.static_L) {
         LCODE_CASE(32) LCODE_CASE(64) LCODE_CASE(128) LCODE_CASE(256)
         LCODE_CASE(512) LCODE_CASE(640) LCODE_CASE(1024) LCODE_CASE(1280)
         LCODE_CASE(2048) LCODE_CASE(2560)
         default:
-            return launch_passes<0, false, S>(g, b, s, prof, lds, ga, gb, gc);
+            return launch_passes<0, false, S, G>(g, b, s, prof, lds, ga, gb, gc);
     }
 #undef LCODE_CASE
 }
@@ -1048,7 +1055,10 @@ static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
 int launch_guess(const GuessArgs& p, hipStream_t s, Profiler* prof) {
     {
         ProfScope ps(prof, "guess", s);
-        guess_kernel<<<(p.n + 255) / 256, 256, 0, s>>>(p);
+        if (p.grid32)
+            guess_kernel<float><<<(p.n + 255) / 256, 256, 0, s>>>(p);
+        else
+            guess_kernel<double><<<(p.n + 255) / 256, 256, 0, s>>>(p);
     }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
@@ -1057,8 +1067,14 @@ int launch_guess(const GuessArgs& p, hipStream_t s, Profiler* prof) {
 int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
                  Profiler* prof) {
     if (b.nfields == 0) return LCODE_OK;
-    return g.fp32 ? launch_solve_t<float>(g, b, s, prof)
-                  : launch_solve_t<double>(g, b, s, prof);
+    // FP32 grids come only with FP32 transforms (the mixed form)
+    if (b.grid32 && !g.fp32) {
+        set_error("FP32 grids need the FP32 transforms");
+        return LCODE_E_INVALID;
+    }
+    if (b.grid32) return launch_solve_t<float, float>(g, b, s, prof);
+    return g.fp32 ? launch_solve_t<float, double>(g, b, s, prof)
+                  : launch_solve_t<double, double>(g, b, s, prof);
 }
 
 }  // namespace lcode

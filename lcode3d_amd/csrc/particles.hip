@@ -67,6 +67,8 @@ constexpr int kPushBlock = LCODE_PUSH_BLOCK;
 #endif
 
 // One coarse particle's push; returns max(|x_offt|, |y_offt|) after it.
+// G: storage type of the field grids (the momentum update is FP64 in both).
+template <typename G>
 __device__ __forceinline__ double push_one(const PushArgs& a, int k) {
     const double m = a.m[k], q = a.q[k];
     const double opx = a.px[k], opy = a.py[k], opz = a.pz[k];
@@ -84,12 +86,12 @@ __device__ __forceinline__ double push_one(const PushArgs& a, int k) {
     const double y_half = yi0 + (y_offt + ey) / 2;
     Tsc t;
     tsc_weights(x_half, y_half, a.N, a.h, t, a.fault);
-    const double Ex = interp9(a.F[0], a.N, t);
-    const double Ey = interp9(a.F[1], a.N, t);
-    const double Ez = interp9(a.F[2], a.N, t);
-    const double Bx = interp9(a.F[3], a.N, t);
-    const double By = interp9(a.F[4], a.N, t);
-    const double Bz = interp9(a.F[5], a.N, t);
+    const double Ex = interp9(gp<G>(a.F[0]), a.N, t);
+    const double Ey = interp9(gp<G>(a.F[1]), a.N, t);
+    const double Ez = interp9(gp<G>(a.F[2]), a.N, t);
+    const double Bx = interp9(gp<G>(a.F[3]), a.N, t);
+    const double By = interp9(gp<G>(a.F[4]), a.N, t);
+    const double Bz = interp9(gp<G>(a.F[5]), a.N, t);
 
     double px = opx, py = opy, pz = opz;
     double dpx = 0, dpy = 0, dpz = 0;
@@ -130,11 +132,12 @@ __device__ __forceinline__ double push_one(const PushArgs& a, int k) {
     return finite ? fmax(fabs(x_offt), fabs(y_offt)) : __builtin_inf();
 }
 
+template <typename G>
 __global__ void __launch_bounds__(kPushBlock) LCODE_PUSH_ATTR
 push_kernel(PushArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.;
-    if (k < a.n && gate_open(a.gate)) d = push_one(a, k);
+    if (k < a.n && gate_open(a.gate)) d = push_one<G>(a, k);
     if (a.max_cells) {
         // cells the particles moved from their initial positions, at most
         // (the deposition sizes its tile halo by it); a non-finite state,
@@ -162,11 +165,12 @@ push_kernel(PushArgs a) {
 // (lcode.py:651-655): dro = q / (1 - pz / gamma), dj = p dro / gamma,
 // evaluated with one division as dro = q gamma r, dj = q p r,
 // r = 1 / (gamma - pz) (the same quantities, rounded differently).
-__device__ __forceinline__ void deposit_values(double m, double q, double px,
-                                               double py, double pz,
-                                               double* v) {
-    const double gamma_m = sqrt(m * m + px * px + py * py + pz * pz);
-    const double qr = q / (gamma_m - pz);
+// G = float: the FP32-grid form, in FP32 arithmetic.
+template <typename G>
+__device__ __forceinline__ void deposit_values(G m, G q, G px, G py, G pz,
+                                               G* v) {
+    const G gamma_m = sqrt(m * m + px * px + py * py + pz * pz);
+    const G qr = q / (gamma_m - pz);
     v[0] = gamma_m * qr;
     v[1] = px * qr;
     v[2] = py * qr;
@@ -175,12 +179,15 @@ __device__ __forceinline__ void deposit_values(double m, double q, double px,
 
 // Coarse->fine interpolation of fine particle (fi, fj) (lcode.py:591-621)
 // and its deposition values, reading everything from global memory.
+template <typename G>
 struct FineParticle {
-    double x, y, v[4];   // position; dro, djx, djy, djz
+    double x, y;         // position (FP64 in every form)
+    G v[4];              // dro, djx, djy, djz
 };
 
+template <typename G>
 __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
-                                              int fj, FineParticle& p) {
+                                              int fj, FineParticle<G>& p) {
     const double ipi = a.ip[fi], ini = a.in[fi];
     const double ipj = a.ip[fj], inj = a.in[fj];
     const double A = ipi * ipj, B = ipi * inj, C = ini * ipj, D = ini * inj;
@@ -195,8 +202,9 @@ __device__ __forceinline__ void fine_particle(const DepositArgs& a, int fi,
     p.x = a.fg[fi] + LCODE_MIX(a.x_offt);
     p.y = a.fg[fj] + LCODE_MIX(a.y_offt);
     const double s = a.smallness;
-    deposit_values(s * LCODE_MIX(a.m), s * LCODE_MIX(a.q), s * LCODE_MIX(a.px),
-                   s * LCODE_MIX(a.py), s * LCODE_MIX(a.pz), p.v);
+    deposit_values<G>((G)(s * LCODE_MIX(a.m)), (G)(s * LCODE_MIX(a.q)),
+                      (G)(s * LCODE_MIX(a.px)), (G)(s * LCODE_MIX(a.py)),
+                      (G)(s * LCODE_MIX(a.pz)), p.v);
 #undef LCODE_MIX
 }
 
@@ -264,10 +272,11 @@ __device__ int wave_first_fine(const double* fg, int nf, int c, int N,
 
 // Adds v[c] * w[k] (or val[c][k]) to the stencil points of cell (i, j) that
 // lie in this block's tile.
+template <typename G>
 __device__ __forceinline__ void lds_deposit(double (*acc)[kTile][kTile],
                                             int i, int j, int ci0, int cj0,
-                                            const double* v, const double* w,
-                                            const double (*val)[9]) {
+                                            const G* v, const G* w,
+                                            const G (*val)[9]) {
     const bool inside = i - 1 >= ci0 && i + 1 < ci0 + kTile &&
                         j - 1 >= cj0 && j + 1 < cj0 + kTile;
 #pragma unroll
@@ -277,33 +286,46 @@ __device__ __forceinline__ void lds_deposit(double (*acc)[kTile][kTile],
         if (inside || (si >= 0 && si < kTile && sj >= 0 && sj < kTile)) {
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                atomicAdd(&acc[c][si][sj], val ? val[c][k] : v[c] * w[k]);
+                atomicAdd(&acc[c][si][sj], (double)(val ? val[c][k] : v[c] * w[k]));
         }
     }
 }
 
+template <typename A, typename G>
 __device__ __forceinline__ void deposit_spill_one(const DepositArgs& a, int fk,
                                                   int N) {
-    FineParticle fp;
-    fine_particle(a, fk / a.nf, fk % a.nf, fp);
-    Tsc t;
-    tsc_weights(fp.x, fp.y, N, a.h, t, nullptr, 1. / a.h);
+    FineParticle<A> fp;
+    fine_particle<A>(a, fk / a.nf, fk % a.nf, fp);
+    TscT<A> t;
+    tsc_weights<A>(fp.x, fp.y, N, a.h, t, nullptr, 1. / a.h);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const size_t idx = (size_t)(t.i + stencil_di(k)) * N + (t.j + stencil_dj(k));
-        unsafeAtomicAdd(a.ro + idx, fp.v[0] * t.w[k]);
-        unsafeAtomicAdd(a.jx + idx, fp.v[1] * t.w[k]);
-        unsafeAtomicAdd(a.jy + idx, fp.v[2] * t.w[k]);
-        unsafeAtomicAdd(a.jz + idx, fp.v[3] * t.w[k]);
+        unsafeAtomicAdd(gp<G>(a.ro) + idx, (G)(fp.v[0] * t.w[k]));
+        unsafeAtomicAdd(gp<G>(a.jx) + idx, (G)(fp.v[1] * t.w[k]));
+        unsafeAtomicAdd(gp<G>(a.jy) + idx, (G)(fp.v[2] * t.w[k]));
+        unsafeAtomicAdd(gp<G>(a.jz) + idx, (G)(fp.v[3] * t.w[k]));
     }
 }
 
-template <bool STAGED>
+// A: type of the per-particle arithmetic and register sums, G: storage
+// type of the output grids (double, double: the plain path; double,
+// float: the mixed form, whose density and currents are stored in FP32).
+// The LDS tile accumulates in FP64: at 641^2 FP32 LDS atomics (ds_add_f32)
+// kept the LDS pipeline 8x longer busy than FP64 ones (ds_add_f64) for the
+// same instruction count (148 against 35 us per launch).  FP32 arithmetic
+// (A = float) took the deposition from 49.8 to 47.9 us only (the kernel is
+// bound by latency, not by its VALU work) while its rounding, relative to
+// the background density 1, left the density response of a small beam
+// (amplitude 0.01) with relative errors up to 4e-4 in Ez; the mixed form
+// keeps FP64 arithmetic.
+template <bool STAGED, typename A, typename G>
 __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     if (!gate_open(a.gate)) return;   // block-uniform: before any barrier
     __shared__ double acc[4][kTile][kTile];
     __shared__ FineTab ti, tj;
-    __shared__ double cs[7][kCoarseMax * kCoarseMax];
+    __shared__ double cpos[2][kCoarseMax * kCoarseMax];   // x_offt, y_offt
+    __shared__ A cval[5][kCoarseMax * kCoarseMax];        // m q px py pz
     __shared__ int range[4];
     __shared__ int tile_nan;
     const int N = a.N;
@@ -327,7 +349,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     const int ext = kTile + 2 * halo + 2;
     const int ci_lo = ci0 - 1 - halo, cj_lo = cj0 - 1 - halo;
     for (int k = threadIdx.x; k < 4 * kTile * kTile; k += blockDim.x)
-        (&acc[0][0][0])[k] = 0.;
+        (&acc[0][0][0])[k] = 0;
     {
         const int wave = threadIdx.x >> 6;
         if (wave < 4) {
@@ -375,8 +397,10 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             const double* src[7] = {a.x_offt, a.y_offt, a.m, a.q, a.px, a.py, a.pz};
             for (int k = threadIdx.x; k < cH * cW; k += blockDim.x) {
                 const size_t g = (size_t)(cI0 + k / cW) * a.nc + cJ0 + k % cW;
+                cpos[0][k] = src[0][g];
+                cpos[1][k] = src[1][g];
 #pragma unroll
-                for (int c = 0; c < 7; ++c) cs[c][k] = src[c][g];
+                for (int c = 0; c < 5; ++c) cval[c][k] = (A)src[2 + c][g];
             }
             // every thread has read the rectangle's corners: turn the
             // neighbour indices into offsets in it, rows scaled by its
@@ -396,39 +420,44 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             }
         }
         __syncthreads();
-        const double sm = a.smallness;
+        const A sm = (A)a.smallness;
         // coarse->fine interpolation of tile-local fine particle (li, lj):
         // position and deposition values (lcode.py:591-621, :651-655)
-        auto particle = [&](int li, int lj, double& x, double& y, double* v) {
-            const double A = ti.ip[li] * tj.ip[lj], B = ti.ip[li] * tj.in[lj];
-            const double C = ti.in[li] * tj.ip[lj], D = ti.in[li] * tj.in[lj];
+        auto particle = [&](int li, int lj, double& x, double& y, A* v) {
+            const double wA = ti.ip[li] * tj.ip[lj], wB = ti.ip[li] * tj.in[lj];
+            const double wC = ti.in[li] * tj.ip[lj], wD = ti.in[li] * tj.in[lj];
+            const A Ag = (A)wA, Bg = (A)wB, Cg = (A)wC, Dg = (A)wD;
             if constexpr (STAGED) {
                 // row offset + column offset (both clamped into the staged
                 // rectangle above)
                 const int pp = ti.P[li] + tj.P[lj], pn = ti.P[li] + tj.Nn[lj];
                 const int np = ti.Nn[li] + tj.P[lj], nn = ti.Nn[li] + tj.Nn[lj];
-#define LCODE_MIX(c) (A * cs[c][pp] + B * cs[c][pn] + C * cs[c][np] + D * cs[c][nn])
-                x = ti.fg[li] + LCODE_MIX(0);
-                y = tj.fg[lj] + LCODE_MIX(1);
-                deposit_values(sm * LCODE_MIX(2), sm * LCODE_MIX(3), sm * LCODE_MIX(4),
-                               sm * LCODE_MIX(5), sm * LCODE_MIX(6), v);
+#define LCODE_MIXP(c) (wA * cpos[c][pp] + wB * cpos[c][pn] + wC * cpos[c][np] + wD * cpos[c][nn])
+#define LCODE_MIX(c) (Ag * cval[c][pp] + Bg * cval[c][pn] + Cg * cval[c][np] + Dg * cval[c][nn])
+                x = ti.fg[li] + LCODE_MIXP(0);
+                y = tj.fg[lj] + LCODE_MIXP(1);
+                deposit_values<A>(sm * LCODE_MIX(0), sm * LCODE_MIX(1), sm * LCODE_MIX(2),
+                                  sm * LCODE_MIX(3), sm * LCODE_MIX(4), v);
 #undef LCODE_MIX
+#undef LCODE_MIXP
             } else {
                 const int nc = a.nc;
                 const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
                 const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
-#define LCODE_MIX(arr) (A * arr[pp] + B * arr[pn] + C * arr[np] + D * arr[nn])
-                x = ti.fg[li] + LCODE_MIX(a.x_offt);
-                y = tj.fg[lj] + LCODE_MIX(a.y_offt);
-                deposit_values(sm * LCODE_MIX(a.m), sm * LCODE_MIX(a.q),
-                               sm * LCODE_MIX(a.px), sm * LCODE_MIX(a.py),
-                               sm * LCODE_MIX(a.pz), v);
+#define LCODE_MIXP(arr) (wA * arr[pp] + wB * arr[pn] + wC * arr[np] + wD * arr[nn])
+#define LCODE_MIX(arr) (Ag * (A)arr[pp] + Bg * (A)arr[pn] + Cg * (A)arr[np] + Dg * (A)arr[nn])
+                x = ti.fg[li] + LCODE_MIXP(a.x_offt);
+                y = tj.fg[lj] + LCODE_MIXP(a.y_offt);
+                deposit_values<A>(sm * LCODE_MIX(a.m), sm * LCODE_MIX(a.q),
+                                  sm * LCODE_MIX(a.px), sm * LCODE_MIX(a.py),
+                                  sm * LCODE_MIX(a.pz), v);
 #undef LCODE_MIX
+#undef LCODE_MIXP
             }
         };
         // TSC centre and weights of the particle; false when it moved more
         // than the halo from its initial cell (its home tile lists it)
-        auto locate = [&](int li, int lj, double x, double y, Tsc& t) {
+        auto locate = [&](int li, int lj, double x, double y, TscT<A>& t) {
             const int ci = ti.cell[li], cj = tj.cell[lj];   // initial cell
             const bool home = ci >= ci0 && ci < ci0 + kTile &&
                               cj >= cj0 && cj < cj0 + kTile;
@@ -443,7 +472,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
                 }
                 return false;
             }
-            tsc_weights(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
+            tsc_weights<A>(x, y, N, a.h, t, home ? a.fault : nullptr, inv_h);
             if (abs(t.i - ci) > halo || abs(t.j - cj) > halo) {
                 // (beyond a bound the pushes reported: cannot happen; counted
                 // as a fault rather than listed for a pass that does not run)
@@ -470,11 +499,11 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         const int ngi = (ni + gs - 1) / gs, ngj = (nj + gs - 1) / gs;
         for (int g = threadIdx.x; g < ngi * ngj; g += blockDim.x) {
             const int gi = g / ngj, gj = g - gi * ngj;
-            double pat[4][9];
+            A pat[4][9];
 #pragma unroll
             for (int c = 0; c < 4; ++c)
 #pragma unroll
-                for (int k = 0; k < 9; ++k) pat[c][k] = 0.;
+                for (int k = 0; k < 9; ++k) pat[c][k] = 0;
             int ai = -1, aj = -1;   // centre of the sums in pat (-1: none)
             int qi = 0, qj = 0;     // particle (qi, qj) of the group
             for (int q = 0; q <= gs * gs; ++q) {
@@ -483,8 +512,8 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
                     qj = 0;
                     ++qi;
                 }
-                double v[4] = {0., 0., 0., 0.};
-                Tsc t;
+                A v[4] = {0, 0, 0, 0};
+                TscT<A> t;
                 t.i = t.j = -1;
                 if (q < gs * gs && li < ni && lj < nj) {
                     double x, y;
@@ -493,11 +522,11 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
                 }
                 if (ai >= 0 && (t.i != ai || t.j != aj)) {
                     if (q == gs * gs || t.i >= 0) {
-                        lds_deposit(acc, ai, aj, ci0, cj0, nullptr, nullptr, pat);
+                        lds_deposit<A>(acc, ai, aj, ci0, cj0, nullptr, nullptr, pat);
 #pragma unroll
                         for (int c = 0; c < 4; ++c)
 #pragma unroll
-                            for (int k = 0; k < 9; ++k) pat[c][k] = 0.;
+                            for (int k = 0; k < 9; ++k) pat[c][k] = 0;
                         ai = -1;
                     }
                 }
@@ -514,11 +543,12 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
 #else
         for (int p = threadIdx.x; p < ni * nj; p += blockDim.x) {
             const int li = p / nj, lj = p - li * nj;
-            double x, y, v[4];
+            double x, y;
+            A v[4];
             particle(li, lj, x, y, v);
-            Tsc t;
+            TscT<A> t;
             if (!locate(li, lj, x, y, t)) continue;
-            lds_deposit(acc, t.i, t.j, ci0, cj0, v, t.w, nullptr);
+            lds_deposit<A>(acc, t.i, t.j, ci0, cj0, v, t.w, nullptr);
         }
 #endif
     }
@@ -543,14 +573,19 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         const int i = ci0 + r, j = cj0 + c;
         if (k >= kTile * kTile || i >= N || j >= N) continue;
         const size_t idx = (size_t)i * N + j;
+        G* const ro = gp<G>(a.ro);
+        G* const jx = gp<G>(a.jx);
+        G* const jy = gp<G>(a.jy);
+        G* const jz = gp<G>(a.jz);
         if (overfull || nonfinite || tile_nan) {
-            a.ro[idx] = a.jx[idx] = a.jy[idx] = a.jz[idx] = __builtin_nan("");
+            ro[idx] = jx[idx] = jy[idx] = jz[idx] = (G)__builtin_nan("");
             continue;
         }
-        a.ro[idx] = a.ro_initial ? acc[0][r][c] + base[u] : acc[0][r][c];
-        a.jx[idx] = acc[1][r][c];
-        a.jy[idx] = acc[2][r][c];
-        a.jz[idx] = acc[3][r][c];
+        // the FP64 ion background is added in FP64, then rounded once
+        ro[idx] = (G)(a.ro_initial ? acc[0][r][c] + base[u] : acc[0][r][c]);
+        jx[idx] = (G)acc[1][r][c];
+        jy[idx] = (G)acc[2][r][c];
+        jz[idx] = (G)acc[3][r][c];
     }
 }
 
@@ -558,6 +593,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
 // halo, with device-scope atomics over the whole grid (after the tile
 // kernel's stores, which it follows on the stream), and zeroes the count
 // slot of the next call.  An empty list costs one load per block.
+template <typename A, typename G>
 __global__ void __launch_bounds__(kBlock)
 deposit_spill_kernel(DepositArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.spill_next = 0;
@@ -565,17 +601,85 @@ deposit_spill_kernel(DepositArgs a) {
     const int count = *a.spill_count;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count;
          p += gridDim.x * blockDim.x)
-        deposit_spill_one(a, a.spill[p], a.N);
+        deposit_spill_one<A, G>(a, a.spill[p], a.N);
 }
 
-template <bool STAGED>
+template <bool STAGED, typename A, typename G>
 __global__ void __launch_bounds__(kDepBlock) LCODE_DEP_ATTR
 deposit_tile_kernel(DepositArgs a) {
-    deposit_tile_body<STAGED>(a);
+    deposit_tile_body<STAGED, A, G>(a);
 }
 
 __global__ void __launch_bounds__(64) zero_ints_kernel(int* p, int n) {
     for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Plasma set-up (make_plasma, lcode.py:452-573) on the device, in the
+// reference's FP64 arithmetic (integer counters converted exactly)
+// ---------------------------------------------------------------------------
+
+// Coarse grid point i of 2 hc - 1: -(hc - 1 - i) s for the left half,
+// (i - hc + 1) s for the right one (lcode.py:452-465).
+__device__ __forceinline__ double coarse_at(int i, int hc, double s) {
+#pragma clang fp contract(off)
+    const int k = i - (hc - 1);
+    return k < 0 ? -((double)(-k) * s) : (double)k * s;
+}
+
+// Fine grid point i (lcode.py:468-487): odd fineness, 2 hf - 1 points on
+// k s; even fineness, 2 hf points on (.5 + k) s.
+__device__ __forceinline__ double fine_at(int i, int hf, int odd, double s) {
+#pragma clang fp contract(off)
+    if (odd) {
+        const int k = i - (hf - 1);
+        return k < 0 ? -((double)(-k) * s) : (double)k * s;
+    }
+    return i < hf ? -((.5 + (double)(hf - 1 - i)) * s) : (.5 + (double)(i - hf)) * s;
+}
+
+__global__ void __launch_bounds__(kBlock)
+plasma_coarse_kernel(PlasmaArgs a) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nc * a.nc) return;
+    const int i = k / a.nc, j = k % a.nc;
+    const double s = a.cell * a.coarseness;
+    a.x_init[k] = coarse_at(i, a.hc, s);
+    a.y_init[k] = coarse_at(j, a.hc, s);
+    const double c2 = (double)(a.coarseness * a.coarseness);
+    a.m[k] = 1. * kElectronMass * c2;        // ones * ELECTRON_MASS * c**2
+    a.q[k] = 1. * kElectronCharge * c2;
+}
+
+// virt_params (lcode.py:526-570): searchsorted of each fine point in the
+// coarse grid (first index with coarse >= fine), the clipped neighbour
+// indices and the interpolation weights with the boundary fixes.
+__global__ void __launch_bounds__(kBlock)
+plasma_fine_kernel(PlasmaArgs a) {
+    // no fused multiply-adds: every operation rounds as NumPy's does
+#pragma clang fp contract(off)
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    const double sc = a.cell * a.coarseness;
+    const double sf = a.cell / a.fineness;
+    const double x = fine_at(f, a.hf, a.fineness % 2, sf);
+    int lo = 0, hi = a.nc;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (coarse_at(mid, a.hc, sc) < x) lo = mid + 1; else hi = mid;
+    }
+    const int nxt = lo < 0 ? 0 : (lo > a.nc - 1 ? a.nc - 1 : lo);
+    const int prv = lo - 1 < 0 ? 0 : (lo - 1 > a.nc - 1 ? a.nc - 1 : lo - 1);
+    double wp = (coarse_at(nxt, a.hc, sc) - x) / sc;
+    double wn = (x - coarse_at(prv, a.hc, sc)) / sc;
+    if (nxt == 0) { wp = 0.; wn = 1.; }
+    if (prv == a.nc - 1) { wn = 0.; wp = 1.; }
+    a.fg[f] = x;
+    a.ip[f] = wp;
+    a.in[f] = wn;
+    a.P[f] = prv;
+    a.Nn[f] = nxt;
 }
 
 inline int blocks_for(long n) { return (int)((n + kBlock - 1) / kBlock); }
@@ -593,8 +697,24 @@ int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof) {
 int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof) {
     if (a.n == 0) return LCODE_OK;
     ProfScope ps(prof, "push", s);
-    push_kernel<<<(a.n + kPushBlock - 1) / kPushBlock, kPushBlock, 0, s>>>(a);
+    const int blocks = (a.n + kPushBlock - 1) / kPushBlock;
+    if (a.grid32)
+        push_kernel<float><<<blocks, kPushBlock, 0, s>>>(a);
+    else
+        push_kernel<double><<<blocks, kPushBlock, 0, s>>>(a);
     LCODE_HIP(hipGetLastError());
+    return LCODE_OK;
+}
+
+int launch_make_plasma(const PlasmaArgs& a, hipStream_t s) {
+    if (a.nc > 0) {
+        plasma_coarse_kernel<<<blocks_for((long)a.nc * a.nc), kBlock, 0, s>>>(a);
+        LCODE_HIP(hipGetLastError());
+    }
+    if (a.nf > 0) {
+        plasma_fine_kernel<<<blocks_for(a.nf), kBlock, 0, s>>>(a);
+        LCODE_HIP(hipGetLastError());
+    }
     return LCODE_OK;
 }
 
@@ -609,15 +729,20 @@ int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof) {
     const dim3 grid((a.N + kTile - 1) / kTile, (a.N + kTile - 1) / kTile);
     {
         ProfScope ps(prof, "deposit_tile", s);
-        if (a.staged)
-            deposit_tile_kernel<true><<<grid, kDepBlock, 0, s>>>(a);
-        else
-            deposit_tile_kernel<false><<<grid, kDepBlock, 0, s>>>(a);
+#define LCODE_DEP_LAUNCH(AT, GT)                                                   \
+    (a.staged ? (deposit_tile_kernel<true, AT, GT><<<grid, kDepBlock, 0, s>>>(a)) \
+              : (deposit_tile_kernel<false, AT, GT><<<grid, kDepBlock, 0, s>>>(a)))
+        if (a.grid32) LCODE_DEP_LAUNCH(double, float);
+        else LCODE_DEP_LAUNCH(double, double);
+#undef LCODE_DEP_LAUNCH
     }
     LCODE_HIP(hipGetLastError());
     {
         ProfScope ps(prof, "deposit_spill", s);
-        deposit_spill_kernel<<<kSpillBlocks, kBlock, 0, s>>>(a);
+        if (a.grid32)
+            deposit_spill_kernel<double, float><<<kSpillBlocks, kBlock, 0, s>>>(a);
+        else
+            deposit_spill_kernel<double, double><<<kSpillBlocks, kBlock, 0, s>>>(a);
     }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;

@@ -119,6 +119,7 @@ def _config_struct(config, nc=0, nf=0):
 _SOLVERS = {
     ("fft", "fp64"): _lib.LCODE_SOLVER_FFT64,
     ("fft", "fp32"): _lib.LCODE_SOLVER_FFT32,
+    ("fft", "mixed"): _lib.LCODE_SOLVER_MIXED,
     ("dense", "fp64"): _lib.LCODE_SOLVER_DENSE64,
     ("dense", "bf16x3"): _lib.LCODE_SOLVER_DENSE_BF16X3,
 }
@@ -132,8 +133,12 @@ def _solver_of(config):
     config.field_solver_precision: 'fp64' (default, the plain path), 'fp32'
     (FFT only: transforms, spectra and eigenvalue tables in FP32) or
     'bf16x3' (dense only: hi/lo BF16 splits, three MFMA products, FP32
-    accumulation).  RHS, Bz mean, Variant A and averages are FP64 in every
-    mode."""
+    accumulation); in these three the RHS, Bz mean, Variant A and averages
+    are FP64.  'mixed' (FFT only): the mixed-precision step - the ten grids
+    of the state (float32 tensors from init), the deposition and the
+    transforms in FP32; particles, the RHS arithmetic with its trick term,
+    the Bz zero mode and mean and ro_initial stay FP64 (the standalone
+    operators run as 'fp32' on float64 grids)."""
     p = getattr(config, "field_solver_precision", "fp64")
     t = getattr(config, "field_solver_transform", "dense" if p == "bf16x3" else "fft")
     try:
@@ -152,11 +157,11 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-def _check_grid(config, *arrays):
+def _check_grid(config, *arrays, dtype=_F64):
     n = int(config.grid_steps)
     out = []
     for a in arrays:
-        t = _as_device(a, _F64)
+        t = _as_device(a, dtype)
         if tuple(t.shape) != (n, n):
             raise ValueError(f"grid array of shape {tuple(t.shape)}, "
                              f"expected ({n}, {n})")
@@ -389,23 +394,36 @@ FIELDS = ("Ex", "Ey", "Ez", "Bx", "By", "Bz")
 GUIDED_THRESHOLD = 2e-3
 
 
+def is_mixed(config):
+    """True when config selects the mixed-precision step (float32 grids)."""
+    return getattr(config, "field_solver_precision", "fp64") == "mixed"
+
+
+def grid_dtype(config):
+    """dtype of the state's ten N x N grids: float32 in the mixed form."""
+    return torch.float32 if is_mixed(config) else _F64
+
+
 def _check_step_arrays(config, const, prev):
     """dtype, layout, shape and device of every array the fused step reads
     through a raw pointer (before anything is launched)."""
     nc = prev.x_offt.shape[0]
     n = int(config.grid_steps)
     coarse = ("x_offt", "y_offt", "px", "py", "pz")
-    checks = [(f"const.{k}", getattr(const, k), (nc, nc))
+    gdt = grid_dtype(config)
+    checks = [(f"const.{k}", getattr(const, k), (nc, nc), _F64)
               for k in ("m", "q", "x_init", "y_init")]
-    checks.append(("const.ro_initial", const.ro_initial, (n, n)))
+    checks.append(("const.ro_initial", const.ro_initial, (n, n), _F64))
     checks += [(f"prev.{k}", getattr(prev, k),
-                (nc, nc) if k in coarse else (n, n)) for k in _lib.STATE_FIELDS]
-    for name, t, shape in checks:
-        if (not isinstance(t, torch.Tensor) or t.dtype != _F64
+                (nc, nc) if k in coarse else (n, n),
+                _F64 if k in coarse else gdt) for k in _lib.STATE_FIELDS]
+    for name, t, shape, dt in checks:
+        if (not isinstance(t, torch.Tensor) or t.dtype != dt
                 or not t.is_contiguous() or tuple(t.shape) != shape):
-            raise ValueError(f"{name} must be a contiguous float64 tensor "
+            raise ValueError(f"{name} must be a contiguous "
+                             f"{str(dt).replace('torch.', '')} tensor "
                              f"of shape {shape}")
-    for name, t, shape in checks:
+    for name, t, shape, _ in checks:
         if t.device.type != "cuda":
             raise ValueError(f"{name} must be a device (HIP) tensor")
     return nc
@@ -458,23 +476,51 @@ def step(config, const, virt_params, prev, beam_ro, guess=None,
            for name in _lib.STATE_FIELDS}
     o = _lib.LcodeState(*(new[name].data_ptr() for name in _lib.STATE_FIELDS))
     beam_ptr = _ptr(beam) if beam is not None else None
+    mixed = is_mixed(config)
     if guess is None:
-        _lib.check(lib.lcode_step(ws, ctypes.byref(cs),
-                                  ctypes.byref(c), ctypes.byref(vp),
-                                  ctypes.byref(p), beam_ptr, beam_scalar,
-                                  ctypes.byref(o), _stream()), "lcode_step")
+        fn = lib.lcode_step_mixed if mixed else lib.lcode_step
+        _lib.check(fn(ws, ctypes.byref(cs), ctypes.byref(c), ctypes.byref(vp),
+                      ctypes.byref(p), beam_ptr, beam_scalar,
+                      ctypes.byref(o), _stream()),
+                   "lcode_step_mixed" if mixed else "lcode_step")
     else:
         g, keep_g = _guess_struct(config, guess)
-        _lib.check(lib.lcode_step_guided(ws, ctypes.byref(cs),
-                                         ctypes.byref(c), ctypes.byref(vp),
-                                         ctypes.byref(p), beam_ptr,
-                                         beam_scalar, ctypes.byref(g),
-                                         float(threshold), ctypes.byref(o),
-                                         _stream()), "lcode_step_guided")
+        fn = lib.lcode_step_guided_mixed if mixed else lib.lcode_step_guided
+        _lib.check(fn(ws, ctypes.byref(cs), ctypes.byref(c), ctypes.byref(vp),
+                      ctypes.byref(p), beam_ptr, beam_scalar, ctypes.byref(g),
+                      float(threshold), ctypes.byref(o), _stream()),
+                   "lcode_step_guided_mixed" if mixed else "lcode_step_guided")
     state = GPUArrays.__new__(GPUArrays)
     for name, t in new.items():
         setattr(state, name, t)
     return state
+
+
+def predictor_fields(config, const, virt_params, prev, beam_ro):
+    """The six fields of the plain step's predictor pass (lcode.py:841-874)
+    from `prev` (FP64 state), composed from this module's operators: what a
+    perfect guess of step(..., guess=...) is (the training target of the
+    learned predictor)."""
+    c = const
+    x, y = move_estimate_wo_fields(config, c.m, c.x_init, c.y_init,
+                                   prev.x_offt, prev.y_offt, prev.px, prev.py, prev.pz)
+    x, y, px, py, pz = move_smart(config, c.m, c.q, c.x_init, c.y_init,
+                                  prev.x_offt, prev.y_offt, x, y,
+                                  prev.px, prev.py, prev.pz,
+                                  *(getattr(prev, k) for k in FIELDS))
+    ro, jx, jy, jz = deposit(config, c.ro_initial, x, y, c.m, c.q, px, py, pz,
+                             virt_params)
+    if config.field_solver_variant_A:
+        ro_in, jz_in = (ro + prev.ro) / 2, (jz + prev.jz) / 2
+    else:
+        ro_in, jz_in = ro, jz
+    Ex, Ey, Bx, By = calculate_Ex_Ey_Bx_By(config, prev.Ex, prev.Ey, prev.Bx, prev.By,
+                                           beam_ro, ro_in, jx, jy, jz_in,
+                                           prev.jx, prev.jy)
+    if config.field_solver_variant_A:
+        Ex, Ey = 2 * Ex - prev.Ex, 2 * Ey - prev.Ey
+        Bx, By = 2 * Bx - prev.Bx, 2 * By - prev.By
+    return (Ex, Ey, calculate_Ez(config, jx, jy), Bx, By, calculate_Bz(config, jx, jy))
 
 
 def _guess_struct(config, guess):
@@ -486,7 +532,7 @@ def _guess_struct(config, guess):
     for t, (coef, fields) in enumerate(terms):
         if not isinstance(fields, (tuple, list)):
             fields = tuple(getattr(fields, name) for name in FIELDS)
-        arrs = _check_grid(config, *fields)
+        arrs = _check_grid(config, *fields, dtype=grid_dtype(config))
         if len(arrs) != 6:
             raise ValueError("a guess term needs the six fields Ex..Bz")
         keep.extend(arrs)
@@ -562,25 +608,60 @@ class GuidedStepper:
     """step() with a guess on the slices GuessBackoff selects: the guess is
     the quadratic extrapolation of the last three states' fields, or the
     fields a predictor(s1, s2, s3, beam) returns (states n-1, n-2, n-3;
-    e.g. predictor.ConvPredictor.guess).  Each guided slice reads the
-    corrector's residual back (one synchronisation) to decide on the next
-    ones; the others run the plain step, asynchronously.  Counts guided,
-    accepted and plain slices."""
+    e.g. predictor.ConvPredictor.guess).  The acceptance of a guess is
+    decided on the device (the plain predictor runs there when the residual
+    is above the threshold), so no slice waits for the host: each guided
+    slice enqueues an asynchronous copy of its residual sums into a pinned
+    ring and records an event, and the backoff policy observes outcomes as
+    their copies complete (`lag` slices late at most, after which it
+    waits; sync=True: each guided slice waits for its own outcome, so the
+    policy's decisions do not depend on timing).  Counts guided, accepted
+    and plain slices; `residuals` lists the observed residuals in slice
+    order."""
 
     def __init__(self, config, const, virt_params, threshold, predictor=None,
-                 backoff=8, max_backoff=64):
+                 backoff=8, max_backoff=64, lag=8, sync=False):
         self.config, self.const, self.vp = config, const, virt_params
         self.threshold = float(threshold)
         self.predictor = predictor
         self.policy = GuessBackoff(backoff, max_backoff)
         self.extrap = FieldExtrapolator(order=2)
         self.guided = self.accepted = self.plain = 0
+        self.residuals = []
+        self.lag = max(int(lag), 1)
+        self.sync = bool(sync)
+        self._ring = torch.zeros((self.lag, 2), dtype=_F64, pin_memory=True)
+        self._events = [None] * self.lag
+        self._pending = []           # ring slots in slice order
+        self._next = 0
 
     def push(self, state):
         """Record a state computed outside the stepper (history only)."""
         self.extrap.push(state)
 
+    def _observe(self, slot):
+        num, den = (float(v) for v in self._ring[slot])
+        r = math.sqrt(num / den) if den > 0 else (math.inf if num > 0 else 0.)
+        ok = r <= self.threshold
+        self.residuals.append(r)
+        self.accepted += int(ok)
+        self.policy.observe(ok)
+
+    def drain(self, wait=False):
+        """Feed the policy every completed residual copy (all of them, in
+        order, when wait is true)."""
+        while self._pending:
+            slot = self._pending[0]
+            ev = self._events[slot]
+            if wait:
+                ev.synchronize()
+            elif not ev.query():
+                break
+            self._pending.pop(0)
+            self._observe(slot)
+
     def step(self, prev, beam_ro):
+        self.drain()
         h = self.extrap.history
         ready = len(h) >= 3
         if ready and self.policy.want_guess():
@@ -590,11 +671,25 @@ class GuidedStepper:
                 guess = [(1.0, self.predictor(h[0], h[1], h[2], beam_ro))]
             st = step(self.config, self.const, self.vp, prev, beam_ro,
                       guess=guess, threshold=self.threshold)
-            _, nf, _ = _virt_struct(self.vp)    # the workspace step() used
-            ok = guided_residual(self.config, prev.x_offt.shape[0], nf) <= self.threshold
-            self.policy.observe(ok)
+            if len(self._pending) == self.lag:
+                # every slot in flight: wait for the oldest
+                old = self._pending.pop(0)
+                self._events[old].synchronize()
+                self._observe(old)
+            slot = self._next
+            self._next = (self._next + 1) % self.lag
+            _, nf, _ = _virt_struct(self.vp)
+            ws = _workspace(_config_struct(self.config, prev.x_offt.shape[0], nf))
+            _lib.check(_lib.load().lcode_step_guided_residual_async(
+                ws, ctypes.c_void_p(self._ring[slot].data_ptr()), _stream()),
+                "lcode_step_guided_residual_async")
+            ev = torch.cuda.Event()
+            ev.record()
+            self._events[slot] = ev
+            self._pending.append(slot)
             self.guided += 1
-            self.accepted += int(ok)
+            if self.sync:
+                self.drain(wait=True)
         else:
             st = step(self.config, self.const, self.vp, prev, beam_ro)
             self.plain += 1
@@ -631,30 +726,28 @@ def make_fine_plasma_grid(steps, step_size, fineness=2):
 
 
 def make_plasma(steps, cell_size, coarseness=3, fineness=2):
-    """lcode.py:490-573: coarse particle arrays (device) and the
-    coarse->fine interpolation parameters (virt_params)."""
-    coarse = make_coarse_plasma_grid(steps, cell_size, coarseness)
-    fine = make_fine_plasma_grid(steps, cell_size, fineness)
-    nc = len(coarse)
-    nxt = np.clip(np.searchsorted(coarse, fine), 0, nc - 1)
-    prv = np.clip(np.searchsorted(coarse, fine) - 1, 0, nc - 1)
-    step = cell_size * coarseness
-    w_prev = (coarse[nxt] - fine) / step
-    w_next = (fine - coarse[prv]) / step
-    # clipped neighbours at both ends use the one existing particle
-    w_prev[nxt == 0], w_next[nxt == 0] = 0, 1
-    w_next[prv == nc - 1], w_prev[prv == nc - 1] = 0, 1
-    virt_params = GPUArrays(influence_prev=w_prev, influence_next=w_next,
-                            indices_prev=prv.astype(np.int64),
-                            indices_next=nxt.astype(np.int64),
-                            fine_grid=fine)
+    """lcode.py:490-573 on the device (lcode_make_plasma): coarse particle
+    arrays and the coarse->fine interpolation parameters (virt_params),
+    bit for bit the reference's NumPy values."""
+    assert coarseness == int(coarseness) and fineness == int(fineness)
+    lib = _lib.load()
+    nc, nf = ctypes.c_int32(), ctypes.c_int32()
+    _lib.check(lib.lcode_plasma_sizes(int(steps), int(coarseness), int(fineness),
+                                      ctypes.byref(nc), ctypes.byref(nf)),
+               "lcode_plasma_sizes")
+    nc, nf = nc.value, nf.value
     dev = _device()
     shape = (nc, nc)
-    x_init = torch.as_tensor(coarse, device=dev)[:, None].expand(shape).contiguous()
-    y_init = torch.as_tensor(coarse, device=dev)[None, :].expand(shape).contiguous()
+    x_init, y_init, m, q = (torch.empty(shape, dtype=_F64, device=dev) for _ in range(4))
+    ip, inn, fg = (torch.empty(nf, dtype=_F64, device=dev) for _ in range(3))
+    prv, nxt = (torch.empty(nf, dtype=torch.int64, device=dev) for _ in range(2))
+    _lib.check(lib.lcode_make_plasma(int(steps), float(cell_size), int(coarseness),
+                                     int(fineness), *(_ptr(t) for t in (
+                                         x_init, y_init, m, q, ip, inn, fg, prv, nxt)),
+                                     _stream()), "lcode_make_plasma")
+    virt_params = GPUArrays(influence_prev=ip, influence_next=inn,
+                            indices_prev=prv, indices_next=nxt, fine_grid=fg)
     z = [torch.zeros(shape, dtype=_F64, device=dev) for _ in range(5)]
-    m = torch.full(shape, float(ELECTRON_MASS * coarseness**2), dtype=_F64, device=dev)
-    q = torch.full(shape, float(ELECTRON_CHARGE * coarseness**2), dtype=_F64, device=dev)
     return (x_init, y_init, z[0], z[1], z[2], z[3], z[4], m, q, virt_params)
 
 
@@ -679,7 +772,7 @@ def init(config):
     dev = _device()
 
     def zeros():
-        return torch.zeros((n, n), dtype=_F64, device=dev)
+        return torch.zeros((n, n), dtype=grid_dtype(config), device=dev)
 
     state = GPUArrays(x_offt=x_offt, y_offt=y_offt, px=px, py=py, pz=pz,
                       Ex=zeros(), Ey=zeros(), Ez=zeros(),

@@ -1,10 +1,10 @@
-"""CPU oracle of the learned field predictor's arithmetic.
+"""CPU oracle of the learned field predictor's building blocks.
 
-TEST INFRASTRUCTURE ONLY.  The checker for the predictor kernels of
-``lcode3d_amd/csrc/predictor.hip``; only ``tests/`` import it.  The reference
-has no learned predictor (its predictor pass is ``lcode.py:841-881``), so
-there is no reference line to follow: each function restates a textbook
-operation, in FP64 NumPy.
+TEST INFRASTRUCTURE ONLY.  Used by the U-Net oracle (oracle/unet_oracle.py,
+the checker of ``lcode3d_amd/csrc/unet.hip``) and by ``tests/``.  The
+reference has no learned predictor (its predictor pass is
+``lcode.py:841-881``), so there is no reference line to follow: each
+function restates a textbook operation, in FP64 NumPy.
 
 * ``conv3x3`` - 3 x 3 cross-correlation with zero padding ("same" size),
   out[co](i, j) = b[co] + sum_ci sum_{di,dj} w[co, ci, di, dj]
@@ -12,6 +12,8 @@ operation, in FP64 NumPy.
 * ``conv3x3_backward`` - its adjoints: the input gradient is the full
   correlation of the output gradient with the flipped kernel, the weight
   gradient the correlation of the input with the output gradient.
+* ``forward``/``backward`` - a plain stack of such layers (with ReLU), the
+  smallest composition the adjoints are pinned through.
 * ``adam`` - Kingma & Ba, "Adam: a method for stochastic optimization"
   (2015), Algorithm 1, with bias-corrected moments.
 

@@ -7,13 +7,17 @@
   within 1e-10 of its peak, and the centre row and column of all 15 state
   arrays every 100 slices within relative L2 1e-10 (Bz against the Bx
   row's norm).
-* 641^2 with the reduced-precision field solvers (FP32 FFT, dense BF16x3)
-  for beam amplitude {0.01, 0.05, 0.1} x sigma {0.5, 1, 2}, 300 slices
-  each, against the plain FP64 HIP step of the same beam (itself pinned by
-  the first test) every 100 slices.  Bounds (relative L2 over the whole
-  array; Bz against ||Bx||): Ez <= 1e-4 in every mode; FP32 FFT: other
-  fields <= 1e-4, momenta <= 1e-5; BF16x3: Ex Ey Bx By <= 2e-3, Bz <= 1e-4,
-  momenta <= 1e-4.  The measured worst values are printed.
+* 641^2 with the reduced-precision field solvers (FP32 FFT, the mixed
+  form with FP32 grids and deposition, dense BF16x3) for beam amplitude
+  {0.01, 0.05, 0.1} x sigma {0.5, 1, 2}, 300 slices each, against the plain
+  FP64 HIP step of the same beam (itself pinned by the first test) every
+  100 slices.  Bounds (relative L2 over the whole array; Bz against
+  ||Bx||): Ez <= 1e-4 in every mode; FP32 FFT: other fields <= 1e-4,
+  momenta <= 1e-5; mixed: Ex Ey Bx By <= 2e-3, Bz <= 1e-4, momenta <= 2e-4;
+  BF16x3: Ex Ey Bx By <= 2e-3, Bz <= 1e-4, momenta <= 1e-4.  The measured
+  worst values are printed.
+* the mixed form with the guided predictor against the plain FP64 step
+  over 600 slices.
 
 The beam is the oracle's expression evaluated the same way: the Gaussian
 profile amplitude * exp(-r^2 / (2 sigma^2)) computed on the host once, then
@@ -108,6 +112,7 @@ def test_plain_641_1500_slices_against_oracle_fixture(lc):
 
 BOUNDS = {
     ("fft", "fp32"): dict(Ez=1e-4, fields=1e-4, Bz=1e-4, momenta=1e-5),
+    ("fft", "mixed"): dict(Ez=1e-4, fields=2e-3, Bz=1e-4, momenta=2e-4),
     ("dense", "bf16x3"): dict(Ez=1e-4, fields=2e-3, Bz=1e-4, momenta=1e-4),
 }
 
@@ -153,3 +158,47 @@ def test_reduced_precision_641_beam_grid(lc, transform, precision):
     assert worst["Bz"] < bounds["Bz"]
     for name in ("px", "py", "pz"):
         assert worst[name] < bounds["momenta"], (name, worst[name])
+
+
+def test_mixed_guided_641_against_plain(lc):
+    """The mixed-precision step with the guided predictor (quadratic
+    extrapolation, corrector residual threshold 2e-3) against the plain
+    FP64 step over 600 slices of the 641^2 workload (amplitude 0.05,
+    sigma 1): Ez(xi, 0, 0) within relative L2 1e-4 and the full Ez within
+    1e-4 at every 100-slice checkpoint; the transverse fields pass a guess's
+    error through the corrector almost unattenuated, so Ex Ey within 2e-2,
+    Bx By within 2e-1 (relative to their own, small, norms); momenta within
+    1e-4; guesses are accepted on most guided slices."""
+    cfg64 = _cfg641()
+    cfgm = _cfg641(field_solver_precision="mixed")
+    _, _, const, vp, sp = lc.init(cfg64)
+    _, _, constm, vpm, sm = lc.init(cfgm)
+    gs = lc.GuidedStepper(cfgm, constm, vpm, 2e-3)
+    beam = beam_source(cfg64, .05, 1.0)
+    c = cfg64.grid_steps // 2
+    ezp, ezm, worst = [], [], {}
+    for xi_i in range(600):
+        b = beam(xi_i)
+        sp = lc.step(cfg64, const, vp, sp, b)
+        sm = gs.step(sm, b)
+        ezp.append(sp.Ez[c, c])
+        ezm.append(sm.Ez[c, c])
+        if (xi_i + 1) % 100 == 0:
+            for name in ("Ex", "Ey", "Ez", "Bx", "By", "Bz", "px", "py", "pz"):
+                r = host(getattr(sp, name))
+                floor = np.linalg.norm(host(sp.Bx)) if name == "Bz" else 0.0
+                e = rel_l2(host(getattr(sm, name)).astype(np.float64), r, floor)
+                worst[name] = max(worst.get(name, 0.0), e)
+    gs.drain(wait=True)
+    ezp = torch.stack(ezp)
+    ezm = torch.stack(ezm).double()
+    axis = (torch.linalg.norm(ezm - ezp) / torch.linalg.norm(ezp)).item()
+    say(f"mixed+guided 641^2 x 600: Ez axis {axis:.1e}; {gs.accepted} of {gs.guided} "
+        f"accepted, {gs.plain} plain; worst", {k: f"{v:.1e}" for k, v in worst.items()})
+    assert axis < 1e-4
+    assert worst["Ez"] < 1e-4 and worst["Bz"] < 1e-4
+    assert worst["Ex"] < 2e-2 and worst["Ey"] < 2e-2
+    assert worst["Bx"] < 2e-1 and worst["By"] < 2e-1
+    for name in ("px", "py", "pz"):
+        assert worst[name] < 1e-4, (name, worst[name])
+    assert gs.accepted > gs.guided // 2

@@ -132,7 +132,8 @@ def test_stepper_backs_off_after_rejections(lc):
     cfg = _cfg()
     xs, ys, const, vp, st_plain = lc.init(cfg)
     st = st_plain
-    gs = lc.GuidedStepper(cfg, const, vp, threshold=0.0, backoff=2, max_backoff=8)
+    gs = lc.GuidedStepper(cfg, const, vp, threshold=0.0, backoff=2, max_backoff=8,
+                          sync=True)
     guided_at = []
     for i in range(40):
         beam = _beam(lc, cfg, i)

@@ -78,3 +78,50 @@ def test_fp32_step_129_200_slices(lc, oracle):
         assert e < 1e-4, (name, e)
     ez_o = np.array(ez_o)
     assert np.abs(np.array(ez_g) - ez_o).max() < 1e-4 * np.abs(ez_o).max()
+
+
+def test_mixed_step_129_200_slices(lc, oracle):
+    """The mixed-precision step (FP32 grids, deposition arithmetic and
+    transforms; FP64 particles, RHS arithmetic, Bz mean, ion background)
+    against the FP64 oracle over the 129^2 workload: every field within
+    relative L2 1e-3, momenta 1e-4, Ez(xi, 0, 0) within 1e-4 of its peak."""
+    cfg_o = _cfg(129, "fp64", grid_step_size=.05)
+    cfg_g = _cfg(129, "mixed", grid_step_size=.05)
+    xs, ys, const, vp, st = oracle.init(cfg_o)
+    _, _, gconst, gvp, gst = lc.init(cfg_g)
+    assert gst.Ex.dtype == torch.float32 and gst.x_offt.dtype == torch.float64
+    worst, ez_o, ez_g = {}, [], []
+    for xi_i in range(200):
+        beam = oracle.beam(cfg_o, xi_i, xs, ys)
+        st = oracle.step(cfg_o, const, vp, st, beam)
+        gst = lc.step(cfg_g, gconst, gvp, gst, beam)
+        ez_o.append(st.Ez[64, 64])
+        ez_g.append(gst.Ez[64, 64].item())
+        if (xi_i + 1) % 50 == 0:
+            for name in STATE:
+                floor = np.linalg.norm(st.Bx) if name == "Bz" else 0.
+                e = rel_l2(host(getattr(gst, name)).astype(np.float64),
+                           getattr(st, name), floor)
+                worst[name] = max(worst.get(name, 0.), e)
+    say("mixed 129^2 worst rel L2:", {k: f"{v:.1e}" for k, v in worst.items()})
+    for name, e in worst.items():
+        bound = 1e-4 if name in ("px", "py", "pz", "x_offt", "y_offt") else 1e-3
+        assert e < bound, (name, e)
+    ez_o = np.array(ez_o)
+    assert np.abs(np.array(ez_g) - ez_o).max() < 1e-4 * np.abs(ez_o).max()
+    assert lc.fault_count(cfg_g, gconst.m.shape[0]) == 0
+
+
+def test_mixed_state_dtypes_checked(lc):
+    """A mixed config takes float32 grids: an FP64 state is refused before
+    any launch, and the standalone operators stay FP64."""
+    cfg = _cfg(33, "mixed")
+    _, _, const, vp, st = lc.init(cfg)
+    cfg64 = _cfg(33, "fp64")
+    _, _, _, _, st64 = lc.init(cfg64)
+    with pytest.raises(ValueError, match="float32"):
+        lc.step(cfg, const, vp, st64, 0)
+    with pytest.raises(ValueError, match="float64"):
+        lc.step(cfg64, const, vp, st, 0)
+    ez = lc.calculate_Ez(cfg, torch.zeros_like(st64.Ex), torch.zeros_like(st64.Ex))
+    assert ez.dtype == torch.float64
