@@ -45,6 +45,12 @@ def algorithmic_bytes_per_step(nc, n):
 
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+#: the step's dominant kernel, timed with HIP events inside the timed region
+ROOFLINE_KERNEL = "deposit_tile"
+EVENT_EVERY = 4
+#: the plain step's kernels, timed one name at a time after the timed region
+KERNEL_NAMES = ("deposit_tile", "solve_pass_a", "solve_pass_b", "solve_pass_c",
+                "push", "deposit_spill")
 
 
 def kernel_bytes(name, nc, n):
@@ -105,7 +111,8 @@ MFMA_PEAK_TFS = {"fp64": 78.6, "bf16x3": 2500.0}
 def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps):
     """The same step with each field-solver back-end from the same developed
     state: whole-step throughput over `steps` slices and the six-field solve
-    time from HIP events around every launch over `prof_steps` slices."""
+    time from HIP events around each solve kernel's launches (one kernel
+    name per pass of `prof_steps` slices)."""
     import copy
     legs = []
     for transform, precision in (("fft", "fp64"), ("fft", "fp32"), ("fft", "mixed"),
@@ -125,13 +132,19 @@ def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps):
             st = lc.step(c, const, vp, st, beams[k % len(beams)])
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        lc.profile_begin(64 * prof_steps)
-        for k in range(prof_steps):
-            st = lc.step(c, const, vp, st, beams[k % len(beams)])
-        prof = lc.profile_end()
-        kus = {k: v[1] / v[0] * 1e3 for k, v in prof.items()
-               if k in SOLVE_KERNELS}
-        calls = {k: v[0] / (2 * prof_steps) for k, v in prof.items() if k in SOLVE_KERNELS}
+        # one solve kernel name at a time (see the headline's kernel table)
+        kus, calls = {}, {}
+        names = SOLVE_KERNELS[:3] if transform == "fft" else SOLVE_KERNELS[3:]
+        for name in names:
+            for k in range(2):
+                st = lc.step(c, const, vp, st, beams[k % len(beams)])
+            lc.profile_begin(16 * prof_steps, only=name)
+            for k in range(prof_steps):
+                st = lc.step(c, const, vp, st, beams[k % len(beams)])
+            prof = lc.profile_end()
+            if name in prof:
+                kus[name] = prof[name][1] / prof[name][0] * 1e3
+                calls[name] = prof[name][0] / (2 * prof_steps)
         solve_us = sum(kus[k] * calls[k] for k in kus)
         leg = {"transform": transform, "precision": precision,
                "xi_steps_per_s": steps / dt, "solve_us": solve_us,
@@ -485,8 +498,9 @@ def main():
     total = args.warmup + args.steps
     w0 = args.settle + args.warmup
     # beam slices of the run, resident in HBM before the clock starts
-    beams = run_beams(cfg, cosine_gaussian, max(RUN_SLICES, w0 + args.steps
-                                                + args.profile_steps))
+    beams = run_beams(cfg, cosine_gaussian,
+                      max(RUN_SLICES, w0 + args.steps
+                          + len(KERNEL_NAMES) * (args.profile_steps + 2)))
     for k in range(args.settle):  # bring the plasma into the developed wake
         state = lc.step(cfg, const, vp, state, beams[k])
     torch.cuda.synchronize()
@@ -495,11 +509,18 @@ def main():
     for k in range(args.warmup):
         state = lc.step(cfg, const, vp, state, beams[args.settle + k])
     torch.cuda.synchronize()
+    # the timed region carries a pair of HIP events on the roofline kernel's
+    # launches in every EVENT_EVERY-th step: events attached to the
+    # dispatch (hipExtLaunchKernelGGL, on the step's stream), which the
+    # runtime stamps with the kernel's own start and end
+    lc.profile_begin(4 * args.steps, only=ROOFLINE_KERNEL)
     t0 = time.perf_counter()
     for k in range(args.steps):
+        lc.profile_pause(k % EVENT_EVERY != 0)
         state = lc.step(cfg, const, vp, state, beams[w0 + k])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    timed_prof = lc.profile_end()
     finite = bool(torch.isfinite(state.Ez).all().item())
     disp = max(state.x_offt.abs().max().item(), state.y_offt.abs().max().item())
     log(f"max coarse displacement {disp / cfg.grid_step_size:.2f} cells")
@@ -512,50 +533,67 @@ def main():
     log(f"{steps_per_s:.1f} xi-steps/s, {ms:.3f} ms/step, "
         f"step-level {step_bytes / (dt / args.steps) / 1e9:.1f} GB/s algorithmic")
 
-    # per-kernel times over the next slices: HIP events around every
-    # launch on the step's stream (an event pair also times the launch's
-    # dispatch, so these durations are upper bounds of the kernels' own;
-    # the rocprofv3 kernel trace in profiles/ gives the kernels alone)
+    # the other kernels' durations over the next slices, one kernel name
+    # at a time (attached events), each pass led by two untimed steps
     roofline, kernels = None, {}
+    calls, kms = timed_prof.get(ROOFLINE_KERNEL, (0, 0.0))
+    if calls:
+        avg_us = kms / calls * 1e3
+        kernels[ROOFLINE_KERNEL] = {
+            "launches_per_step": calls / len(range(0, args.steps, EVENT_EVERY)),
+            "avg_us": avg_us,
+            "window": f"timed region, every {EVENT_EVERY}th step ({calls} launches)"}
     if args.profile_steps:
-        lc.profile_begin(64 * args.profile_steps)
-        for k in range(args.profile_steps):
-            state = lc.step(cfg, const, vp, state, beams[w0 + args.steps + k])
-        prof = lc.profile_end()
-        event_sum = 0.0
-        for name, (calls, kms) in sorted(prof.items(), key=lambda kv: -kv[1][1]):
-            avg_us = kms / calls * 1e3
+        sl = w0 + args.steps
+        for name in KERNEL_NAMES:
+            if name == ROOFLINE_KERNEL and calls:
+                continue
+            for k in range(2):
+                state = lc.step(cfg, const, vp, state, beams[sl])
+                sl += 1
+            lc.profile_begin(8 * args.profile_steps, only=name)
+            for k in range(args.profile_steps):
+                state = lc.step(cfg, const, vp, state, beams[sl])
+                sl += 1
+            prof = lc.profile_end()
+            if name in prof:
+                c, t = prof[name]
+                kernels[name] = {"launches_per_step": c / args.profile_steps,
+                                 "avg_us": t / c * 1e3,
+                                 "window": f"slices {sl - args.profile_steps}-{sl - 1}"}
+    if kernels:
+        ksum = sum(v["launches_per_step"] * v["avg_us"] for v in kernels.values())
+        for name, v in kernels.items():
             b = kernel_bytes(name, nc, n)
-            kernels[name] = {"launches_per_step": calls / args.profile_steps,
-                             "avg_us": avg_us,
-                             "share": kms / sum(v[1] for v in prof.values()),
-                             "GBps": (b / (avg_us * 1e-6) / 1e9) if b else None}
-            event_sum += calls / args.profile_steps * avg_us
-        log(f"  kernel durations (HIP events; their sum {event_sum:.1f} us per step "
+            v["share"] = v["launches_per_step"] * v["avg_us"] / ksum
+            v["GBps"] = (b / (v["avg_us"] * 1e-6) / 1e9) if b else None
+        kernels = dict(sorted(kernels.items(), key=lambda kv: -kv[1]["share"]))
+        log(f"  kernel durations (HIP events; their sum {ksum:.1f} us per step "
             f"against {ms * 1e3:.1f} us measured):")
         for name, v in kernels.items():
             log(f"  {name:14s} {v['launches_per_step']:4.1f}/step "
                 f"{v['avg_us']:8.2f} us  {v['share'] * 100:5.1f}%"
-                + (f"  {v['GBps']:7.0f} GB/s" if v["GBps"] else ""))
-        dom = max((k for k in kernels if kernels[k]["GBps"]),
-                  key=lambda k: kernels[k]["share"])
-        ach = kernels[dom]["GBps"]
-        roofline = {"bound": "hbm", "kernel": dom, "achieved": ach,
+                + (f"  {v['GBps']:7.0f} GB/s" if v["GBps"] else "")
+                + f"  ({v['window']})")
+    if ROOFLINE_KERNEL in kernels:
+        v = kernels[ROOFLINE_KERNEL]
+        roofline = {"bound": "hbm", "kernel": ROOFLINE_KERNEL, "achieved": v["GBps"],
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
-                    "bytes_per_launch": kernel_bytes(dom, nc, n),
-                    "avg_us": kernels[dom]["avg_us"]}
-        if "deposit_tile" in kernels:
-            # the reference's deposition rate measure: 4 quantities x 9
-            # stencil cells = 36 atomic adds per fine particle
-            # (lcode.py:434-449, :660-663); the tile kernel sums them in
-            # registers and LDS but carries out the same 36 updates
-            v = kernels["deposit_tile"]
-            updates = 36 * vp.fine_grid.numel() ** 2
-            v["fine_particle_updates_per_launch"] = updates
-            v["fine_particle_updates_per_s"] = updates / (v["avg_us"] * 1e-6)
-            log(f"  deposit_tile: {updates / 1e6:.1f} M fine-particle stencil updates per launch, "
-                f"{v['fine_particle_updates_per_s'] / 1e12:.2f} T updates/s")
+                    "frac": v["GBps"] / HBM_PEAK_GBS,
+                    "traffic": measured_traffic(ROOFLINE_KERNEL),
+                    "bytes_per_launch": kernel_bytes(ROOFLINE_KERNEL, nc, n),
+                    "avg_us": v["avg_us"], "launches": calls}
+    if "deposit_tile" in kernels:
+        # the reference's deposition rate measure: 4 quantities x 9 stencil
+        # cells = 36 atomic adds per fine particle (lcode.py:434-449,
+        # :660-663); the tile kernel sums them in registers and LDS but
+        # carries out the same 36 updates
+        v = kernels["deposit_tile"]
+        updates = 36 * vp.fine_grid.numel() ** 2
+        v["fine_particle_updates_per_launch"] = updates
+        v["fine_particle_updates_per_s"] = updates / (v["avg_us"] * 1e-6)
+        log(f"  deposit_tile: {updates / 1e6:.1f} M fine-particle stencil updates per launch, "
+            f"{v['fine_particle_updates_per_s'] / 1e12:.2f} T updates/s")
     if args.copy_probe_mib:
         src = torch.ones(args.copy_probe_mib << 17, dtype=torch.float64,
                          device="cuda")

@@ -159,10 +159,19 @@ int lcode_workspace_nonfinite(lcode_workspace *ws, int32_t reset,
                               int32_t *flag);
 
 /* Per-kernel timing (instrumentation; no counterpart in the reference).
- * Between begin and end every kernel the library launches is bracketed by
- * HIP events on its stream; end waits for the stream and returns, per
+ * Between begin and end every kernel the library launches is timed with a
+ * pair of HIP events on its stream: for the step's kernels (push,
+ * deposition, spill, the solve passes and the dense products) events
+ * attached to the dispatch (hipExtLaunchKernel), which the runtime stamps
+ * with the kernel's own start and end; for the others events recorded
+ * before and after the launch.  end waits for the stream and returns, per
  * kernel name, the launch count and the summed event time.  max_launches
- * bounds the number of timed launches (later ones run untimed). */
+ * bounds the number of timed launches (later ones run untimed).
+ * lcode_profile_only, called after begin, restricts the timing to the
+ * launches of one kernel name (NULL or "": every kernel again).
+ * lcode_profile_pause(ws, 1) stops timing until lcode_profile_pause(ws, 0)
+ * (launches in between run untimed), so that a long loop can time a
+ * sample of its launches. */
 typedef struct {
     char name[32];
     int64_t calls;
@@ -170,6 +179,8 @@ typedef struct {
 } lcode_kernel_time;
 
 int lcode_profile_begin(lcode_workspace *ws, int32_t max_launches);
+int lcode_profile_only(lcode_workspace *ws, const char *kernel_name);
+int lcode_profile_pause(lcode_workspace *ws, int32_t paused);
 int lcode_profile_end(lcode_workspace *ws, hipStream_t stream,
                       lcode_kernel_time *out, int32_t max_out,
                       int32_t *n_out);

@@ -7,6 +7,6 @@ from .lcode import (  # noqa: F401
     calculate_Bz, calculate_Ex_Ey_Bx_By, calculate_Ez, deposit, fault_count,
     init, initial_deposition, make_coarse_plasma_grid, make_fine_plasma_grid,
     make_plasma, move_estimate_wo_fields, move_smart, profile_begin,
-    profile_end, step, guided_residual, FieldExtrapolator, GuessBackoff, GuidedStepper,
+    profile_end, profile_pause, step, guided_residual, FieldExtrapolator, GuessBackoff, GuidedStepper,
     predictor_fields, is_mixed, grid_dtype,
 )

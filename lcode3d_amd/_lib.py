@@ -104,6 +104,8 @@ _SIGNATURES = {
                                 ctypes.POINTER(ctypes.c_void_p)]),
     "lcode_workspace_destroy": (ctypes.c_int, [_vp]),
     "lcode_profile_begin": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "lcode_profile_only": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "lcode_profile_pause": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "lcode_profile_end": (ctypes.c_int,
                           [_vp, _vp, ctypes.POINTER(LcodeKernelTime),
                            ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -58,6 +59,10 @@ struct lcode_workspace {
     int* spill_ctr = nullptr;      // [0], [1] spill counts, [2], [3] the
                                    // pushes' displacement bound in cells,
                                    // each pair of slots used in turn
+    int* regions = nullptr;        // the same bound per deposition tile:
+                                   // two slots of region_n^2 ints
+    int region_n = 0;              // tiles per side (0: no per-tile bounds)
+    int* tile_order = nullptr;     // deposition tiles, centre first
     int bound_slot = 0;            // slot the next push raises
     int spill_slot = 0;            // count slot of the next deposition
     int* host_flag = nullptr;      // pinned host int: a non-finite particle
@@ -99,11 +104,27 @@ namespace {
 #endif
 
 // Cells a fine particle may move from its initial cell and still be
-// deposited through the LDS tiles (beyond: the atomic spill path, which the
-// last block of the kernel runs alone).  The adaptive halo is the pushes'
+// deposited through the LDS tiles (beyond: the spill list, deposited with
+// device atomics by a grid-wide kernel).  The adaptive halo is the pushes'
 // displacement bound, so small displacements keep small tiles; 6 covers the
 // nonlinear 1281^2 case (up to ~4.4 cells) within the LDS tables.
 constexpr int kDepositHalo = 6;
+// 1: per-tile displacement bounds size each deposition tile's halo; 0: the
+// largest bound over all particles sizes every tile's halo (build option)
+#ifndef LCODE_DEP_REGIONS
+#define LCODE_DEP_REGIONS 1
+#endif
+constexpr bool kDepRegions = LCODE_DEP_REGIONS;
+// 1: deposition blocks take the tiles in order of Chebyshev distance from
+// the grid centre (the wake of an axial beam makes those the slowest, so
+// they would start first when there are more tiles than block slots); 0:
+// row-major (build option).  Measured at 641^2 in the developed wake: no
+// change with 24-cell tiles (43.03 us either way: every tile is resident
+// at once), slower with 20-cell ones (51.2 against 46.9 us); off.
+#ifndef LCODE_DEP_ORDER
+#define LCODE_DEP_ORDER 0
+#endif
+constexpr bool kDepOrder = LCODE_DEP_ORDER;
 
 // Round-to-nearest-even BF16 bits of a float (finite inputs).
 uint16_t bf16_bits(float f) {
@@ -509,6 +530,33 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     const size_t nfp = (size_t)cfg->nf * cfg->nf;
     if ((rc = dev_alloc(ws, (void**)&ws->spill, (nfp ? nfp : 1) * sizeof(int)))) return fail(rc);
     if ((rc = dev_alloc(ws, (void**)&ws->spill_ctr, 4 * sizeof(int)))) return fail(rc);
+    // per-tile displacement bounds: a deposition tile takes the largest of
+    // its 3 x 3 neighbourhood, which holds every coarse particle a fine
+    // particle within the largest halo + 1 cells of the tile interpolates
+    // from when 2 coarseness + 10 <= tile side (a fine particle lies within
+    // 2 coarseness + 2 cells of its coarse neighbours, at the plasma edge)
+    const int nt = (N + kDepTile - 1) / kDepTile;
+    if (kDepRegions && 2 * cfg->plasma_coarseness + kDepositHalo + 4 <= kDepTile) {
+        ws->region_n = nt;
+        if ((rc = dev_alloc(ws, (void**)&ws->regions, 2 * (size_t)nt * nt * sizeof(int))))
+            return fail(rc);
+    }
+    if (kDepOrder) {
+        std::vector<int> order((size_t)nt * nt);
+        for (int k = 0; k < nt * nt; ++k) order[k] = k;
+        const double mid = (nt - 1) / 2.;
+        auto dist = [&](int k) {
+            return std::max(std::fabs(k / nt - mid), std::fabs(k % nt - mid));
+        };
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int u, int v) { return dist(u) < dist(v); });
+        if ((rc = dev_alloc(ws, (void**)&ws->tile_order, order.size() * sizeof(int))) ||
+            (rc = hipMemcpy(ws->tile_order, order.data(), order.size() * sizeof(int),
+                            hipMemcpyHostToDevice) == hipSuccess ? LCODE_OK : LCODE_E_HIP)) {
+            if (rc == LCODE_E_HIP) set_error("hipMemcpy of the tile order failed");
+            return fail(rc);
+        }
+    }
     if (hipHostMalloc((void**)&ws->host_flag, sizeof(int), hipHostMallocDefault) != hipSuccess) {
         ws->host_flag = nullptr;
         set_error("hipHostMalloc of the non-finite flag failed");
@@ -517,6 +565,8 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     *ws->host_flag = 0;
     if (hipMemset(ws->fault, 0, sizeof(int)) != hipSuccess ||
         hipMemset(ws->spill_ctr, 0, 4 * sizeof(int)) != hipSuccess ||
+        (ws->regions &&
+         hipMemset(ws->regions, 0, 2 * (size_t)nt * nt * sizeof(int)) != hipSuccess) ||
         hipDeviceSynchronize() != hipSuccess) {
         set_error("hipMemset of the workspace counters failed");
         return fail(LCODE_E_HIP);
@@ -562,6 +612,20 @@ int lcode_profile_begin(lcode_workspace* ws, int32_t max_launches) {
     p.capacity = max_launches;
     p.used = 0;
     p.on = true;
+    return LCODE_OK;
+}
+
+int lcode_profile_only(lcode_workspace* ws, const char* kernel_name) {
+    LCODE_REQUIRE(ws, "NULL workspace");
+    const char* n = kernel_name ? kernel_name : "";
+    LCODE_REQUIRE(std::strlen(n) < sizeof(ws->prof.only), "kernel name too long");
+    std::strcpy(ws->prof.only, n);
+    return LCODE_OK;
+}
+
+int lcode_profile_pause(lcode_workspace* ws, int32_t paused) {
+    LCODE_REQUIRE(ws, "NULL workspace");
+    ws->prof.paused = paused != 0;
     return LCODE_OK;
 }
 
@@ -669,6 +733,11 @@ static int do_push(lcode_workspace* ws, const lcode_config* cfg,
     a.out_px = out[2]; a.out_py = out[3]; a.out_pz = out[4];
     a.fault = ws->fault;
     a.max_cells = ws->spill_ctr + 2 + ws->bound_slot;
+    if (ws->regions) {
+        a.region_max = ws->regions + (size_t)ws->bound_slot * ws->region_n * ws->region_n;
+        a.region_n = ws->region_n;
+        a.region_t = kDepTile;
+    }
     a.host_flag = ws->host_flag;
     a.grid32 = ws->grid32;
     a.gate = gate;
@@ -740,12 +809,18 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.host_flag = ws->host_flag;
     a.spill_count = ws->spill_ctr + ws->spill_slot;
     a.spill_next = ws->spill_ctr + (1 - ws->spill_slot);
+    a.tile_order = ws->tile_order;
     // after a push of this workspace: a halo of the pushes' displacement
     // bound, read from the slot the pushes raised; the deposition zeroes
     // the other slot, which the next pushes raise
     if (adaptive_halo) {
         a.halo_cells = ws->spill_ctr + 2 + ws->bound_slot;
         a.halo_next = ws->spill_ctr + 2 + (1 - ws->bound_slot);
+        if (ws->regions) {
+            const size_t nr = (size_t)ws->region_n * ws->region_n;
+            a.halo_regions = ws->regions + ws->bound_slot * nr;
+            a.halo_regions_next = ws->regions + (1 - ws->bound_slot) * nr;
+        }
     }
     a.gate = gate;
     LCODE_REQUIRE(cfg->nf <= ws->nf, "workspace was built for fewer fine particles");

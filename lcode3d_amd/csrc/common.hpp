@@ -4,9 +4,11 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <type_traits>
 #include <cstdint>
+#include <cstring>
 #include <string>
 
 #include "../../include/lcode3d_hip.h"
@@ -47,25 +49,51 @@ struct Profiler {
     int used = 0;
     hipEvent_t* ev = nullptr;         // 2 * capacity events
     const char** names = nullptr;     // name of pair k
+    char only[32] = {0};              // time only this kernel name ("": all)
+    bool paused = false;
 };
 
-// Brackets one kernel launch; a no-op when timing is off or full.
+// Brackets one kernel launch; a no-op when timing is off or full, or when
+// the launch is not the one kernel being timed.  By default it records an
+// event before and after the launch (marker packets: the pair then also
+// spans the launch's dispatch, a few us).  With `attached` it records
+// nothing itself: the caller launches through hipExtLaunchKernelGGL with
+// start() and stop(), and the runtime stamps the two events with the
+// kernel's own start and end, as a kernel trace does.
 struct ProfScope {
     Profiler* p;
     hipStream_t s;
     int k = -1;
-    ProfScope(Profiler* prof, const char* name, hipStream_t stream)
-        : p(prof), s(stream) {
-        if (p && p->on && p->used < p->capacity) {
+    bool attached = false;
+    ProfScope(Profiler* prof, const char* name, hipStream_t stream,
+              bool attach = false)
+        : p(prof), s(stream), attached(attach) {
+        if (p && p->on && !p->paused && p->used < p->capacity &&
+            (!p->only[0] || std::strcmp(p->only, name) == 0)) {
             k = p->used++;
             p->names[k] = name;
-            (void)hipEventRecord(p->ev[2 * k], s);
+            if (!attached) (void)hipEventRecord(p->ev[2 * k], s);
         }
     }
+    bool timed() const { return k >= 0; }
+    hipEvent_t start() const { return p->ev[2 * k]; }
+    hipEvent_t stop() const { return p->ev[2 * k + 1]; }
     ~ProfScope() {
-        if (k >= 0) (void)hipEventRecord(p->ev[2 * k + 1], s);
+        if (k >= 0 && !attached) (void)hipEventRecord(p->ev[2 * k + 1], s);
     }
 };
+
+// Launches kernel<<<grid, block, lds, stream>>>(args...), timed with
+// events attached to the dispatch when `ps` is timing this launch.
+template <typename K, typename... Args>
+inline void launch_timed(const ProfScope& ps, K kernel, dim3 grid, dim3 block,
+                         size_t lds, hipStream_t stream, Args... args) {
+    if (ps.timed())
+        hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)lds, stream,
+                              ps.start(), ps.stop(), 0u, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, lds, stream, args...);
+}
 
 // ---------------------------------------------------------------------------
 // Device-side gate of the guided step's fallback kernels
@@ -85,11 +113,14 @@ __device__ __forceinline__ bool gate_open(const Gate& g) {
 }
 
 // Deposition tile side in cells and threads per tile block (build options).
-// At 641^2, coarseness 3, with grouped particles: 28 x 28 cells with 256
-// threads (three blocks per CU: 165 VGPRs, 51 KiB LDS each) 43.8 us per
-// launch; 512 threads 58.9 us; 24 or 32 cells 49 us.
+// At 641^2, coarseness 3, with grouped particles, 256 threads per block
+// (165 VGPRs: three blocks per CU), in the developed wake (slices 420-719,
+// HIP events on a sample of the launches, three boxes): 24 cells (729
+// tiles, all resident at once) 43.0-43.2 us per launch, 28 cells (529
+// tiles) 43.7-47.8 us, 30-36 cells 49-58 us, 16-22 cells 44-51 us (more
+// tiles than block slots); 512 threads 54-59 us.
 #ifndef LCODE_DEP_TILE
-#define LCODE_DEP_TILE 28
+#define LCODE_DEP_TILE 24
 #endif
 #ifndef LCODE_DEP_BLOCK
 #define LCODE_DEP_BLOCK 256
@@ -231,6 +262,10 @@ struct PushArgs {
     double *out_x, *out_y, *out_px, *out_py, *out_pz;
     int* fault;
     int* max_cells;                // atomic max of ceil(|offset| / h), or NULL
+    // the same bound per deposition tile of the particles' initial cells
+    // ([region_n][region_n] of region_t-cell squares), or NULL
+    int* region_max;
+    int region_n, region_t;
     // non-finite (or beyond kFarCells) particle state: flagged in
     // host-visible memory (NULL: not reported)
     int* host_flag;
@@ -272,6 +307,14 @@ struct DepositArgs {
     int* halo_cells;               // NULL, or the pushes' displacement bound
     int* halo_next;                // NULL, or the bound slot of the next
                                    // pushes: set to 0 by the deposition
+    // NULL, or the pushes' bound per tile ([gridDim.y][gridDim.x]): a tile's
+    // halo is the largest bound of its 3 x 3 tile neighbourhood; and the
+    // next pushes' per-tile slots, each zeroed by its tile
+    const int* halo_regions;
+    int* halo_regions_next;
+    // NULL, or the tile each block takes ([gridDim.y * gridDim.x], a
+    // permutation of the row-major tile indices)
+    const int* tile_order;
     int staged;                    // coarse rectangle of a tile fits LDS
     int group;                     // fine particles per group side (fineness)
     int grid32;                    // ro jx jy jz are float arrays (the

@@ -660,16 +660,16 @@ int launch_solve_bf16x3(int N, const DenseGeom& d, const SolveBatch& b,
     g1.tiles_n = g2.tiles_n = g3.tiles_n = g4.tiles_n = maxN / BN;
     const dim3 grid_mm((maxM / BM) * (maxN / BN) * b.nfields);
     {
-        ProfScope ps(prof, "dense_rhs", s);
-        bf_rhs_kernel<<<grid_rhs, kRhsThreads, 0, s>>>(ra);
+        ProfScope ps(prof, "dense_rhs", s, true);
+        launch_timed(ps, bf_rhs_kernel, grid_rhs, dim3(kRhsThreads), 0, s, ra);
     }
     LCODE_HIP(hipGetLastError());
     const BfBatch* gs[4] = {&g1, &g2, &g3, &g4};
     const char* names[4] = {"dense_gemm_1", "dense_gemm_2", "dense_gemm_3", "dense_gemm_4"};
     for (int k = 0; k < 4; ++k) {
         {
-            ProfScope ps(prof, names[k], s);
-            bf_gemm_kernel<TM, TN><<<grid_mm, 256, 0, s>>>(*gs[k]);
+            ProfScope ps(prof, names[k], s, true);
+            launch_timed(ps, bf_gemm_kernel<TM, TN>, grid_mm, dim3(256), 0, s, *gs[k]);
         }
         LCODE_HIP(hipGetLastError());
     }
@@ -687,8 +687,8 @@ int launch_gemm(const GemmBatch& b, hipStream_t s, Profiler* prof,
     }
     const dim3 grid((maxN + kBN - 1) / kBN, (maxM + kBM - 1) / kBM, b.n);
     {
-        ProfScope ps(prof, name, s);
-        gemm_kernel<<<grid, 256, 0, s>>>(b);
+        ProfScope ps(prof, name, s, true);
+        launch_timed(ps, gemm_kernel, grid, dim3(256), 0, s, b);
     }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
@@ -698,8 +698,8 @@ int launch_dense_rhs(const DenseRhsArgs& a, int nfields, int max_total,
                      hipStream_t s, Profiler* prof) {
     const dim3 grid((max_total + 255) / 256, nfields);
     {
-        ProfScope ps(prof, "dense_rhs", s);
-        dense_rhs_kernel<<<grid, 256, 0, s>>>(a);
+        ProfScope ps(prof, "dense_rhs", s, true);
+        launch_timed(ps, dense_rhs_kernel, grid, dim3(256), 0, s, a);
     }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;

@@ -969,18 +969,18 @@ static int launch_passes(const SolverGeom& g, const SolveBatch& b,
                          dim3 ga, dim3 gb, dim3 gc) {
     const dim3 block(g.P * g.Tp);
     {
-        ProfScope ps(prof, "solve_pass_a", s);
-        pass_a_kernel<L_, TW_LDS, S, G><<<ga, block, lds, s>>>(g, b);
+        ProfScope ps(prof, "solve_pass_a", s, true);
+        launch_timed(ps, pass_a_kernel<L_, TW_LDS, S, G>, ga, block, lds, s, g, b);
     }
     LCODE_HIP(hipGetLastError());
     {
-        ProfScope ps(prof, "solve_pass_b", s);
-        pass_b_kernel<L_, TW_LDS, S><<<gb, block, lds, s>>>(g, b);
+        ProfScope ps(prof, "solve_pass_b", s, true);
+        launch_timed(ps, pass_b_kernel<L_, TW_LDS, S>, gb, block, lds, s, g, b);
     }
     LCODE_HIP(hipGetLastError());
     {
-        ProfScope ps(prof, "solve_pass_c", s);
-        pass_c_kernel<L_, TW_LDS, S, G><<<gc, block, lds, s>>>(g, b);
+        ProfScope ps(prof, "solve_pass_c", s, true);
+        launch_timed(ps, pass_c_kernel<L_, TW_LDS, S, G>, gc, block, lds, s, g, b);
     }
     LCODE_HIP(hipGetLastError());
     if (b.resid) {

@@ -1,5 +1,8 @@
-// Particle kernels: field-free estimate, fused interpolation + pusher, and
-// coarse->fine interpolation + TSC deposition.  FP64 throughout.
+// Particle kernels: field-free estimate, fused interpolation + pusher,
+// coarse->fine interpolation + TSC deposition, and the device-side plasma
+// set-up.  Particle positions and momenta are FP64 in every form; the
+// deposition's arithmetic and LDS sums are FP64 too (its output grids are
+// FP32 in the mixed form).
 
 #include "common.hpp"
 
@@ -8,8 +11,11 @@ namespace lcode {
 namespace {
 
 constexpr int kBlock = 256;
-// Blocks of the grid-wide spill deposition (one per CU).
-constexpr int kSpillBlocks = 256;
+// Blocks of the grid-wide spill deposition (build option).
+#ifndef LCODE_SPILL_BLOCKS
+#define LCODE_SPILL_BLOCKS 256
+#endif
+constexpr int kSpillBlocks = LCODE_SPILL_BLOCKS;
 
 __device__ __forceinline__ int clampi(int v, int n) {
     return v < 0 ? 0 : (v >= n ? n - 1 : v);
@@ -54,6 +60,9 @@ estimate_kernel(EstimateArgs a) {
 #define LCODE_PUSH_BLOCK 64
 #endif
 constexpr int kPushBlock = LCODE_PUSH_BLOCK;
+// Deposition tiles per row that the push's per-tile bound table holds
+// (more: each particle raises its tile's slot with a device atomic).
+constexpr int kRegionRowMax = 128;
 // A launch has about one wave per SIMD, so occupancy buys nothing: with a
 // low waves-per-EU target the compiler issues the 54 field gathers of a
 // particle together instead of one round trip at a time.
@@ -145,6 +154,38 @@ push_kernel(PushArgs a) {
         // the deposition writes NaN instead of depositing
         const double c = ceil(d / a.h);
         int cells = c <= kFarCells ? (int)c : kNonFinite;
+        if (a.region_max) {
+            // the bound of the deposition tile that holds the particle's
+            // initial cell: the block's particles (consecutive, fewer than
+            // a coarse row) lie in at most two rows of tiles, max-reduced
+            // in an LDS table first (one LDS atomic per particle), then one
+            // device atomic per tile touched (a slot takes those of the ~10
+            // blocks over its particles); a shuffle loop over the wave's
+            // tiles instead cost the push ~3 us of dependent latency
+            __shared__ int rtab[2 * kRegionRowMax];
+            const int nr = a.region_n, rn = nr - 1;
+            const int k0 = blockIdx.x * blockDim.x;
+            const int ri0 = min(max(cell_of(a.x_init[k0], a.N, a.h) / a.region_t, 0), rn);
+            for (int e = threadIdx.x; e < 2 * kRegionRowMax; e += blockDim.x) rtab[e] = 0;
+            __syncthreads();
+            if (cells > 0) {
+                const int ri = min(max(cell_of(a.x_init[k], a.N, a.h) / a.region_t, 0), rn);
+                const int rj = min(max(cell_of(a.y_init[k], a.N, a.h) / a.region_t, 0), rn);
+                const int dr = ri - ri0;
+                if (dr >= 0 && dr < 2 && nr <= kRegionRowMax)
+                    atomicMax(&rtab[dr * kRegionRowMax + rj], cells);
+                else
+                    atomicMax(a.region_max + ri * nr + rj, cells);
+            }
+            __syncthreads();
+            if (nr <= kRegionRowMax) {
+                for (int e = threadIdx.x; e < 2 * nr; e += blockDim.x) {
+                    const int dr = e / nr, rj = e - dr * nr;
+                    const int v = rtab[dr * kRegionRowMax + rj];
+                    if (v > 0 && ri0 + dr <= rn) atomicMax(a.region_max + (ri0 + dr) * nr + rj, v);
+                }
+            }
+        }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) cells = max(cells, __shfl_xor(cells, off, 64));
         // one atomic per wave, and only when it raises the bound: after the
@@ -246,7 +287,7 @@ __device__ int first_fine_at_or_after(const double* fg, int nf, int c, int N,
 // (or, when the pushes reported one, every tile) is written as NaN.
 constexpr int kFineMax = kDepFineMax;       // fine indices per side
 constexpr int kCoarseMax = kDepCoarseMax;   // coarse particles per side staged in LDS
-constexpr int kExtMax = 48;      // initial cells per side: kTile + 2 halo + 2
+constexpr int kExtMax = kTile + 14;   // initial cells per side: kTile + 2 halo + 2, halo <= 6
 
 struct FineTab {                 // per fine index along one axis
     double ip[kFineMax], in[kFineMax], fg[kFineMax];
@@ -330,21 +371,49 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     __shared__ int tile_nan;
     const int N = a.N;
     const double inv_h = 1. / a.h;
-    const int ci0 = blockIdx.y * kTile, cj0 = blockIdx.x * kTile;
+    // tile of this block: row-major, or the order of a.tile_order (tiles
+    // near the grid centre, where the wake displaces and compresses the
+    // plasma and tiles take longest, first)
+    const int ntx = gridDim.x;
+    int tile = blockIdx.y * ntx + blockIdx.x;
+    if (a.tile_order) tile = a.tile_order[tile];
+    const int ty = tile / ntx, tx = tile - ty * ntx;
+    const int ci0 = ty * kTile, cj0 = tx * kTile;
     // halo: the displacement bound of the last pushes when one is given.
     // A fine particle's offset is a convex mix of coarse offsets, so its
     // cell moved at most `bound` cells: with bound <= a.halo no particle
     // can leave the halo, and every block (all read the same bound) skips
     // the spill list.  Otherwise particles beyond the halo take the exact
     // spill path.
-    const int bound = a.halo_cells ? max(*a.halo_cells, 0) : a.halo + 1;
-    const int halo = min(bound, a.halo);
-    const bool may_spill = bound > a.halo;
+    // With per-tile bounds the tile takes the largest of its 3 x 3 tile
+    // neighbourhood: the fine particles it visits (initial cells within
+    // halo + 1 of it) interpolate from coarse particles whose initial cells
+    // lie within coarseness + 1 more cells, inside that neighbourhood (the
+    // host enables the per-tile bounds only when that holds), and every tile
+    // a particle's stencil reaches has the particle's home tile in its
+    // neighbourhood, so all of them visit it, or (beyond a.halo) none does
+    // and its home tile lists it.
+    const int global = a.halo_cells ? max(*a.halo_cells, 0) : a.halo + 1;
     // the pushes met a non-finite (or absurdly far) particle: the whole
     // deposition is NaN, with no particle visited (already reported)
-    const bool nonfinite = bound >= kNonFinite;
-    if (a.halo_next && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
-        *a.halo_next = 0;   // for the next pushes; nothing reads it here
+    const bool nonfinite = global >= kNonFinite;
+    int bound = global;
+    if (a.halo_regions && !nonfinite) {
+        bound = 0;
+        for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj) {
+                const int r = ty + di, c = tx + dj;
+                if (r >= 0 && r < (int)gridDim.y && c >= 0 && c < ntx)
+                    bound = max(bound, a.halo_regions[r * ntx + c]);
+            }
+    }
+    const int halo = min(bound, a.halo);
+    const bool may_spill = bound > a.halo;
+    if (threadIdx.x == 0) {
+        // zero the slots the next pushes raise; nothing reads them here
+        if (a.halo_next && tile == 0) *a.halo_next = 0;
+        if (a.halo_regions_next) a.halo_regions_next[tile] = 0;
+    }
     if (threadIdx.x == 0) tile_nan = 0;
     const int ext = kTile + 2 * halo + 2;
     const int ci_lo = ci0 - 1 - halo, cj_lo = cj0 - 1 - halo;
@@ -421,39 +490,56 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         }
         __syncthreads();
         const A sm = (A)a.smallness;
-        // coarse->fine interpolation of tile-local fine particle (li, lj):
-        // position and deposition values (lcode.py:591-621, :651-655)
-        auto particle = [&](int li, int lj, double& x, double& y, A* v) {
+        // coarse->fine interpolation of tile-local fine particle (li, lj)
+        // (lcode.py:591-621): its position, and apart from it (only for the
+        // particles whose stencil reaches the tile) its deposition values
+        // (lcode.py:651-655)
+        auto position = [&](int li, int lj, double& x, double& y) {
             const double wA = ti.ip[li] * tj.ip[lj], wB = ti.ip[li] * tj.in[lj];
             const double wC = ti.in[li] * tj.ip[lj], wD = ti.in[li] * tj.in[lj];
-            const A Ag = (A)wA, Bg = (A)wB, Cg = (A)wC, Dg = (A)wD;
             if constexpr (STAGED) {
                 // row offset + column offset (both clamped into the staged
                 // rectangle above)
                 const int pp = ti.P[li] + tj.P[lj], pn = ti.P[li] + tj.Nn[lj];
                 const int np = ti.Nn[li] + tj.P[lj], nn = ti.Nn[li] + tj.Nn[lj];
 #define LCODE_MIXP(c) (wA * cpos[c][pp] + wB * cpos[c][pn] + wC * cpos[c][np] + wD * cpos[c][nn])
-#define LCODE_MIX(c) (Ag * cval[c][pp] + Bg * cval[c][pn] + Cg * cval[c][np] + Dg * cval[c][nn])
                 x = ti.fg[li] + LCODE_MIXP(0);
                 y = tj.fg[lj] + LCODE_MIXP(1);
-                deposit_values<A>(sm * LCODE_MIX(0), sm * LCODE_MIX(1), sm * LCODE_MIX(2),
-                                  sm * LCODE_MIX(3), sm * LCODE_MIX(4), v);
-#undef LCODE_MIX
 #undef LCODE_MIXP
             } else {
                 const int nc = a.nc;
                 const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
                 const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
 #define LCODE_MIXP(arr) (wA * arr[pp] + wB * arr[pn] + wC * arr[np] + wD * arr[nn])
-#define LCODE_MIX(arr) (Ag * (A)arr[pp] + Bg * (A)arr[pn] + Cg * (A)arr[np] + Dg * (A)arr[nn])
                 x = ti.fg[li] + LCODE_MIXP(a.x_offt);
                 y = tj.fg[lj] + LCODE_MIXP(a.y_offt);
+#undef LCODE_MIXP
+            }
+        };
+        auto values = [&](int li, int lj, A* v) {
+            const A Ag = (A)(ti.ip[li] * tj.ip[lj]), Bg = (A)(ti.ip[li] * tj.in[lj]);
+            const A Cg = (A)(ti.in[li] * tj.ip[lj]), Dg = (A)(ti.in[li] * tj.in[lj]);
+            if constexpr (STAGED) {
+                const int pp = ti.P[li] + tj.P[lj], pn = ti.P[li] + tj.Nn[lj];
+                const int np = ti.Nn[li] + tj.P[lj], nn = ti.Nn[li] + tj.Nn[lj];
+#define LCODE_MIX(c) (Ag * cval[c][pp] + Bg * cval[c][pn] + Cg * cval[c][np] + Dg * cval[c][nn])
+                deposit_values<A>(sm * LCODE_MIX(0), sm * LCODE_MIX(1), sm * LCODE_MIX(2),
+                                  sm * LCODE_MIX(3), sm * LCODE_MIX(4), v);
+#undef LCODE_MIX
+            } else {
+                const int nc = a.nc;
+                const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
+                const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
+#define LCODE_MIX(arr) (Ag * (A)arr[pp] + Bg * (A)arr[pn] + Cg * (A)arr[np] + Dg * (A)arr[nn])
                 deposit_values<A>(sm * LCODE_MIX(a.m), sm * LCODE_MIX(a.q),
                                   sm * LCODE_MIX(a.px), sm * LCODE_MIX(a.py),
                                   sm * LCODE_MIX(a.pz), v);
 #undef LCODE_MIX
-#undef LCODE_MIXP
             }
+        };
+        // does the 3 x 3 stencil around centre (i, j) reach this tile?
+        auto reaches = [&](int i, int j) {
+            return i + 1 >= ci0 && i - 1 < ci0 + kTile && j + 1 >= cj0 && j - 1 < cj0 + kTile;
         };
         // TSC centre and weights of the particle; false when it moved more
         // than the halo from its initial cell (its home tile lists it)
@@ -517,8 +603,9 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
                 t.i = t.j = -1;
                 if (q < gs * gs && li < ni && lj < nj) {
                     double x, y;
-                    particle(li, lj, x, y, v);
-                    if (!locate(li, lj, x, y, t)) t.i = -1;
+                    position(li, lj, x, y);
+                    if (!locate(li, lj, x, y, t) || !reaches(t.i, t.j)) t.i = -1;
+                    else values(li, lj, v);
                 }
                 if (ai >= 0 && (t.i != ai || t.j != aj)) {
                     if (q == gs * gs || t.i >= 0) {
@@ -545,9 +632,10 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             const int li = p / nj, lj = p - li * nj;
             double x, y;
             A v[4];
-            particle(li, lj, x, y, v);
+            position(li, lj, x, y);
             TscT<A> t;
-            if (!locate(li, lj, x, y, t)) continue;
+            if (!locate(li, lj, x, y, t) || !reaches(t.i, t.j)) continue;
+            values(li, lj, v);
             lds_deposit<A>(acc, t.i, t.j, ci0, cj0, v, t.w, nullptr);
         }
 #endif
@@ -696,12 +784,12 @@ int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof) {
 
 int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof) {
     if (a.n == 0) return LCODE_OK;
-    ProfScope ps(prof, "push", s);
-    const int blocks = (a.n + kPushBlock - 1) / kPushBlock;
+    ProfScope ps(prof, "push", s, true);
+    const dim3 blocks((a.n + kPushBlock - 1) / kPushBlock);
     if (a.grid32)
-        push_kernel<float><<<blocks, kPushBlock, 0, s>>>(a);
+        launch_timed(ps, push_kernel<float>, blocks, dim3(kPushBlock), 0, s, a);
     else
-        push_kernel<double><<<blocks, kPushBlock, 0, s>>>(a);
+        launch_timed(ps, push_kernel<double>, blocks, dim3(kPushBlock), 0, s, a);
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;
 }
@@ -728,21 +816,26 @@ int launch_zero_ints(int* p, int n, hipStream_t s) {
 int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof) {
     const dim3 grid((a.N + kTile - 1) / kTile, (a.N + kTile - 1) / kTile);
     {
-        ProfScope ps(prof, "deposit_tile", s);
-#define LCODE_DEP_LAUNCH(AT, GT)                                                   \
-    (a.staged ? (deposit_tile_kernel<true, AT, GT><<<grid, kDepBlock, 0, s>>>(a)) \
-              : (deposit_tile_kernel<false, AT, GT><<<grid, kDepBlock, 0, s>>>(a)))
-        if (a.grid32) LCODE_DEP_LAUNCH(double, float);
-        else LCODE_DEP_LAUNCH(double, double);
-#undef LCODE_DEP_LAUNCH
+        ProfScope ps(prof, "deposit_tile", s, true);
+        const dim3 block(kDepBlock);
+        if (a.grid32 && a.staged)
+            launch_timed(ps, deposit_tile_kernel<true, double, float>, grid, block, 0, s, a);
+        else if (a.grid32)
+            launch_timed(ps, deposit_tile_kernel<false, double, float>, grid, block, 0, s, a);
+        else if (a.staged)
+            launch_timed(ps, deposit_tile_kernel<true, double, double>, grid, block, 0, s, a);
+        else
+            launch_timed(ps, deposit_tile_kernel<false, double, double>, grid, block, 0, s, a);
     }
     LCODE_HIP(hipGetLastError());
     {
-        ProfScope ps(prof, "deposit_spill", s);
+        ProfScope ps(prof, "deposit_spill", s, true);
         if (a.grid32)
-            deposit_spill_kernel<double, float><<<kSpillBlocks, kBlock, 0, s>>>(a);
+            launch_timed(ps, deposit_spill_kernel<double, float>, dim3(kSpillBlocks),
+                         dim3(kBlock), 0, s, a);
         else
-            deposit_spill_kernel<double, double><<<kSpillBlocks, kBlock, 0, s>>>(a);
+            launch_timed(ps, deposit_spill_kernel<double, double>, dim3(kSpillBlocks),
+                         dim3(kBlock), 0, s, a);
     }
     LCODE_HIP(hipGetLastError());
     return LCODE_OK;

@@ -210,15 +210,29 @@ def fault_count(config=None, nc=None):
     return total
 
 
-def profile_begin(max_launches=8192):
+def profile_begin(max_launches=8192, only=None):
     """Start bracketing every kernel launch of this device's workspaces
-    with HIP events (instrumentation, not in the reference)."""
+    with HIP events (instrumentation, not in the reference); with `only`
+    (a kernel name such as "deposit_tile") just that kernel's launches."""
     lib = _lib.load()
     dev = torch.cuda.current_device()
     for key, ws in _workspaces.items():
         if key[0] == dev:
             _lib.check(lib.lcode_profile_begin(ws, int(max_launches)),
                        "lcode_profile_begin")
+            _lib.check(lib.lcode_profile_only(ws, (only or "").encode()),
+                       "lcode_profile_only")
+
+
+def profile_pause(paused=True):
+    """Stop (or resume) the timing begun by profile_begin; launches in
+    between run untimed."""
+    lib = _lib.load()
+    dev = torch.cuda.current_device()
+    for key, ws in _workspaces.items():
+        if key[0] == dev:
+            _lib.check(lib.lcode_profile_pause(ws, int(bool(paused))),
+                       "lcode_profile_pause")
 
 
 def profile_end():

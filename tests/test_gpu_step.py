@@ -207,6 +207,50 @@ def test_step_641_particles_ten_cells_out(lc, oracle):
     assert lc.fault_count(cfg_g, const.m.shape[0]) == 0
 
 
+@pytest.mark.parametrize("cells", [4.5, 8.0])
+def test_step_641_local_displacement(lc, oracle, cells):
+    """641^2: offsets that are zero except in two bumps (one pushing
+    particles outward, one inward) of up to `cells` cells, so the
+    deposition tiles' per-tile halos differ from tile to tile: particles
+    leave tiles with large bounds for tiles whose own bound is zero (4.5
+    cells), or go beyond the largest halo and take the spill kernel (8
+    cells); one step against the oracle, relative L2 <= 1e-11."""
+    say(f"641^2 one step, two local displacement bumps of up to {cells} cells")
+    kw = dict(grid_steps=641, grid_step_size=.025, xi_step_size=.005,
+              plasma_coarseness=3, plasma_fineness=2)
+    cfg_o, cfg_g = _cfg(**kw), _cfg(**kw)
+    xs, ys, const, vp, st = perturbed_state(oracle, cfg_o, seed=23)
+    h, sig = cfg_o.grid_step_size, .3
+    x0, y0 = const.x_init, const.y_init
+    st.x_offt = np.zeros_like(x0)
+    st.y_offt = np.zeros_like(y0)
+    # peak |offset| of A r / sig exp(-r^2 / 2 sig^2) is A exp(-1/2)
+    amp = cells * h / np.exp(-.5)
+    for (cx, cy), sign in (((2.0, -1.3), 1), ((-1.1, 0.7), -1)):
+        dx, dy = x0 - cx, y0 - cy
+        g = sign * amp / sig * np.exp(-(dx * dx + dy * dy) / (2 * sig * sig))
+        st.x_offt += g * dx
+        st.y_offt += g * dy
+    lc.init(cfg_g)
+    gconst = lc.GPUArrays(**{k: dev(getattr(const, k)) for k in (
+        "m", "q", "x_init", "y_init", "ro_initial")})
+    gvp = lc.GPUArrays(**{k: getattr(vp, k) for k in (
+        "influence_prev", "influence_next", "indices_prev", "indices_next",
+        "fine_grid")})
+    beam = .05 * np.exp(-.5 * (xs**2 + ys**2))
+    ref = oracle.step(cfg_o, const, vp, st, beam)
+    got = lc.step(cfg_g, gconst, gvp, to_device_state(lc, st), dev(beam))
+    torch.cuda.synchronize()
+    errs = {name: rel_l2(host(getattr(got, name)), getattr(ref, name),
+                         _floor(name, ref)) for name in STATE}
+    say("  rel L2:", {k: f"{v:.1e}" for k, v in errs.items()})
+    for name, err in errs.items():
+        assert err < 1e-11, (name, err)
+    disp = max(np.abs(st.x_offt).max(), np.abs(st.y_offt).max()) / h
+    assert cells - .1 < disp < cells + .1
+    assert lc.fault_count(cfg_g, const.m.shape[0]) == 0
+
+
 def test_step_nonfinite_particle_is_reported(lc, oracle):
     """A NaN momentum makes the step's depositions write NaN grids (no
     particle is deposited, so nothing grinds on a NaN cell index) and the
