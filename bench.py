@@ -268,19 +268,25 @@ def mode_legs(lc, torch, cosine_gaussian, window, plain_ref, modes):
 
 
 def train_learned(lc, cosine_gaussian, torch, runs=4, slices=RUN_SLICES, every=2,
-                  epochs=4):
+                  epochs=4, cfg=None, beam_kw=None):
     """The U-Net predictor trained on the GPU on ~3000 slices of `runs` plain
-    FP64 641^2 runs (amplitude ~ U[0.01, 0.1], sigma ~ U[0.5, 2] from
-    default_rng(0)), every `every`-th slice; target F1 - Q (the predictor
-    pass's fields minus the quadratic extrapolation)."""
+    FP64 runs of `cfg` (default the 641^2 workload; amplitude ~ U[0.01,
+    0.1], sigma ~ U[0.5, 2] from default_rng(0), other beam arguments from
+    `beam_kw`), every `every`-th slice; target F1 - Q (the predictor pass's
+    fields minus the quadratic extrapolation)."""
     import numpy as np
     from lcode3d_amd import unet
-    cfg = make_config()
+    cfg = cfg or make_config()
     rng = np.random.default_rng(0)
     params = [(float(rng.uniform(.01, .1)), float(rng.uniform(.5, 2))) for _ in range(runs)]
     net = unet.UNetPredictor(cfg.grid_steps, seed=0, lr=1e-3)
+    beam_fn = None
+    if beam_kw:
+        def beam_fn(c, i, amplitude, sigma):
+            return cosine_gaussian(c, i, amplitude=amplitude, sigma=sigma, **beam_kw)
     t0 = time.perf_counter()
-    samples, sx, st = unet.collect_training_data(lc, net, cfg, params, slices, every=every)
+    samples, sx, st = unet.collect_training_data(lc, net, cfg, params, slices, every=every,
+                                                 beam_fn=beam_fn)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     hist = unet.train(net, samples, (sx, st), epochs)
@@ -288,8 +294,9 @@ def train_learned(lc, cosine_gaussian, torch, runs=4, slices=RUN_SLICES, every=2
     t2 = time.perf_counter()
     n = len(samples)
     del samples
-    log(f"learned predictor: {n} samples from {runs} runs ({t1 - t0:.1f} s), "
-        f"{epochs} epochs ({t2 - t1:.1f} s), loss {hist[0]:.3g} -> {hist[-1]:.3g}")
+    log(f"learned predictor at {cfg.grid_steps}^2: {n} samples from {runs} runs "
+        f"({t1 - t0:.1f} s), {epochs} epochs ({t2 - t1:.1f} s), "
+        f"loss {hist[0]:.3g} -> {hist[-1]:.3g}")
     info = {"predictor": "U-Net, 4 levels x 32 channels, BF16 MFMA, target F1 - Q",
             "train_runs": params, "train_samples": n, "epochs": epochs,
             "collect_seconds": t1 - t0, "train_seconds": t2 - t1, "train_loss": hist}
@@ -346,13 +353,15 @@ LARGE_WORKLOAD = dict(grid_steps=1281, grid_step_size=.0125, xi_step_size=.0025,
 LARGE_BEAM = dict(amplitude=.1, sigma=.7, shift_x=.5)
 
 
-def large_workload_leg(lc, torch, cosine_gaussian, slices):
+def large_workload_leg(lc, torch, cosine_gaussian, slices, learned=True):
     """The large non-axisymmetric case: 1281^2 (h 0.0125, xi step 0.0025,
     coarseness 3, fineness 3), beam amplitude 0.1, sigma 0.7, offset +0.5
-    in x.  The plain FP64 run and the mixed-precision run with the guided
-    step (quadratic extrapolation) over the same `slices` slices, each a
-    continuous run timed whole and over slices [slices/3, slices/3 + 300);
-    Ez(xi, 0, 0) and the fields every 100 slices compared between them."""
+    in x.  The plain FP64 run, the mixed-precision run, the mixed run with
+    the guided step (quadratic extrapolation) and, with `learned`, with the
+    U-Net guess (trained at 1281^2 on two plain runs of other beams with
+    the same offset) over the same `slices` slices, each a continuous run
+    timed whole and over slices [slices/3, slices/3 + 300); Ez(xi, 0, 0)
+    and the fields every 100 slices compared with the plain run."""
     def cfg(precision):
         c = make_config()
         for k, v in LARGE_WORKLOAD.items():
@@ -367,8 +376,14 @@ def large_workload_leg(lc, torch, cosine_gaussian, slices):
     beams = run_beams(c, cosine_gaussian, slices, **LARGE_BEAM)
     plain, ez_p, ch_p = continuous_run(lc, torch, c, beams, window, record=True)
     out["plain"] = plain
-    for name, prec, stepper in (("mixed", "mixed", None),
-                                ("mixed_guided", "mixed", stepper_factory(lc))):
+    modes = [("mixed", "mixed", None), ("mixed_guided", "mixed", stepper_factory(lc))]
+    if learned:
+        shift = {"shift_x": LARGE_BEAM["shift_x"]}
+        net, info = train_learned(lc, cosine_gaussian, torch, runs=2, slices=min(slices, 1500),
+                                  every=1, cfg=c, beam_kw=shift)
+        out["learned_training"] = info
+        modes.append(("mixed_learned", "mixed", stepper_factory(lc, net.guess)))
+    for name, prec, stepper in modes:
         r, ez, ch = continuous_run(lc, torch, cfg(prec), beams, window, stepper,
                                    record=True)
         r.update(compare_runs(torch, ez_p, ch_p, ez, ch))
@@ -472,9 +487,9 @@ def main():
                     help="skip the 129^2 leg (GPU and oracle)")
     ap.add_argument("--guided-backoff", type=int, default=GUIDED_BACKOFF,
                     help="plain slices after a rejected guess (0: guess every slice)")
-    ap.add_argument("--large", type=int, default=0, metavar="SLICES",
-                    help="also run the 1281^2 fineness-3 off-axis case, plain FP64 "
-                         "against mixed and mixed + guided, for SLICES slices (0: skip)")
+    ap.add_argument("--large", type=int, default=3000, metavar="SLICES",
+                    help="the 1281^2 fineness-3 off-axis case, plain FP64 against mixed, "
+                         "mixed + guided and mixed + learned, for SLICES slices (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run a device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -635,8 +650,8 @@ def main():
 
     large = None
     if args.large:
-        large = large_workload_leg(lc, torch, cosine_gaussian, args.large)
-        for k in ("mixed", "mixed_guided"):
+        large = large_workload_leg(lc, torch, cosine_gaussian, args.large, args.learned)
+        for k in [m for m in ("mixed", "mixed_guided", "mixed_learned") if m in large]:
             r = large[k]
             log(f"1281^2 x {args.large} {k}: {r['run_xi_steps_per_s']:.1f} xi-steps/s "
                 f"({r['run_vs_plain']:.2f}x plain {large['plain']['run_xi_steps_per_s']:.1f}); "

@@ -58,15 +58,20 @@ struct lcode_workspace {
     int* spill = nullptr;          // nf * nf spilled fine particle indices
     int* spill_ctr = nullptr;      // [0], [1] spill counts, [2], [3] the
                                    // pushes' displacement bound in cells,
-                                   // each pair of slots used in turn
+                                   // each pair of slots used in turn; [4]
+                                   // tile blocks done (the in-kernel spill)
     int* regions = nullptr;        // the same bound per deposition tile:
                                    // two slots of region_n^2 ints
     int region_n = 0;              // tiles per side (0: no per-tile bounds)
     int* tile_order = nullptr;     // deposition tiles, centre first
     int bound_slot = 0;            // slot the next push raises
     int spill_slot = 0;            // count slot of the next deposition
-    int* host_flag = nullptr;      // pinned host int: a non-finite particle
-                                   // was met (written by the kernels)
+    int grid_spill_left = 0;       // depositions still followed by the spill
+                                   // kernel after spills were last seen
+    int* host_flag = nullptr;      // pinned host ints: [0] a non-finite particle
+                                   // was met, [1] a deposition listed a
+                                   // particle beyond its halo (written by
+                                   // the kernels)
     double* guess[6] = {};         // guided step: the field guess
     double* resid = nullptr;       // {sum (F - guess)^2, sum F^2}
     double* resid_part = nullptr;  // 2 per block of the last FFT pass
@@ -529,7 +534,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     if ((rc = dev_alloc(ws, (void**)&ws->fault, sizeof(int)))) return fail(rc);
     const size_t nfp = (size_t)cfg->nf * cfg->nf;
     if ((rc = dev_alloc(ws, (void**)&ws->spill, (nfp ? nfp : 1) * sizeof(int)))) return fail(rc);
-    if ((rc = dev_alloc(ws, (void**)&ws->spill_ctr, 4 * sizeof(int)))) return fail(rc);
+    if ((rc = dev_alloc(ws, (void**)&ws->spill_ctr, 5 * sizeof(int)))) return fail(rc);
     // per-tile displacement bounds: a deposition tile takes the largest of
     // its 3 x 3 neighbourhood, which holds every coarse particle a fine
     // particle within the largest halo + 1 cells of the tile interpolates
@@ -557,14 +562,14 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
             return fail(rc);
         }
     }
-    if (hipHostMalloc((void**)&ws->host_flag, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void**)&ws->host_flag, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         ws->host_flag = nullptr;
-        set_error("hipHostMalloc of the non-finite flag failed");
+        set_error("hipHostMalloc of the host flags failed");
         return fail(LCODE_E_HIP);
     }
-    *ws->host_flag = 0;
+    ws->host_flag[0] = ws->host_flag[1] = 0;
     if (hipMemset(ws->fault, 0, sizeof(int)) != hipSuccess ||
-        hipMemset(ws->spill_ctr, 0, 4 * sizeof(int)) != hipSuccess ||
+        hipMemset(ws->spill_ctr, 0, 5 * sizeof(int)) != hipSuccess ||
         (ws->regions &&
          hipMemset(ws->regions, 0, 2 * (size_t)nt * nt * sizeof(int)) != hipSuccess) ||
         hipDeviceSynchronize() != hipSuccess) {
@@ -810,6 +815,20 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.spill_count = ws->spill_ctr + ws->spill_slot;
     a.spill_next = ws->spill_ctr + (1 - ws->spill_slot);
     a.tile_order = ws->tile_order;
+    // the spill list is deposited by the tile kernel's last block, unless
+    // a deposition listed particles recently: then for the next 64 the
+    // grid-wide spill kernel follows (a long list is many blocks' work)
+    {
+        volatile int* seen = ws->host_flag + 1;
+        if (*seen) {
+            *seen = 0;
+            ws->grid_spill_left = 64;
+        }
+        a.spill_in_tiles = ws->grid_spill_left > 0 ? 0 : 1;
+        if (ws->grid_spill_left > 0) --ws->grid_spill_left;
+        a.done = ws->spill_ctr + 4;
+        a.host_spill = ws->host_flag + 1;
+    }
     // after a push of this workspace: a halo of the pushes' displacement
     // bound, read from the slot the pushes raised; the deposition zeroes
     // the other slot, which the next pushes raise

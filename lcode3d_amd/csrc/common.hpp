@@ -145,6 +145,11 @@ constexpr int kDepCoarseMax = LCODE_DEP_COARSE_MAX;
 #ifndef LCODE_DEP_GROUP
 #define LCODE_DEP_GROUP 1
 #endif
+// 1: the lanes of a deposition wave take groups a prime stride apart
+// instead of adjacent ones (build option)
+#ifndef LCODE_DEP_SPREAD
+#define LCODE_DEP_SPREAD 0
+#endif
 
 // XCD-aware block order.  Workgroups go to the 8 XCDs round-robin by linear
 // id and each XCD has its own L2: map linear id `lin` of `total` to a
@@ -315,6 +320,13 @@ struct DepositArgs {
     // NULL, or the tile each block takes ([gridDim.y * gridDim.x], a
     // permutation of the row-major tile indices)
     const int* tile_order;
+    // 1: no spill kernel follows; when a particle may spill (the pushes'
+    // bound exceeds a.halo) every tile block ends with a fence and a count
+    // in *done, and the last one deposits the spill list and resets *done
+    int spill_in_tiles;
+    int* done;
+    // host-visible flag set when a tile lists a particle (or NULL)
+    int* host_spill;
     int staged;                    // coarse rectangle of a tile fits LDS
     int group;                     // fine particles per group side (fineness)
     int grid32;                    // ro jx jy jz are float arrays (the

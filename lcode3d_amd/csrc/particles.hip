@@ -362,8 +362,13 @@ __device__ __forceinline__ void deposit_spill_one(const DepositArgs& a, int fk,
 // keeps FP64 arithmetic.
 template <bool STAGED, typename A, typename G>
 __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
+    // the count slot of the next call is zeroed here (also when gated off)
+    // unless a spill kernel follows, which does it
+    if (a.spill_in_tiles && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+        *a.spill_next = 0;
     if (!gate_open(a.gate)) return;   // block-uniform: before any barrier
     __shared__ double acc[4][kTile][kTile];
+    __shared__ int last_block;
     __shared__ FineTab ti, tj;
     __shared__ double cpos[2][kCoarseMax * kCoarseMax];   // x_offt, y_offt
     __shared__ A cval[5][kCoarseMax * kCoarseMax];        // m q px py pz
@@ -563,10 +568,14 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
                 // (beyond a bound the pushes reported: cannot happen; counted
                 // as a fault rather than listed for a pass that does not run)
                 if (home) {
-                    if (may_spill)
+                    if (may_spill) {
                         a.spill[atomicAdd(a.spill_count, 1)] = (fi_a + li) * a.nf + fj_a + lj;
-                    else
+                        if (a.host_spill)
+                            __hip_atomic_store(a.host_spill, 1, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_SYSTEM);
+                    } else {
                         atomicAdd(a.fault, 1);
+                    }
                 }
                 return false;
             }
@@ -583,7 +592,22 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         // instruction hit distinct LDS addresses.
         const int gs = a.group > 0 ? a.group : 1;
         const int ngi = (ni + gs - 1) / gs, ngj = (nj + gs - 1) / gs;
-        for (int g = threadIdx.x; g < ngi * ngj; g += blockDim.x) {
+        // With LCODE_DEP_SPREAD the lanes of a wave take groups kSpread
+        // apart (g = L * stride mod n, the stride a prime that does not
+        // divide n, so the map is a permutation): where the wake compresses
+        // the plasma, adjacent groups' particles share cells, and adjacent
+        // lanes would then add to the same LDS addresses in one instruction.
+        const int ng = ngi * ngj;
+        int stride = 1;
+        if (LCODE_DEP_SPREAD) {
+            constexpr int kPrimes[6] = {13, 17, 19, 23, 29, 31};
+            stride = kPrimes[5];
+#pragma unroll
+            for (int u = 5; u >= 0; --u)
+                if (ng % kPrimes[u] != 0) stride = kPrimes[u];
+        }
+        for (int L = threadIdx.x; L < ng; L += blockDim.x) {
+            const int g = LCODE_DEP_SPREAD ? (int)((long long)L * stride % ng) : L;
             const int gi = g / ngj, gj = g - gi * ngj;
             A pat[4][9];
 #pragma unroll
@@ -674,6 +698,24 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         jx[idx] = (G)acc[1][r][c];
         jy[idx] = (G)acc[2][r][c];
         jz[idx] = (G)acc[3][r][c];
+    }
+    // a particle may have spilled somewhere (the bound every block read
+    // exceeds the largest halo) and no spill kernel follows: the last block
+    // to finish, after every block's stores are visible, deposits the list
+    if (a.spill_in_tiles && global > a.halo && !nonfinite) {
+        __threadfence();   // every thread's stores visible at device scope
+        __syncthreads();
+        if (threadIdx.x == 0)
+            last_block = atomicAdd(a.done, 1) == (int)(gridDim.x * gridDim.y) - 1;
+        __syncthreads();
+        if (last_block) {
+            __threadfence();
+            const int count = __hip_atomic_load(a.spill_count, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+            for (int p = threadIdx.x; p < count; p += blockDim.x)
+                deposit_spill_one<A, G>(a, a.spill[p], N);
+            if (threadIdx.x == 0) *a.done = 0;
+        }
     }
 }
 
@@ -828,7 +870,7 @@ int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof) {
             launch_timed(ps, deposit_tile_kernel<false, double, double>, grid, block, 0, s, a);
     }
     LCODE_HIP(hipGetLastError());
-    {
+    if (!a.spill_in_tiles) {
         ProfScope ps(prof, "deposit_spill", s, true);
         if (a.grid32)
             launch_timed(ps, deposit_spill_kernel<double, float>, dim3(kSpillBlocks),

@@ -622,7 +622,7 @@ class GuidedStepper:
     """step() with a guess on the slices GuessBackoff selects: the guess is
     the quadratic extrapolation of the last three states' fields, or the
     fields a predictor(s1, s2, s3, beam) returns (states n-1, n-2, n-3;
-    e.g. predictor.ConvPredictor.guess).  The acceptance of a guess is
+    e.g. unet.UNetPredictor.guess).  The acceptance of a guess is
     decided on the device (the plain predictor runs there when the residual
     is above the threshold), so no slice waits for the host: each guided
     slice enqueues an asynchronous copy of its residual sums into a pinned
