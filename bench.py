@@ -165,6 +165,8 @@ def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps):
 
 GUIDED_THRESHOLD = 2e-3   # above the one-pass corrector's own residual
 GUIDED_BACKOFF = 8        # plain slices after a rejected guess (doubling to 64)
+LEARNED_REFRESH = 4       # network evaluations: every 4th guided slice
+LEARNED_THRESHOLD = 2e-3  # corrector residual threshold of the learned legs
                           # (about 0.1 xi_step_size = 5.5e-4, p90 1.1e-3)
 RUN_SLICES = 1500         # the 641^2 run of the headline workload
 
@@ -229,10 +231,16 @@ def compare_runs(torch, ez_ref, checks_ref, ez, checks):
     return {"ez_axis_rel_l2": axis, "checkpoint_rel_l2_max": worst}
 
 
-def stepper_factory(lc, predictor=None):
+def stepper_factory(lc, predictor=None, alternating=False):
     def make(cfg, const, vp):
-        return lc.GuidedStepper(cfg, const, vp, GUIDED_THRESHOLD,
-                                predictor=predictor, backoff=GUIDED_BACKOFF)
+        # the learned legs have a threshold of their own (measured at 641^2
+        # with the network evaluated every 4th guided slice: 5e-3 and 1e-2
+        # accept no larger share of guesses than 2e-3 but let the transverse
+        # magnetic field drift from the plain step by 10-40 %)
+        t = LEARNED_THRESHOLD if predictor is not None else GUIDED_THRESHOLD
+        return lc.GuidedStepper(cfg, const, vp, t,
+                                predictor=predictor, backoff=GUIDED_BACKOFF,
+                                alternating=alternating)
     return make
 
 
@@ -280,6 +288,7 @@ def train_learned(lc, cosine_gaussian, torch, runs=4, slices=RUN_SLICES, every=2
     rng = np.random.default_rng(0)
     params = [(float(rng.uniform(.01, .1)), float(rng.uniform(.5, 2))) for _ in range(runs)]
     net = unet.UNetPredictor(cfg.grid_steps, seed=0, lr=1e-3)
+    net.refresh_every = LEARNED_REFRESH
     beam_fn = None
     if beam_kw:
         def beam_fn(c, i, amplitude, sigma):
@@ -298,6 +307,7 @@ def train_learned(lc, cosine_gaussian, torch, runs=4, slices=RUN_SLICES, every=2
         f"({t1 - t0:.1f} s), {epochs} epochs ({t2 - t1:.1f} s), "
         f"loss {hist[0]:.3g} -> {hist[-1]:.3g}")
     info = {"predictor": "U-Net, 4 levels x 32 channels, BF16 MFMA, target F1 - Q",
+            "refresh_every": LEARNED_REFRESH,
             "train_runs": params, "train_samples": n, "epochs": epochs,
             "collect_seconds": t1 - t0, "train_seconds": t2 - t1, "train_loss": hist}
     return net, info
@@ -376,7 +386,8 @@ def large_workload_leg(lc, torch, cosine_gaussian, slices, learned=True):
     beams = run_beams(c, cosine_gaussian, slices, **LARGE_BEAM)
     plain, ez_p, ch_p = continuous_run(lc, torch, c, beams, window, record=True)
     out["plain"] = plain
-    modes = [("mixed", "mixed", None), ("mixed_guided", "mixed", stepper_factory(lc))]
+    modes = [("mixed", "mixed", None), ("mixed_guided", "mixed", stepper_factory(lc)),
+             ("mixed_guided_alt", "mixed", stepper_factory(lc, alternating=True))]
     if learned:
         shift = {"shift_x": LARGE_BEAM["shift_x"]}
         net, info = train_learned(lc, cosine_gaussian, torch, runs=2, slices=min(slices, 1500),
@@ -460,7 +471,7 @@ def cpu_baseline(steps):
 
 
 def main():
-    global GUIDED_THRESHOLD, GUIDED_BACKOFF
+    global GUIDED_THRESHOLD, GUIDED_BACKOFF, LEARNED_REFRESH, LEARNED_THRESHOLD
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
@@ -483,6 +494,11 @@ def main():
                     help="mixed-precision error table over SLICES slices (0: skip)")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
                     help="skip the learned-predictor leg")
+    ap.add_argument("--learned-threshold", type=float, default=LEARNED_THRESHOLD,
+                    help="corrector residual threshold of the learned legs")
+    ap.add_argument("--learned-refresh", type=int, default=LEARNED_REFRESH,
+                    help="evaluate the network on every k-th guided slice and reuse "
+                         "its last correction in between")
     ap.add_argument("--no-small-workload", dest="small_workload", action="store_false",
                     help="skip the 129^2 leg (GPU and oracle)")
     ap.add_argument("--guided-backoff", type=int, default=GUIDED_BACKOFF,
@@ -495,6 +511,8 @@ def main():
     args = ap.parse_args()
     GUIDED_THRESHOLD = args.guided_threshold
     GUIDED_BACKOFF = args.guided_backoff
+    LEARNED_REFRESH = args.learned_refresh
+    LEARNED_THRESHOLD = args.learned_threshold
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): "
               "the xi step is sequential in xi and this build has no "
@@ -635,6 +653,7 @@ def main():
         _, ez_p, ch_p = continuous_run(lc, torch, cfg, run_beams_plain, window, record=True)
         mode_list = [("mixed", "mixed", None),
                      ("mixed_guided", "mixed", stepper_factory(lc)),
+                     ("mixed_guided_alt", "mixed", stepper_factory(lc, alternating=True)),
                      ("fp64_guided", "fp64", stepper_factory(lc))]
         learned_info = None
         if args.learned:
@@ -651,7 +670,8 @@ def main():
     large = None
     if args.large:
         large = large_workload_leg(lc, torch, cosine_gaussian, args.large, args.learned)
-        for k in [m for m in ("mixed", "mixed_guided", "mixed_learned") if m in large]:
+        for k in [m for m in ("mixed", "mixed_guided", "mixed_guided_alt", "mixed_learned")
+                  if m in large]:
             r = large[k]
             log(f"1281^2 x {args.large} {k}: {r['run_xi_steps_per_s']:.1f} xi-steps/s "
                 f"({r['run_vs_plain']:.2f}x plain {large['plain']['run_xi_steps_per_s']:.1f}); "

@@ -571,19 +571,33 @@ def guided_residual(config, nc, nf):
 class FieldExtrapolator:
     """Predictor for the guided step: polynomial extrapolation in xi of the
     last slices' fields, quadratic 3 F(n-1) - 3 F(n-2) + F(n-3) once three
-    slices are known (linear, then none, before that)."""
+    slices are known (linear, then none, before that).
+
+    alternating=True (for Variant A, lcode.py:869-871): the transverse
+    fields leave each step as 2 F - F_prev, so an error in them returns
+    with the opposite sign on the next slice and never decays; the
+    quadratic extrapolation turns such an alternating part e (-1)^k into
+    -7 e (-1)^n where the next slice carries +e (-1)^n.  Once four slices are known
+    the guess is 2 F(n-1) - 2 F(n-3) + F(n-4): twice the quadratic
+    extrapolation of the averages (F(k) + F(k-1)) / 2, minus F(n-1).  It
+    is exact for fields quadratic in xi, and it carries an alternating
+    part e (-1)^k on as e (-1)^n, as the step does."""
 
     COEFS = {1: None, 2: ((2., 0), (-1., 1)), 3: ((3., 0), (-3., 1), (1., 2))}
+    ALTERNATING = ((2., 0), (-2., 2), (1., 3))
 
-    def __init__(self, order=2):
+    def __init__(self, order=2, alternating=False):
         self.order = int(order)
+        self.alternating = bool(alternating) and self.order == 2
         self.history = []            # newest first
 
     def push(self, state):
         self.history.insert(0, state)
-        del self.history[self.order + 1:]
+        del self.history[self.order + (2 if self.alternating else 1):]
 
     def guess(self):
+        if self.alternating and len(self.history) >= 4:
+            return [(c, self.history[k]) for c, k in self.ALTERNATING]
         n = min(len(self.history), self.order + 1)
         if n < 2:
             return None
@@ -631,15 +645,21 @@ class GuidedStepper:
     waits; sync=True: each guided slice waits for its own outcome, so the
     policy's decisions do not depend on timing).  Counts guided, accepted
     and plain slices; `residuals` lists the observed residuals in slice
-    order."""
+    order.  alternating=True extrapolates with the Variant-A-aware form
+    (FieldExtrapolator): at 641^2 it doubles the accepted slices, and with
+    them the drift of the transverse magnetic field from the plain step
+    (each accepted slice passes the corrector's one-pass change on)."""
 
     def __init__(self, config, const, virt_params, threshold, predictor=None,
-                 backoff=8, max_backoff=64, lag=8, sync=False):
+                 backoff=8, max_backoff=64, lag=8, sync=False, alternating=False):
         self.config, self.const, self.vp = config, const, virt_params
         self.threshold = float(threshold)
         self.predictor = predictor
         self.policy = GuessBackoff(backoff, max_backoff)
-        self.extrap = FieldExtrapolator(order=2)
+        # alternating: the Variant-A-aware extrapolation (FieldExtrapolator)
+        self.extrap = FieldExtrapolator(
+            order=2, alternating=bool(alternating) and bool(
+                getattr(config, "field_solver_variant_A", True)))
         self.guided = self.accepted = self.plain = 0
         self.residuals = []
         self.lag = max(int(lag), 1)

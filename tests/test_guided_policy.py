@@ -35,3 +35,39 @@ def test_acceptance_restores_every_slice_and_the_initial_wait():
 def test_zero_backoff_guesses_every_slice():
     p = GuessBackoff(backoff=0, max_backoff=64)
     assert _run(p, [False] * 10, 10) == "G" * 10
+
+
+def test_alternating_extrapolation_coefficients():
+    """2 F(n-1) - 2 F(n-3) + F(n-4) is exact for fields quadratic in xi and
+    carries an alternating part e (-1)^k on as e (-1)^n, where the
+    quadratic extrapolation 3 F(n-1) - 3 F(n-2) + F(n-3) gives -7 e (-1)^n."""
+    import types
+
+    import numpy as np
+
+    from lcode3d_amd.lcode import FieldExtrapolator
+
+    def state(v):
+        return types.SimpleNamespace(Ex=np.float64(v))
+
+    def apply(terms):
+        return sum(c * s.Ex for c, s in terms)
+
+    n = 10
+    for quad in (lambda k: 1.0, lambda k: 0.5 * k, lambda k: 0.25 * k * k - k + 3):
+        for alt in (False, True):
+            e = FieldExtrapolator(order=2, alternating=alt)
+            for k in range(n):
+                e.push(state(quad(k)))
+            assert abs(apply(e.guess()) - quad(n)) < 1e-9
+    e_alt, e_quad = (FieldExtrapolator(order=2, alternating=a) for a in (True, False))
+    for k in range(n):
+        for e in (e_alt, e_quad):
+            e.push(state((-1.0) ** k))
+    assert abs(apply(e_alt.guess()) - (-1.0) ** n) < 1e-12
+    assert abs(apply(e_quad.guess()) + 7 * (-1.0) ** n) < 1e-12
+    # before four slices are known the alternating form falls back
+    e = FieldExtrapolator(order=2, alternating=True)
+    for k in range(3):
+        e.push(state(k * k))
+    assert len(e.guess()) == 3 and abs(apply(e.guess()) - 9) < 1e-12
