@@ -165,6 +165,7 @@ def solver_legs(lc, torch, cfg, const, vp, state, beams, steps, prof_steps):
 
 GUIDED_THRESHOLD = 2e-3   # above the one-pass corrector's own residual
 GUIDED_BACKOFF = 8        # plain slices after a rejected guess (doubling to 64)
+GUIDED_ANCHOR = 8         # a plain slice after this many guided ones in a row
 LEARNED_REFRESH = 4       # network evaluations: every 4th guided slice
 LEARNED_THRESHOLD = 2e-3  # corrector residual threshold of the learned legs
                           # (about 0.1 xi_step_size = 5.5e-4, p90 1.1e-3)
@@ -231,7 +232,7 @@ def compare_runs(torch, ez_ref, checks_ref, ez, checks):
     return {"ez_axis_rel_l2": axis, "checkpoint_rel_l2_max": worst}
 
 
-def stepper_factory(lc, predictor=None, alternating=False):
+def stepper_factory(lc, predictor=None, extrapolation="gap"):
     def make(cfg, const, vp):
         # the learned legs have a threshold of their own (measured at 641^2
         # with the network evaluated every 4th guided slice: 5e-3 and 1e-2
@@ -240,7 +241,7 @@ def stepper_factory(lc, predictor=None, alternating=False):
         t = LEARNED_THRESHOLD if predictor is not None else GUIDED_THRESHOLD
         return lc.GuidedStepper(cfg, const, vp, t,
                                 predictor=predictor, backoff=GUIDED_BACKOFF,
-                                alternating=alternating)
+                                extrapolation=extrapolation, anchor=GUIDED_ANCHOR)
     return make
 
 
@@ -387,7 +388,7 @@ def large_workload_leg(lc, torch, cosine_gaussian, slices, learned=True):
     plain, ez_p, ch_p = continuous_run(lc, torch, c, beams, window, record=True)
     out["plain"] = plain
     modes = [("mixed", "mixed", None), ("mixed_guided", "mixed", stepper_factory(lc)),
-             ("mixed_guided_alt", "mixed", stepper_factory(lc, alternating=True))]
+             ("mixed_guided_quad", "mixed", stepper_factory(lc, extrapolation="quadratic"))]
     if learned:
         shift = {"shift_x": LARGE_BEAM["shift_x"]}
         net, info = train_learned(lc, cosine_gaussian, torch, runs=2, slices=min(slices, 1500),
@@ -471,7 +472,7 @@ def cpu_baseline(steps):
 
 
 def main():
-    global GUIDED_THRESHOLD, GUIDED_BACKOFF, LEARNED_REFRESH, LEARNED_THRESHOLD
+    global GUIDED_THRESHOLD, GUIDED_BACKOFF, LEARNED_REFRESH, LEARNED_THRESHOLD, GUIDED_ANCHOR
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
@@ -501,6 +502,8 @@ def main():
                          "its last correction in between")
     ap.add_argument("--no-small-workload", dest="small_workload", action="store_false",
                     help="skip the 129^2 leg (GPU and oracle)")
+    ap.add_argument("--guided-anchor", type=int, default=GUIDED_ANCHOR,
+                    help="a plain slice after this many guided slices in a row (0: never)")
     ap.add_argument("--guided-backoff", type=int, default=GUIDED_BACKOFF,
                     help="plain slices after a rejected guess (0: guess every slice)")
     ap.add_argument("--large", type=int, default=3000, metavar="SLICES",
@@ -512,6 +515,7 @@ def main():
     GUIDED_THRESHOLD = args.guided_threshold
     GUIDED_BACKOFF = args.guided_backoff
     LEARNED_REFRESH = args.learned_refresh
+    GUIDED_ANCHOR = args.guided_anchor
     LEARNED_THRESHOLD = args.learned_threshold
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): "
@@ -653,7 +657,8 @@ def main():
         _, ez_p, ch_p = continuous_run(lc, torch, cfg, run_beams_plain, window, record=True)
         mode_list = [("mixed", "mixed", None),
                      ("mixed_guided", "mixed", stepper_factory(lc)),
-                     ("mixed_guided_alt", "mixed", stepper_factory(lc, alternating=True)),
+                     ("mixed_guided_quad", "mixed", stepper_factory(lc, extrapolation="quadratic")),
+                     ("mixed_guided_alt", "mixed", stepper_factory(lc, extrapolation="alternating")),
                      ("fp64_guided", "fp64", stepper_factory(lc))]
         learned_info = None
         if args.learned:
@@ -670,7 +675,7 @@ def main():
     large = None
     if args.large:
         large = large_workload_leg(lc, torch, cosine_gaussian, args.large, args.learned)
-        for k in [m for m in ("mixed", "mixed_guided", "mixed_guided_alt", "mixed_learned")
+        for k in [m for m in ("mixed", "mixed_guided", "mixed_guided_quad", "mixed_learned")
                   if m in large]:
             r = large[k]
             log(f"1281^2 x {args.large} {k}: {r['run_xi_steps_per_s']:.1f} xi-steps/s "

@@ -288,12 +288,13 @@ int lcode_step(lcode_workspace *ws, const lcode_config *cfg,
                hipStream_t stream);
 
 /* A field guess for the guided step: guess = sum_t coef[t] f[t], the six
- * fields Ex Ey Ez Bx By Bz of up to three N*N field sets (e.g. a predictor
- * network's output with coef {1}, or the quadratic extrapolation
- * 3 F(n-1) - 3 F(n-2) + F(n-3) with coef {3, -3, 1}). */
+ * fields Ex Ey Ez Bx By Bz of up to four N*N field sets (e.g. a predictor
+ * network's output with coef {1}, the quadratic extrapolation
+ * 3 F(n-1) - 3 F(n-2) + F(n-3) with coef {3, -3, 1}, or a term taken from
+ * lcode_workspace_predictor_fields). */
 typedef struct {
-    const double *f[3][6];
-    double coef[3];
+    const double *f[4][6];
+    double coef[4];
     int32_t nterms;
 } lcode_guess;
 
@@ -327,8 +328,8 @@ int lcode_step_mixed(lcode_workspace *ws, const lcode_config *cfg,
 
 /* A field guess over FP32 field sets (the mixed step's fields). */
 typedef struct {
-    const float *f[3][6];
-    double coef[3];
+    const float *f[4][6];
+    double coef[4];
     int32_t nterms;
 } lcode_guess_mixed;
 
@@ -351,6 +352,22 @@ int lcode_step_guided_residual(lcode_workspace *ws, hipStream_t stream,
  * sqrt(dst[0] / dst[1]) once the stream has reached the copy. */
 int lcode_step_guided_residual_async(lcode_workspace *ws, double *dst,
                                      hipStream_t stream);
+
+/* The predictor pass's fields of the last step on this workspace, kept for
+ * a guess of the next slice (no counterpart in the reference; the
+ * reference's predictor pass is lcode.py:841-874).  With keep != 0 every
+ * plain step (lcode_step, lcode_step_mixed) writes its predictor-pass
+ * fields F1 (after Variant A) into six workspace grids; a guided step
+ * leaves its guess there, or F1 when its guess was rejected, in either
+ * case whether or not keep is set.  lcode_workspace_predictor_fields
+ * returns the six device pointers (FP64 grids, or FP32 ones after a mixed
+ * step), valid until the workspace is destroyed; they may be passed as a
+ * term of the next guess, an lcode_guess: the guess kernel reads each
+ * element before it overwrites it.  A guess of the next slice's F1 from them: the
+ * extrapolation of the corrector's fields minus the last slice's one-pass
+ * change F - F1. */
+int lcode_workspace_keep_predictor_fields(lcode_workspace *ws, int32_t keep);
+int lcode_workspace_predictor_fields(lcode_workspace *ws, void *fields[6]);
 
 /* Plasma-noise diagnostic of one density grid (lcode.py:980-989,
  * diags_ro_zn before its running maximum): zn = mean |ro - G ro| /

@@ -67,8 +67,8 @@ class LcodeState(ctypes.Structure):
 
 class LcodeGuess(ctypes.Structure):
     """lcode_guess, and lcode_guess_mixed (the same layout)."""
-    _fields_ = [("f", (ctypes.c_void_p * 6) * 3),
-                ("coef", ctypes.c_double * 3),
+    _fields_ = [("f", (ctypes.c_void_p * 6) * 4),
+                ("coef", ctypes.c_double * 4),
                 ("nterms", ctypes.c_int32)]
 
 
@@ -105,6 +105,8 @@ _SIGNATURES = {
     "lcode_workspace_destroy": (ctypes.c_int, [_vp]),
     "lcode_profile_begin": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "lcode_profile_only": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "lcode_workspace_keep_predictor_fields": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "lcode_workspace_predictor_fields": (ctypes.c_int, [_vp, ctypes.c_void_p * 6]),
     "lcode_profile_pause": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "lcode_profile_end": (ctypes.c_int,
                           [_vp, _vp, ctypes.POINTER(LcodeKernelTime),

@@ -72,7 +72,10 @@ struct lcode_workspace {
                                    // was met, [1] a deposition listed a
                                    // particle beyond its halo (written by
                                    // the kernels)
-    double* guess[6] = {};         // guided step: the field guess
+    double* guess[6] = {};         // guided step: the field guess (after a
+                                   // step: its guess, or F1 when rejected,
+                                   // or a plain step's F1 with keep_f1)
+    int keep_f1 = 0;               // plain steps keep F1 in guess[]
     double* resid = nullptr;       // {sum (F - guess)^2, sum F^2}
     double* resid_part = nullptr;  // 2 per block of the last FFT pass
     double* diag_w = nullptr;      // zn diagnostic: Gaussian taps [2 r + 1]
@@ -1112,9 +1115,13 @@ static int step_impl(lcode_workspace* ws, const lcode_config* cfg,
         return rc;
     // 4: fields with prev fields in the trick term; only the averages are
     // used afterwards (lcode.py:862-881)
+    // (kept in the workspace's guess grids when asked: the next slice's
+    // guess can start from them)
     const double* trick_prev[4] = {prev->Ex, prev->Ey, prev->Bx, prev->By};
+    double* const keepF[6] = {ws->guess[0], ws->guess[1], ws->guess[2],
+                              ws->guess[3], ws->guess[4], ws->guess[5]};
     if ((rc = field_pass(ws, cfg, prev, trick_prev, beam_ro, beam_scalar,
-                         ws->dep, nullptr, ws->favg, stream)))
+                         ws->dep, ws->keep_f1 ? keepF : nullptr, ws->favg, stream)))
         return rc;
     // 5: corrector push with the averaged fields from the pass-1 positions
     // (lcode.py:884-889)
@@ -1166,7 +1173,7 @@ static int guided_impl(lcode_workspace* ws, const lcode_config* cfg,
     LCODE_REQUIRE(cfg->nc == ws->nc, "workspace was built for another nc");
     LCODE_REQUIRE(prev->Ex != out->Ex && prev->x_offt != out->x_offt,
                   "prev and out must not alias");
-    LCODE_REQUIRE(guess->nterms >= 1 && guess->nterms <= 3, "guess needs 1-3 terms");
+    LCODE_REQUIRE(guess->nterms >= 1 && guess->nterms <= 4, "guess needs 1-4 terms");
     LCODE_REQUIRE(threshold >= 0, "negative threshold");
     for (int t = 0; t < guess->nterms; ++t)
         for (int f = 0; f < 6; ++f) LCODE_REQUIRE(guess->f[t][f], "NULL guess field");
@@ -1224,8 +1231,11 @@ static int guided_impl(lcode_workspace* ws, const lcode_config* cfg,
                          c->q, ws->p1[2], ws->p1[3], ws->p1[4], vp, ws->dep,
                          stream, fallback)))
         return rc;
+    // (the predictor's fields replace the rejected guess in the guess grids)
+    double* const keepF[6] = {ws->guess[0], ws->guess[1], ws->guess[2],
+                              ws->guess[3], ws->guess[4], ws->guess[5]};
     if ((rc = field_pass(ws, cfg, prev, trick_prev, beam_ro, beam_scalar,
-                         ws->dep, nullptr, ws->favg, stream, fallback)))
+                         ws->dep, keepF, ws->favg, stream, fallback)))
         return rc;
     if ((rc = do_push(ws, cfg, c->m, c->q, c->x_init, c->y_init,
                       prev->x_offt, prev->y_offt, ws->p1[0], ws->p1[1],
@@ -1324,7 +1334,7 @@ int lcode_step_guided_mixed(lcode_workspace* ws, const lcode_config* cfg,
     const lcode_state p = as_state(prev), o = as_state(out);
     lcode_guess gd{};
     gd.nterms = guess->nterms;
-    for (int t = 0; t < 3; ++t) {
+    for (int t = 0; t < 4; ++t) {
         gd.coef[t] = guess->coef[t];
         for (int f = 0; f < 6; ++f)
             gd.f[t][f] = reinterpret_cast<const double*>(guess->f[t][f]);
@@ -1332,6 +1342,18 @@ int lcode_step_guided_mixed(lcode_workspace* ws, const lcode_config* cfg,
     Grid32Scope g(ws, 1);
     return guided_impl(ws, cfg, c, vp, &p, beam_ro, beam_scalar, &gd, threshold,
                        &o, stream);
+}
+
+int lcode_workspace_keep_predictor_fields(lcode_workspace* ws, int32_t keep) {
+    LCODE_REQUIRE(ws, "NULL workspace");
+    ws->keep_f1 = keep != 0;
+    return LCODE_OK;
+}
+
+int lcode_workspace_predictor_fields(lcode_workspace* ws, void* fields[6]) {
+    LCODE_REQUIRE(ws && fields, "NULL argument");
+    for (int f = 0; f < 6; ++f) fields[f] = ws->guess[f];
+    return LCODE_OK;
 }
 
 int lcode_step_guided_residual(lcode_workspace* ws, hipStream_t stream,

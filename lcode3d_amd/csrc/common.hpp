@@ -499,8 +499,8 @@ struct SolveBatch {
 // guess[f] = sum_t c[t] h[t][f]; avg[f] = (guess[f] + prev[f]) / 2 over n
 // values of the six fields; acc_zero[0..1] = 0.
 struct GuessArgs {
-    const double* h[3][6];
-    double c[3];
+    const double* h[4][6];
+    double c[4];
     int nt;
     const double* prev[6];
     double* guess[6];

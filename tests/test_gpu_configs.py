@@ -20,8 +20,9 @@
   momenta <= 1e-5; mixed: Ex Ey Bx By <= 2e-3, Bz <= 1e-4, momenta <= 2e-4;
   BF16x3: Ex Ey Bx By <= 2e-3, Bz <= 1e-4, momenta <= 1e-4.  The measured
   worst values are printed.
-* the mixed form with the guided predictor against the plain FP64 step
-  over 600 slices.
+* the mixed form with the guided predictor (the default guess of the
+  predictor pass's fields, and the quadratic extrapolation) against the
+  plain FP64 step over 600 slices.
 
 The beam is the oracle's expression evaluated the same way: the Gaussian
 profile amplitude * exp(-r^2 / (2 sigma^2)) computed on the host once, then
@@ -172,20 +173,32 @@ def test_reduced_precision_641_beam_grid(lc, transform, precision):
         assert worst[name] < bounds["momenta"], (name, worst[name])
 
 
-def test_mixed_guided_641_against_plain(lc):
-    """The mixed-precision step with the guided predictor (quadratic
-    extrapolation, corrector residual threshold 2e-3) against the plain
-    FP64 step over 600 slices of the 641^2 workload (amplitude 0.05,
-    sigma 1): Ez(xi, 0, 0) within relative L2 1e-4 and the full Ez within
-    1e-4 at every 100-slice checkpoint; the transverse fields pass a guess's
-    error through the corrector almost unattenuated, so Ex Ey within 2e-2,
-    Bx By within 2e-1 (relative to their own, small, norms); momenta within
-    1e-4; guesses are accepted on most guided slices."""
+GUIDED_BOUNDS = {
+    # the default guess of the predictor pass's own fields: an accepted
+    # slice differs from the plain step by the guess's error (~1e-5)
+    "gap": dict(fields=5e-3, accepted=0.9),
+    # a guess of the corrector's fields: accepted slices pass the one-pass
+    # change (~1e-3 of the fields) through to the transverse fields
+    "quadratic": dict(fields=2e-1, accepted=0.5),
+}
+
+
+@pytest.mark.parametrize("extrapolation", sorted(GUIDED_BOUNDS))
+def test_mixed_guided_641_against_plain(lc, extrapolation):
+    """The mixed-precision step with the guided predictor (corrector
+    residual threshold 2e-3) against the plain FP64 step over 600 slices of
+    the 641^2 workload (amplitude 0.05, sigma 1): Ez(xi, 0, 0) within
+    relative L2 1e-4 and the full Ez and Bz within 1e-4 at every 100-slice
+    checkpoint, momenta within 1e-4; Ex Ey Bx By (relative to their own
+    norms) and the share of guided slices accepted as GUIDED_BOUNDS gives
+    for each extrapolation (the corrector passes a guess's error in them
+    through almost unattenuated)."""
+    bounds = GUIDED_BOUNDS[extrapolation]
     cfg64 = _cfg641()
     cfgm = _cfg641(field_solver_precision="mixed")
     _, _, const, vp, sp = lc.init(cfg64)
     _, _, constm, vpm, sm = lc.init(cfgm)
-    gs = lc.GuidedStepper(cfgm, constm, vpm, 2e-3)
+    gs = lc.GuidedStepper(cfgm, constm, vpm, 2e-3, extrapolation=extrapolation)
     beam = beam_source(cfg64, .05, 1.0)
     c = cfg64.grid_steps // 2
     ezp, ezm, worst = [], [], {}
@@ -205,12 +218,14 @@ def test_mixed_guided_641_against_plain(lc):
     ezp = torch.stack(ezp)
     ezm = torch.stack(ezm).double()
     axis = (torch.linalg.norm(ezm - ezp) / torch.linalg.norm(ezp)).item()
-    say(f"mixed+guided 641^2 x 600: Ez axis {axis:.1e}; {gs.accepted} of {gs.guided} "
-        f"accepted, {gs.plain} plain; worst", {k: f"{v:.1e}" for k, v in worst.items()})
+    say(f"mixed+guided ({extrapolation}) 641^2 x 600: Ez axis {axis:.1e}; {gs.accepted} of "
+        f"{gs.guided} accepted, {gs.plain} plain; worst", {k: f"{v:.1e}" for k, v in worst.items()})
     assert axis < 1e-4
     assert worst["Ez"] < 1e-4 and worst["Bz"] < 1e-4
-    assert worst["Ex"] < 2e-2 and worst["Ey"] < 2e-2
-    assert worst["Bx"] < 2e-1 and worst["By"] < 2e-1
+    for name in ("Ex", "Ey", "Bx", "By"):
+        assert worst[name] < bounds["fields"], (name, worst[name])
     for name in ("px", "py", "pz"):
         assert worst[name] < 1e-4, (name, worst[name])
+    assert gs.accepted >= bounds["accepted"] * gs.guided
+    assert lc.fault_count(cfgm, constm.m.shape[0]) == 0
     assert gs.accepted > gs.guided // 2

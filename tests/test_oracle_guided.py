@@ -77,3 +77,36 @@ def test_accepted_imperfect_guess_is_used(oracle):
         ref = np.linalg.norm(getattr(plain, k))
         if ref > 0:
             assert np.linalg.norm(getattr(got, k) - getattr(plain, k)) < 1e-2 * ref, k
+
+
+def test_gap_corrected_extrapolation_predicts_the_predictor_fields(oracle):
+    """The guided stepper's default guess of slice n, E(n) - (F(n-1) -
+    F1(n-1)) with E = 2 F(n-1) - 2 F(n-3) + F(n-4) (the corrector's fields F
+    extrapolated, minus the last slice's one-pass change), predicts the
+    predictor pass's fields F1(n) (oracle.predictor_fields) far better than
+    the quadratic extrapolation of F: at 65^2, xi step 0.05, slices 20-69,
+    within 2e-3 everywhere (measured 3.5e-4 ... 1.3e-3) and at least 10x
+    closer on average (measured ~15x)."""
+    cfg = oracle.make_config(grid_steps=65, grid_step_size=.1, xi_step_size=.05,
+                             coarseness=2, fineness=2, plasma_padding_steps=6)
+    xs, ys, const, vp, st = oracle.init(cfg)
+    hist, f1_prev = [], None
+    eg, eq = [], []
+    for i in range(70):
+        b = oracle.beam(cfg, i, xs, ys, amplitude=.05, sigma=1.0)
+        f1 = oracle.predictor_fields(cfg, const, vp, st, b)
+        if i >= 20:
+            E = [2 * getattr(hist[0], k) - 2 * getattr(hist[2], k) + getattr(hist[3], k)
+                 for k in FIELDS]
+            Q = [3 * getattr(hist[0], k) - 3 * getattr(hist[1], k) + getattr(hist[2], k)
+                 for k in FIELDS]
+            G = [E[j] - getattr(hist[0], FIELDS[j]) + f1_prev[j] for j in range(6)]
+            nrm = np.sqrt(sum((x * x).sum() for x in f1))
+            eg.append(np.sqrt(sum(((G[j] - f1[j]) ** 2).sum() for j in range(6))) / nrm)
+            eq.append(np.sqrt(sum(((Q[j] - f1[j]) ** 2).sum() for j in range(6))) / nrm)
+        st = oracle.step(cfg, const, vp, st, b)
+        hist.insert(0, st)
+        del hist[4:]
+        f1_prev = f1
+    assert max(eg) < 2e-3, max(eg)
+    assert np.mean(eq) > 10 * np.mean(eg), (np.mean(eq), np.mean(eg))
