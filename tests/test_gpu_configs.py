@@ -175,8 +175,10 @@ def test_reduced_precision_641_beam_grid(lc, transform, precision):
 
 GUIDED_BOUNDS = {
     # the default guess of the predictor pass's own fields: an accepted
-    # slice differs from the plain step by the guess's error (~1e-5)
-    "gap": dict(fields=5e-3, accepted=0.9),
+    # slice differs from the plain step by the guess's error (~1e-5 of the
+    # fields); the transverse magnetic field, whose differences the step
+    # carries on without damping (Variant A), drifted 4e-3 … 1.9e-2
+    "gap": dict(fields=5e-2, accepted=0.9),
     # a guess of the corrector's fields: accepted slices pass the one-pass
     # change (~1e-3 of the fields) through to the transverse fields
     "quadratic": dict(fields=2e-1, accepted=0.5),
