@@ -290,6 +290,7 @@ def train_learned(lc, cosine_gaussian, torch, runs=4, slices=RUN_SLICES, every=2
     params = [(float(rng.uniform(.01, .1)), float(rng.uniform(.5, 2))) for _ in range(runs)]
     net = unet.UNetPredictor(cfg.grid_steps, seed=0, lr=1e-3)
     net.refresh_every = LEARNED_REFRESH
+    net.base = "alternating"     # correct the Variant-A-aware extrapolation
     beam_fn = None
     if beam_kw:
         def beam_fn(c, i, amplitude, sigma):
@@ -307,7 +308,8 @@ def train_learned(lc, cosine_gaussian, torch, runs=4, slices=RUN_SLICES, every=2
     log(f"learned predictor at {cfg.grid_steps}^2: {n} samples from {runs} runs "
         f"({t1 - t0:.1f} s), {epochs} epochs ({t2 - t1:.1f} s), "
         f"loss {hist[0]:.3g} -> {hist[-1]:.3g}")
-    info = {"predictor": "U-Net, 4 levels x 32 channels, BF16 MFMA, target F1 - Q",
+    info = {"predictor": "U-Net, 4 levels x 32 channels, BF16 MFMA, target F1 - E "
+                         "(E = 2 F(n-1) - 2 F(n-3) + F(n-4))",
             "refresh_every": LEARNED_REFRESH,
             "train_runs": params, "train_samples": n, "epochs": epochs,
             "collect_seconds": t1 - t0, "train_seconds": t2 - t1, "train_loss": hist}

@@ -442,10 +442,12 @@ int lcode_unet_adam(const lcode_unet *net, int32_t t, float lr,
  * f32 != 0).  which = 0: input planes[S][S][32] = 4 x 4 means of
  * F(n-1)/scale (6), (F(n-1) - F(n-2))/scale (6), beam(n)/scale,
  * ro(n-1)/scale, zeros; 1: targets[S][S][6] = 4 x 4 means of
- * (next - Q)/tscale with Q = 3 F(n-1) - 3 F(n-2) + F(n-3); 2: guess =
- * Q + tscale y(i / 4, j / 4) on the grid (after lcode_unet_forward). */
+ * (next - Q)/tscale; 2: guess = Q + tscale y(i / 4, j / 4) on the grid
+ * (after lcode_unet_forward).  Q = 3 F(n-1) - 3 F(n-2) + F(n-3), or with
+ * f4 set (slice n-4) the Variant-A-aware 2 F(n-1) - 2 F(n-3) + F(n-4). */
 typedef struct {
     const void *f1[6], *f2[6], *f3[6];   /* slices n-1, n-2, n-3 */
+    const void *f4[6];                   /* slice n-4, or NULLs */
     const void *next[6];                 /* targets: the fields to learn */
     const void *ro;                      /* plasma density of slice n-1 */
     const double *beam;                  /* beam of slice n, or NULL */

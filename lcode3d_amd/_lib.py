@@ -80,7 +80,8 @@ class LcodeUnet(ctypes.Structure):
 
 class LcodeUnetIo(ctypes.Structure):
     _fields_ = [("f1", ctypes.c_void_p * 6), ("f2", ctypes.c_void_p * 6),
-                ("f3", ctypes.c_void_p * 6), ("next", ctypes.c_void_p * 6),
+                ("f3", ctypes.c_void_p * 6), ("f4", ctypes.c_void_p * 6),
+                ("next", ctypes.c_void_p * 6),
                 ("ro", ctypes.c_void_p), ("beam", ctypes.c_void_p),
                 ("beam_scalar", ctypes.c_double), ("f32", ctypes.c_int32),
                 ("grid_steps", ctypes.c_int32),

@@ -629,6 +629,8 @@ struct UnetWgradArgs {
 // Planes of the network from the step's fields (FP64 or FP32 grids).
 struct UnetIoArgs {
     const void *f1[6], *f2[6], *f3[6];   // fields of slices n-1, n-2, n-3
+    const void* f4[6];                   // slice n-4 (the Variant-A-aware
+                                         // base), or NULLs
     const void* next[6];                 // targets: the fields to predict
     const void* ro;                      // plasma density of slice n-1
     const double* beam;                  // beam of slice n, or NULL

@@ -471,8 +471,21 @@ unet_inputs_kernel(UnetIoArgs a) {
     }
 }
 
-// Targets at level-0 size: the 4 x 4 mean of (F1(n) - Q(n)) / t with
-// Q = 3 F(n-1) - 3 F(n-2) + F(n-3); FP32 [S][S][6].
+// The base the network's output corrects: Q = 3 F(n-1) - 3 F(n-2) +
+// F(n-3), or with slice n-4 given 2 F(n-1) - 2 F(n-3) + F(n-4) (exact for
+// quadratic fields, and it carries the alternating part that Variant A
+// leaves in the transverse fields on as the step does, so the correction
+// the network learns is smooth from slice to slice).
+__device__ __forceinline__ double unet_base(const UnetIoArgs& a, int f, size_t idx) {
+    if (a.f4[f])
+        return 2 * fld(a.f1[f], a.f32, idx) - 2 * fld(a.f3[f], a.f32, idx) +
+               fld(a.f4[f], a.f32, idx);
+    return 3 * fld(a.f1[f], a.f32, idx) - 3 * fld(a.f2[f], a.f32, idx) +
+           fld(a.f3[f], a.f32, idx);
+}
+
+// Targets at level-0 size: the 4 x 4 mean of (F1(n) - Q(n)) / t; FP32
+// [S][S][6].
 __global__ void __launch_bounds__(256)
 unet_targets_kernel(UnetIoArgs a) {
     const int S = a.S, N = a.N;
@@ -485,9 +498,7 @@ unet_targets_kernel(UnetIoArgs a) {
             const int i = 4 * I + a4, j = 4 * J + b4;
             if (i >= N || j >= N) continue;
             const size_t idx = (size_t)i * N + j;
-            const double q = 3 * fld(a.f1[f], a.f32, idx) - 3 * fld(a.f2[f], a.f32, idx) +
-                             fld(a.f3[f], a.f32, idx);
-            s += fld(a.next[f], a.f32, idx) - q;
+            s += fld(a.next[f], a.f32, idx) - unet_base(a, f, idx);
         }
     a.targets[k] = (float)(s / 16 / a.tscale[f]);
 }
@@ -502,9 +513,7 @@ unet_guess_kernel(UnetIoArgs a) {
     const size_t p = (size_t)(i / 4) * S + j / 4;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-        const double q = 3 * fld(a.f1[f], a.f32, k) - 3 * fld(a.f2[f], a.f32, k) +
-                         fld(a.f3[f], a.f32, k);
-        const double g = q + (double)a.tscale[f] * a.y[p * 16 + f];
+        const double g = unet_base(a, f, k) + (double)a.tscale[f] * a.y[p * 16 + f];
         if (a.f32) static_cast<float*>(a.guess[f])[k] = (float)g;
         else static_cast<double*>(a.guess[f])[k] = g;
     }
@@ -925,6 +934,7 @@ int lcode_unet_io(const lcode_unet* n, const lcode_unet_io_args* io, int32_t whi
         a.f1[f] = io->f1[f];
         a.f2[f] = io->f2[f];
         a.f3[f] = io->f3[f];
+        a.f4[f] = io->f4[f];
         a.next[f] = io->next[f];
         a.guess[f] = io->guess[f];
         a.tscale[f] = io->tscale[f];

@@ -656,6 +656,22 @@ class GuessBackoff:
             self.next_wait = min(2 * self.next_wait, self.max_backoff) if self.backoff else 0
 
 
+def _takes_five(fn):
+    """Whether fn accepts a fifth positional argument."""
+    import inspect
+    try:
+        sig = inspect.signature(fn)
+    except (TypeError, ValueError):
+        return False
+    n = 0
+    for p in sig.parameters.values():
+        if p.kind == p.VAR_POSITIONAL:
+            return True
+        if p.kind in (p.POSITIONAL_ONLY, p.POSITIONAL_OR_KEYWORD):
+            n += 1
+    return n >= 5
+
+
 class GuidedStepper:
     """step() with a guess on the slices GuessBackoff selects: the guess is
     the quadratic extrapolation of the last three states' fields, or the
@@ -684,6 +700,9 @@ class GuidedStepper:
       fields (an accepted slice then differs from the plain step by the
       one-pass change, ~1e-3, in the transverse magnetic field);
     'alternating' — the Variant-A-aware form of it.
+    A predictor is called as predictor(s1, s2, s3, beam), with slice n-4
+    as a fifth argument once known when it takes one (UNetPredictor.guess
+    does, for its alternating base).
 
     anchor: after this many guided slices in a row one plain slice runs
     (0: never).  The gap guess is built from the previous guess, so on
@@ -706,8 +725,12 @@ class GuidedStepper:
         self.policy = GuessBackoff(backoff, max_backoff)
         variant_a = bool(getattr(config, "field_solver_variant_A", True))
         self.mode = extrapolation if predictor is None else "predictor"
+        # four states of history (the alternating forms, and a predictor
+        # that takes slice n-4 as its fifth argument)
+        self._pred_four = predictor is not None and _takes_five(predictor)
         self.extrap = FieldExtrapolator(
-            order=2, alternating=variant_a and self.mode in ("gap", "alternating"))
+            order=2, alternating=(variant_a and self.mode in ("gap", "alternating"))
+            or self._pred_four)
         self._x = None               # kept predictor fields (gap mode)
         self._x_valid = False
         self.anchor = max(int(anchor), 0)
@@ -774,6 +797,8 @@ class GuidedStepper:
                 guess = self._gap_guess(h)
             elif self.predictor is None:
                 guess = self.extrap.guess()
+            elif self._pred_four and len(h) >= 4:
+                guess = [(1.0, self.predictor(h[0], h[1], h[2], beam_ro, h[3]))]
             else:
                 guess = [(1.0, self.predictor(h[0], h[1], h[2], beam_ro))]
             st = step(self.config, self.const, self.vp, prev, beam_ro,

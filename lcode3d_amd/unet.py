@@ -76,6 +76,12 @@ class UNetPredictor:
         #: apply its last output in between (the learned correction changes
         #: on the plasma-period scale, hundreds of slices)
         self.refresh_every = 1
+        #: the extrapolation the network corrects: 'quadratic' 3 F(n-1) -
+        #: 3 F(n-2) + F(n-3), or 'alternating' 2 F(n-1) - 2 F(n-3) + F(n-4)
+        #: (Variant A's sign-alternating part carried on as the step does, so
+        #: the learned correction is smooth from slice to slice and can be
+        #: reused between evaluations); training and guesses must agree
+        self.base = "quadratic"
         self._calls = 0
         self.set_params(self.init_params(seed))
 
@@ -129,7 +135,7 @@ class UNetPredictor:
         return n
 
     def _io(self, s1, s2, s3=None, beam=0, nxt=None, planes=None, planes_f32=None,
-            targets=None, guess=None):
+            targets=None, guess=None, s4=None):
         io = _lib.LcodeUnetIo()
         f32 = s1.Ex.dtype == torch.float32
         for f, name in enumerate(FIELDS):
@@ -137,6 +143,8 @@ class UNetPredictor:
             io.f2[f] = getattr(s2, name).data_ptr()
             if s3 is not None:
                 io.f3[f] = getattr(s3, name).data_ptr()
+            if s4 is not None:
+                io.f4[f] = getattr(s4, name).data_ptr()
             if nxt is not None:
                 io.next[f] = nxt[f].data_ptr()
             if guess is not None:
@@ -175,12 +183,20 @@ class UNetPredictor:
             self.scale = scale
         return out
 
-    def raw_targets(self, s1, s2, s3, f1):
-        """Unscaled FP32 targets [S, S, 6]: 4 x 4 means of F1(n) - Q(n)."""
+    def _s4(self, s4):
+        if self.base == "alternating":
+            if s4 is None:
+                raise ValueError("the alternating base needs slice n-4")
+            return s4
+        return None
+
+    def raw_targets(self, s1, s2, s3, f1, s4=None):
+        """Unscaled FP32 targets [S, S, 6]: 4 x 4 means of F1(n) - Q(n)
+        (Q the base extrapolation; s4 = slice n-4 for the alternating one)."""
         out = torch.empty((self.S, self.S, 6), dtype=torch.float32, device=self.dev)
         tscale, self.tscale = self.tscale, np.ones(N_OUT, dtype=np.float32)
         try:
-            self._run_io(self._io(s1, s2, s3, nxt=f1, targets=out), 1)
+            self._run_io(self._io(s1, s2, s3, nxt=f1, targets=out, s4=self._s4(s4)), 1)
         finally:
             self.tscale = tscale
         return out
@@ -243,11 +259,12 @@ class UNetPredictor:
             hist.append(float(torch.stack(losses).mean().item()))
         return hist
 
-    def guess(self, s1, s2, s3, beam):
-        """Field guess of slice n from states n-1, n-2, n-3 and beam n: the
-        six fields in the states' grid type (buffers reused per call).  The
-        network runs on every refresh_every-th call; the other calls add
-        its last output to this slice's extrapolation."""
+    def guess(self, s1, s2, s3, beam, s4=None):
+        """Field guess of slice n from states n-1, n-2, n-3 (and n-4 for the
+        alternating base; without it the first slices use the quadratic
+        one) and beam n: the six fields in the states' grid type (buffers
+        reused per call).  The network runs on every refresh_every-th call;
+        the other calls add its last output to this slice's extrapolation."""
         key = s1.Ex.dtype
         if key not in self._guess:
             self._guess[key] = tuple(torch.empty((self.N, self.N), dtype=key, device=self.dev)
@@ -257,7 +274,8 @@ class UNetPredictor:
             self._run_io(self._io(s1, s2, beam=beam, planes=self._x), 0)
             self.infer(self._x)
         self._calls += 1
-        self._run_io(self._io(s1, s2, s3, guess=g), 2)
+        older = s4 if self.base == "alternating" else None
+        self._run_io(self._io(s1, s2, s3, guess=g, s4=older), 2)
         return g
 
 
@@ -267,8 +285,10 @@ def collect_training_data(lc, net, config, runs, slices, every=1, first=3,
     sigma) of `runs`, one run of `slices` slices from init; every `every`
     slices from slice `first` on, the raw input planes of states n-1, n-2
     and beam n and the raw target F1(n) - Q(n), with F1(n) the plain
-    predictor pass's fields (lcode.predictor_fields).  Returns the list of
-    (raw_x, raw_t) and the per-channel sums of squares."""
+    predictor pass's fields (lcode.predictor_fields) and Q(n) the net's
+    base extrapolation (four states back for the alternating base).
+    Returns the list of (raw_x, raw_t) and the per-channel sums of
+    squares."""
     if beam_fn is None:
         from .beams import cosine_gaussian as beam_fn
     samples = []
@@ -277,18 +297,19 @@ def collect_training_data(lc, net, config, runs, slices, every=1, first=3,
     for amp, sig in runs:
         _, _, const, vp, st = lc.init(config)
         hist = []
+        depth = 4 if net.base == "alternating" else 3
         for i in range(slices):
             beam = beam_fn(config, i, amplitude=amp, sigma=sig)
-            if len(hist) >= 3 and i >= first and (i - first) % every == 0:
+            if len(hist) >= depth and i >= first and (i - first) % every == 0:
                 f1 = lc.predictor_fields(config, const, vp, hist[0], beam)
                 x = net.raw_inputs(hist[0], hist[1], beam)
-                t = net.raw_targets(hist[0], hist[1], hist[2], f1)
+                t = net.raw_targets(hist[0], hist[1], hist[2], f1, *hist[3:4])
                 sx += x[..., :N_IN].double().pow(2).sum(dim=(0, 1))
                 st_ += t.double().pow(2).sum(dim=(0, 1))
                 samples.append((x, t))
             st = lc.step(config, const, vp, st, beam)
             hist.insert(0, st)
-            del hist[3:]
+            del hist[depth:]
             if on_slice:
                 on_slice(i)
     return samples, sx, st_

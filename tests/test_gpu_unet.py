@@ -116,13 +116,21 @@ def _mean4(a, s):
     return p.reshape(s, 4, s, 4).sum(axis=(1, 3)) / 16
 
 
+@pytest.mark.parametrize("base", ["quadratic", "alternating"])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
-def test_io_planes_targets_guess(lc, dtype):
+def test_io_planes_targets_guess(lc, dtype, base):
     from lcode3d_amd.unet import FIELDS, UNetPredictor
-    say(f"unet io planes, {dtype}")
+    say(f"unet io planes, {dtype}, {base} base")
     rng = np.random.default_rng(2)
     net = UNetPredictor(GRID, seed=0)
-    s1, s2, s3 = (_state(rng, GRID, dtype) for _ in range(3))
+    net.base = base
+    s1, s2, s3, s4 = (_state(rng, GRID, dtype) for _ in range(4))
+
+    def base_of(name):
+        a = [h(getattr(s, name)) for s in (s1, s2, s3, s4)]
+        if base == "alternating":
+            return 2 * a[0] - 2 * a[2] + a[3]
+        return 3 * a[0] - 3 * a[1] + a[2]
     beam = torch.as_tensor(rng.standard_normal((GRID, GRID)), device="cuda")
     raw = host(net.raw_inputs(s1, s2, beam))
     h = lambda t: t.double().cpu().numpy()  # noqa: E731
@@ -134,15 +142,15 @@ def test_io_planes_targets_guess(lc, dtype):
     assert np.allclose(raw[..., 13], _mean4(h(s1.ro), S0), rtol=1e-6, atol=1e-6)
     nxt = [torch.as_tensor(rng.standard_normal((GRID, GRID)), device="cuda").to(dtype)
            for _ in FIELDS]
-    tg = host(net.raw_targets(s1, s2, s3, nxt))
+    tg = host(net.raw_targets(s1, s2, s3, nxt, s4))
     for f, name in enumerate(FIELDS):
-        q = 3 * h(getattr(s1, name)) - 3 * h(getattr(s2, name)) + h(getattr(s3, name))
+        q = base_of(name)
         assert np.allclose(tg[..., f], _mean4(h(nxt[f]) - q, S0), rtol=1e-5, atol=1e-5)
     net.tscale = np.linspace(.5, 2, 6).astype(np.float32)
-    g = net.guess(s1, s2, s3, beam)
+    g = net.guess(s1, s2, s3, beam, s4)
     y = host(net.y.view(S0, S0, 16))
     for f, name in enumerate(FIELDS):
-        q = 3 * h(getattr(s1, name)) - 3 * h(getattr(s2, name)) + h(getattr(s3, name))
+        q = base_of(name)
         up = np.repeat(np.repeat(y[..., f].astype(np.float64), 4, 0), 4, 1)[:GRID, :GRID]
         want = q + float(net.tscale[f]) * up
         tol = 1e-12 if dtype == torch.float64 else 1e-5
