@@ -32,6 +32,7 @@ struct lcode_workspace {
     SolverGeom geom{};
     double* costab = nullptr;      // quarter-wave cosine table
     double2* stage_tw = nullptr;   // static-plan stage twiddles
+    double2* wave_tw = nullptr;    // wave FFT twiddles W_L^t
     float* costab32 = nullptr;
     int fp32 = 0;                  // FP32 transforms (FFT32, MIXED)
     int solver = 0;                // LCODE_SOLVER_*
@@ -381,6 +382,18 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         g.stage_tw = ws->stage_tw;
     }
 
+    g.wave_tw = nullptr;
+    if (solver_wave_length(L)) {
+        // W_L^t = exp(-2 pi i t / L) for every t < L
+        std::vector<double2> wt(L);
+        for (int t = 0; t < L; ++t) {
+            const long double a = 2.0L * 3.14159265358979323846264338327950288L * t / L;
+            wt[t] = make_double2((double)std::cos(a), (double)-std::sin(a));
+        }
+        if ((rc = upload(ws, &ws->wave_tw, wt))) return fail(rc);
+        g.wave_tw = ws->wave_tw;
+    }
+
     std::vector<double> costab(L / 4 + 1);
     for (int r = 0; r <= L / 4; ++r) {
         const long double a = 2.0L * 3.14159265358979323846264338327950288L * r / L;
@@ -528,8 +541,9 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         LCODE_HIP(hipMemset(ws->zero64, 0, n2 * sizeof(double)));
     }
     {
-        // last FFT pass: ceil(N / (2 P)) blocks per field, six fields
-        const int per_block = 2 * ws->geom.P;
+        // last FFT pass: ceil(N / (2 P)) blocks per field (one line pair per
+        // block in the wave FFT), six fields
+        const int per_block = ws->geom.wave_tw ? 2 : 2 * ws->geom.P;
         const size_t nblk = (size_t)6 * ((N + per_block - 1) / per_block);
         if ((rc = dev_alloc(ws, (void**)&ws->resid_part, 2 * nblk * sizeof(double))))
             return fail(rc);

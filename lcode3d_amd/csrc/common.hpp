@@ -587,11 +587,13 @@ struct SolverGeom {
     int static_L;             // L with a compile-time plan, else 0
     int tw_lds;               // static plan: twiddles copied to LDS
     int P, Tp;                // pairs of lines per block, threads per pair
+    const double2* wave_tw;   // wave FFT (L = 1280, 2560): W_L^t, t < L; else NULL
 };
 
 int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
                  Profiler* prof);
 bool solver_static_length(int L);
+bool solver_wave_length(int L);   // L has the wave FFT passes
 int solver_padded_len(int L);
 int solver_tw_count(int L);   // static plan: twiddle table entries
 int solver_threads_per_pair(int L, bool fp32, bool static_plan);   // threads of one line pair

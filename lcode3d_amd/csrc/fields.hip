@@ -961,6 +961,443 @@ guess_kernel(GuessArgs p) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Wave FFT (L = 1280 and 2560): one line pair per block of T = L / 20 lanes
+// (one wave at L = 1280, two at 2560).  With n = l + T m and k = k1 + 20 k2,
+//   X[k1 + 20 k2] = sum_l W_T^(l k2) W_L^(l k1) sum_m x[l + T m] W_20^(m k1):
+// lane l holds the 20 points x[l + T m] in registers and transforms them
+// there (a Good-Thomas 4 x 5 DFT, no internal twiddles), multiplies by
+// W_L^(l k1), and the 20 transforms of length T across the lanes run as
+// Stockham radix-4 (and a last radix-2) stages through LDS, batched over k1
+// with k1 fastest: z[row * 20 + (k1 ^ swz(row))], lanes taking butterflies
+// beta = k1 + 20 b in order, so every LDS access of an instruction is a run
+// of consecutive 16-B points (the XOR spreads the first exchange's
+// stride-20 stores over the banks).  The last stage keeps its outputs in
+// registers: lane l's output (r, q) is X[l + T r + q L / R], consecutive
+// across lanes, so the passes read and write global memory coalesced.
+// Against the radix-4/5 plan above: four LDS exchanges per transform instead
+// of six, no barrier across waves at L = 1280, a third less VALU work.
+// ---------------------------------------------------------------------------
+
+constexpr int kWR1 = 20;   // points per lane
+
+template <int L> struct WaveLanes {
+    static constexpr int T = L / kWR1;
+    static_assert(T == 64 || T == 128, "wave FFT: L = 1280 or 2560");
+};
+
+__device__ __forceinline__ int wswz(int row) { return (row >> 1) & 3; }
+__device__ __forceinline__ int waddr(int row, int k1) {
+    return row * kWR1 + (k1 ^ wswz(row));
+}
+
+// In-place DFT of 20 points: Good-Thomas with n = (5 n1 + 4 n2) mod 20,
+// k = (5 k1 + 16 k2) mod 20 (W_20^(nk) = W_4^(n1 k1) W_5^(n2 k2)).
+template <typename V>
+__device__ __forceinline__ void dft20(V (&x)[kWR1]) {
+    V a[5][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 5; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) a[n2][n1] = x[(5 * n1 + 4 * n2) % 20];
+        Dft<4, V>::run(a[n2]);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        V b[5];
+#pragma unroll
+        for (int n2 = 0; n2 < 5; ++n2) b[n2] = a[n2][k1];
+        Dft<5, V>::run(b);
+#pragma unroll
+        for (int k2 = 0; k2 < 5; ++k2) x[(5 * k1 + 16 * k2) % 20] = b[k2];
+    }
+}
+
+// Lane l's extension points x[l + T m] of the type-1 transform whose n
+// inputs (pairs) are e[0 .. n) in LDS: cosine x[p] = e[p] (p <= L/2),
+// e[L - p] above; sine x[p] = e[p - 1] below L/2, -e[L - 1 - p] above,
+// 0 at p = 0 and L/2.
+template <int L, typename V>
+__device__ __forceinline__ void wext_load(const V* e, int xf, int lane,
+                                          V (&x)[kWR1]) {
+    constexpr int T = WaveLanes<L>::T, H = L / 2;
+#pragma unroll
+    for (int m = 0; m < kWR1; ++m) {
+        const int p = lane + T * m;        // p < H exactly for m < 10
+        if (xf == XF_DCT) {
+            x[m] = e[m < kWR1 / 2 ? p : L - p];
+        } else if (m < kWR1 / 2) {
+            const V v = e[max(p - 1, 0)];
+            x[m] = p == 0 ? mk<V>(0, 0) : v;
+        } else {
+            const V v = e[min(L - 1 - p, H - 2)];
+            x[m] = p == H ? mk<V>(0, 0) : mk<V>(-v.x, -v.y);
+        }
+    }
+}
+
+// One Stockham stage of radix R and span SPAN over the 20 transforms of
+// length T in z; the last one leaves output (r, q) in x[r * R + q].
+template <int L, int R, int SPAN, bool LAST, typename V>
+__device__ __forceinline__ void wstage(V* z, V (&x)[kWR1],
+                                       const double2* __restrict__ tw, int lane) {
+    constexpr int T = WaveLanes<L>::T, NB = kWR1 / R, STRIDE = T / R;
+    V v[NB][R];
+    double2 w[NB];   // W_(SPAN R)^k = W_L^(k L / (SPAN R)) of each butterfly
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        const int beta = lane + T * r;
+        const int k1 = beta % kWR1, b = beta / kWR1, k = b % SPAN;
+        if constexpr (SPAN > 1) w[r] = tw[k * (L / (SPAN * R))];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[r][q] = z[waddr(b + q * STRIDE, k1)];
+    }
+    __syncthreads();   // every read of the stage before its first write
+#pragma unroll
+    for (int r = 0; r < NB; ++r) {
+        if constexpr (SPAN > 1) {
+            // the powers formed in FP64, each rounded once to the transform's
+            // precision
+            double2 wq = w[r];
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                if (q > 1) wq = cmul(wq, w[r]);
+                v[r][q] = cmul(v[r][q], mk<V>(wq.x, wq.y));
+            }
+        }
+        Dft<R, V>::run(v[r]);
+        if constexpr (LAST) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) x[r * R + q] = v[r][q];
+        } else {
+            const int beta = lane + T * r;
+            const int k1 = beta % kWR1, b = beta / kWR1, k = b % SPAN;
+            const int j = (b - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) z[waddr(j + q * SPAN, k1)] = v[r][q];
+        }
+    }
+    if constexpr (!LAST) __syncthreads();
+}
+
+// Forward FFT of length L of the extension points in x (from wext_load).
+// z: T * 20 points of LDS, free on entry (the caller synchronised after its
+// last read).  On return x[r * R + q] = X[lane + T r + q L / R], R the last
+// radix (4 at L = 1280, 2 at 2560), and z is free again.
+template <int L, typename V>
+__device__ __forceinline__ void wave_fft(V* z, V (&x)[kWR1],
+                                         const double2* __restrict__ tw, int lane) {
+    // W_L^(lane k1) = W_L^(lane 4a) W_L^(lane c) for k1 = 4a + c: seven
+    // table entries, one FP64 product each
+    double2 lo[4], hi[5];
+#pragma unroll
+    for (int c = 1; c < 4; ++c) lo[c] = tw[lane * c];
+#pragma unroll
+    for (int a = 1; a < 5; ++a) hi[a] = tw[lane * 4 * a];
+    dft20(x);
+#pragma unroll
+    for (int k1 = 1; k1 < kWR1; ++k1) {
+        const int a = k1 / 4, c = k1 % 4;
+        const double2 w = a == 0 ? lo[c] : c == 0 ? hi[a] : cmul(hi[a], lo[c]);
+        x[k1] = cmul(x[k1], mk<V>(w.x, w.y));
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < kWR1; ++k1) z[waddr(lane, k1)] = x[k1];
+    __syncthreads();
+    wstage<L, 4, 1, false>(z, x, tw, lane);
+    wstage<L, 4, 4, false>(z, x, tw, lane);
+    if constexpr (WaveLanes<L>::T == 64) {
+        wstage<L, 4, 16, true>(z, x, tw, lane);
+    } else {
+        wstage<L, 4, 16, false>(z, x, tw, lane);
+        wstage<L, 2, 64, true>(z, x, tw, lane);
+    }
+}
+
+template <int L> struct WaveOut {
+    static constexpr int R = WaveLanes<L>::T == 64 ? 4 : 2;   // last radix
+};
+
+// Calls f(u, k, a, b) for every output k of the type-1 transform held by
+// this lane (register u): cosine (L / 2 + 1 outputs) a = Re X[k],
+// b = Im X[k]; sine (L / 2 - 1 outputs) a = -Im X[k + 1], b = Re X[k + 1].
+// The registers with q < R / 2 hold X[idx] for idx < L / 2; X[L / 2] is
+// lane 0's register R / 2.
+template <int L, typename V, typename F>
+__device__ __forceinline__ void wext_out(const V (&x)[kWR1], int xf, int lane,
+                                         F&& f) {
+    constexpr int T = WaveLanes<L>::T, R = WaveOut<L>::R, H = L / 2;
+#pragma unroll
+    for (int u = 0; u < kWR1; ++u) {
+        const int r = u / R, q = u % R;
+        if (q >= R / 2) continue;
+        const int idx = lane + T * r + q * (L / R);
+        if (xf == XF_DCT) f(u, idx, (double)x[u].x, (double)x[u].y);
+        else if (idx >= 1) f(u, idx - 1, -(double)x[u].y, (double)x[u].x);
+    }
+    if (xf == XF_DCT && lane == 0) f(R / 2, H, (double)x[R / 2].x, (double)x[R / 2].y);
+}
+
+// Waves per SIMD the LDS allows (160 KiB per CU, T * 20 points per block):
+// two with FP64 points, four with FP32 (three asked for: 168 registers
+// keep the FP32 passes from spilling).
+#define LCODE_WAVE_ATTR(S) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 2 : 3)))
+
+__device__ __forceinline__ double block_sum_waves(double v, double* scratch) {
+    v = wave_sum(v);
+    if (blockDim.x == 64) return v;
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) scratch[w] = v;
+    __syncthreads();
+    double t = 0.;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += scratch[k];
+    __syncthreads();
+    return t;
+}
+
+template <int L, typename S, typename G>
+__global__ void __launch_bounds__(WaveLanes<L>::T) LCODE_WAVE_ATTR(S)
+pass_a_wave_kernel(SolverGeom g, SolveBatch b) {
+    using V = typename Vec<S>::T;
+    constexpr int T = WaveLanes<L>::T;
+    constexpr int kPer = (L / 2 + 1 + T - 1) / T;   // inputs per lane
+    __shared__ V z[T * kWR1];
+    if (!gate_open(b.gate)) return;
+    const BlockPos bp = xcd_block_rows();
+    const FieldPlan& fp = b.f[bp.field];
+    const int N = g.N, lines = fp.rowsA, lane = threadIdx.x;
+    const int ra = bp.block * 2, rb = ra + 1;
+    if (ra >= lines) return;
+    const bool vb = rb < lines;
+    {
+        double a[kPer], c[kPer];
+        rhs_rows<kPer, G>(fp.kind, fp.rowsA0 + ra, fp.rowsA0 + (vb ? rb : ra), fp.colA0,
+                          fp.lenA, lane, T, N, b.in, a, c);
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int n = lane + u * T;
+            if (n < fp.lenA) z[n] = mk<V>((S)a[u], vb ? (S)c[u] : (S)0);
+        }
+    }
+    __syncthreads();
+    V x[kWR1];
+    wext_load<L>(z, fp.xfA, lane, x);
+    __syncthreads();
+    wave_fft<L>(z, x, g.wave_tw, lane);
+    S* Tsp = static_cast<S*>(fp.T);
+    wext_out<L>(x, fp.xfA, lane, [&](int, int k, double a, double c) {
+        Tsp[(size_t)k * lines + ra] = (S)a;
+        if (vb) Tsp[(size_t)k * lines + rb] = (S)c;
+    });
+}
+
+template <int L, typename S>
+__global__ void __launch_bounds__(WaveLanes<L>::T) LCODE_WAVE_ATTR(S)
+pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
+    using V = typename Vec<S>::T;
+    constexpr int T = WaveLanes<L>::T;
+    constexpr int kPer = (L / 2 + 1 + T - 1) / T;
+    __shared__ V z[T * kWR1];
+    if (!gate_open(b.gate)) return;
+    const BlockPos bp = xcd_block();
+    const FieldPlan& fp = b.f[bp.field];
+    const int lines = fp.lenA, len = fp.rowsA, xf = fp.xfB, lane = threadIdx.x;
+    const int la = bp.block * 2, lb = la + 1;
+    if (la >= lines) return;
+    const bool vb = lb < lines;
+    S* rowa = static_cast<S*>(fp.T) + (size_t)la * len;
+    S* rowb = static_cast<S*>(fp.T) + (size_t)(vb ? lb : la) * len;
+    {
+        S xa[kPer], xb[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int n = min(lane + u * T, len - 1);
+            xa[u] = rowa[n];
+            xb[u] = rowb[n];
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int n = lane + u * T;
+            if (n < len) z[n] = mk<V>(xa[u], vb ? xb[u] : (S)0);
+        }
+    }
+    __syncthreads();
+    V x[kWR1];
+    wext_load<L>(z, xf, lane, x);
+    __syncthreads();
+    wave_fft<L>(z, x, g.wave_tw, lane);
+    {
+        // the spectral multipliers of this lane's outputs, loaded first
+        const S* tab = static_cast<const S*>(fp.table);
+        const S* ta = tab + (size_t)la * len;
+        const S* tb = tab + (size_t)(vb ? lb : la) * len;
+        S ma[kWR1], mb[kWR1];
+        wext_out<L>(x, xf, lane, [&](int u, int k, double, double) {
+            ma[u] = ta[k];
+            mb[u] = tb[k];
+        });
+        wext_out<L>(x, xf, lane, [&](int u, int k, double a, double c) {
+            z[k] = mk<V>((S)a * ma[u], vb ? (S)c * mb[u] : (S)0);
+        });
+    }
+    __syncthreads();
+    wext_load<L>(z, xf, lane, x);
+    __syncthreads();
+    wave_fft<L>(z, x, g.wave_tw, lane);
+    double sa = 0., sb = 0.;
+    wext_out<L>(x, xf, lane, [&](int, int k, double a, double c) {
+        rowa[k] = (S)a;
+        if (vb) rowb[k] = (S)c;
+        sa += a;
+        sb += c;
+    });
+    if (fp.partial) {
+        // Bz mean: the sum of all Bz equals sum_k colsum[k] * sum_i T[k][i]
+        double* scratch = reinterpret_cast<double*>(z);
+        sa = block_sum_waves(sa, scratch);
+        sb = block_sum_waves(sb, scratch);
+        if (lane == 0) {
+            fp.partial[la] = fp.colsum[la] * sa;
+            if (vb) fp.partial[lb] = fp.colsum[lb] * sb;
+        }
+    }
+}
+
+template <int L, typename S, typename G>
+__global__ void __launch_bounds__(WaveLanes<L>::T) LCODE_WAVE_ATTR(S)
+pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
+    using V = typename Vec<S>::T;
+    constexpr int T = WaveLanes<L>::T;
+    constexpr int kPer = (L / 2 + 1 + T - 1) / T;
+    __shared__ V z[T * kWR1];
+    if (!gate_open(b.gate)) return;
+    const BlockPos bp = xcd_block();
+    const FieldPlan& fp = b.f[bp.field];
+    const int N = g.N, lane = threadIdx.x;
+    const int ra = bp.block * 2, rb = ra + 1;   // ra < N: the grid has ceil(N / 2) blocks
+    const int ca = ra - fp.rowsA0, cb = rb - fp.rowsA0;
+    const bool ha = ca >= 0 && ca < fp.rowsA;
+    const bool hb = rb < N && cb >= 0 && cb < fp.rowsA;
+    {
+        const S* Tsp = static_cast<const S*>(fp.T);
+        const int stride = fp.rowsA;
+        const int cac = ha ? ca : 0, cbc = hb ? cb : 0;
+        S xa[kPer], xb[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const S* col = Tsp + (size_t)min(lane + u * T, fp.lenA - 1) * stride;
+            xa[u] = col[cac];
+            xb[u] = col[cbc];
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int n = lane + u * T;
+            if (n < fp.lenA) z[n] = mk<V>(ha ? xa[u] : (S)0, hb ? xb[u] : (S)0);
+        }
+    }
+    __syncthreads();
+    V x[kWR1];
+    wext_load<L>(z, fp.xfA, lane, x);
+    __syncthreads();
+    wave_fft<L>(z, x, g.wave_tw, lane);
+
+    double mean = 0.;
+    if (fp.partial) {
+        double s = 0.;
+        for (int k = lane; k < fp.lenA; k += T) s += fp.partial[k];
+        mean = block_sum_waves(s, reinterpret_cast<double*>(z)) / ((double)N * N);
+    }
+    const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
+                            fp.kind == F_BX || fp.kind == F_BY;
+    const bool varA = b.variantA && transverse;
+    const G* guess = b.resid ? gp<G>(b.guess[fp.kind]) : nullptr;
+    const G* prevF = gp<G>(fp.prevF);
+    G* const outF = gp<G>(fp.outF);
+    G* const outAvg = gp<G>(fp.outAvg);
+    double r_num = 0., r_den = 0.;
+    auto emit = [&](int i, int j, double v, double p, double q) {
+        const size_t idx = (size_t)i * N + j;
+        double F = v;
+        if (fp.partial) F -= mean;                         // lcode.py:362
+        if (varA) F = 2 * F - p;                           // lcode.py:869-871
+        if (outF) outF[idx] = (G)F;
+        if (outAvg) outAvg[idx] = (G)((F + p) / 2);        // lcode.py:876-881
+        if (guess) {
+            const double d = F - q;
+            r_num += d * d;
+            r_den += F * F;
+        }
+    };
+    // output column j of the lane's register u: j = lane + T r + q L / R
+    // (the cosine transform's index, 1 + the sine transform's), below L / 2
+    // for q < R / 2; column L / 2 = N - 1 is lane 0's register R / 2.
+    // Columns outside [colA0, colA0 + lenA) are exact zeros.
+    constexpr int R = WaveOut<L>::R, H = L / 2;
+    const size_t oa = (size_t)ra * N, ob = (size_t)min(rb, N - 1) * N;
+    auto value = [&](int u, int j, double& va, double& vb) {
+        va = vb = 0.;
+        const int k = j - fp.colA0;
+        if (k >= 0 && k < fp.lenA) {
+            if (fp.xfA == XF_DCT) {
+                va = x[u].x;
+                vb = x[u].y;
+            } else {
+                va = -(double)x[u].y;
+                vb = x[u].x;
+            }
+        }
+        if (!ha) va = 0.;
+        if (!hb) vb = 0.;
+    };
+    {
+        constexpr int kOut = kWR1 / 2;   // registers with q < R / 2
+        G pa[kOut], pb[kOut], qa[kOut], qb[kOut];
+#pragma unroll
+        for (int s = 0; s < kOut; ++s) {
+            const int r = s / (R / 2), q = s % (R / 2), u = r * R + q;
+            (void)u;
+            const int j = lane + T * r + q * (L / R);
+            pa[s] = pb[s] = qa[s] = qb[s] = (G)0;
+            if (prevF) {
+                pa[s] = prevF[oa + j];
+                pb[s] = prevF[ob + j];
+            }
+            if (guess) {
+                qa[s] = guess[oa + j];
+                qb[s] = guess[ob + j];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kOut; ++s) {
+            const int r = s / (R / 2), q = s % (R / 2), u = r * R + q;
+            const int j = lane + T * r + q * (L / R);
+            double va, vb;
+            value(u, j, va, vb);
+            emit(ra, j, va, pa[s], qa[s]);
+            if (rb < N) emit(rb, j, vb, pb[s], qb[s]);
+        }
+    }
+    if (lane == 0) {
+        double va, vb;
+        value(R / 2, H, va, vb);
+        emit(ra, H, va, prevF ? (double)prevF[oa + H] : 0., guess ? (double)guess[oa + H] : 0.);
+        if (rb < N)
+            emit(rb, H, vb, prevF ? (double)prevF[ob + H] : 0., guess ? (double)guess[ob + H] : 0.);
+    }
+    if (b.resid) {
+        double* scratch = reinterpret_cast<double*>(z);
+        r_num = block_sum_waves(r_num, scratch);
+        r_den = block_sum_waves(r_den, scratch);
+        if (lane == 0) {
+            const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+            b.resid_part[2 * lin] = r_num;
+            b.resid_part[2 * lin + 1] = r_den;
+        }
+    }
+}
+
 }  // namespace
 
 template <int L_, bool TW_LDS, typename S, typename G>
@@ -1013,6 +1450,12 @@ int solver_static_radix(int rem) { return next_radix(rem); }
 #define LCODE_FFT_STATIC_PLANS 1
 #endif
 
+// LCODE_FFT_WAVE (1): the wave FFT passes for L = 1280 and 2560 (grids of
+// 641^2 and 1281^2); 0 keeps the radix-4/5 block plan there too.
+#ifndef LCODE_FFT_WAVE
+#define LCODE_FFT_WAVE 1
+#endif
+
 bool solver_static_length(int L) {
     if (!LCODE_FFT_STATIC_PLANS) return false;
     switch (L) {
@@ -1024,6 +1467,40 @@ bool solver_static_length(int L) {
     }
 }
 
+template <int L, typename S, typename G>
+static int launch_wave_passes(const SolverGeom& g, const SolveBatch& b,
+                              hipStream_t s, Profiler* prof, int max_a, int max_b) {
+    const dim3 block(WaveLanes<L>::T);
+    {
+        ProfScope ps(prof, "solve_pass_a", s, true);
+        launch_timed(ps, pass_a_wave_kernel<L, S, G>, dim3((max_a + 1) / 2, b.nfields),
+                     block, 0, s, g, b);
+    }
+    LCODE_HIP(hipGetLastError());
+    {
+        ProfScope ps(prof, "solve_pass_b", s, true);
+        launch_timed(ps, pass_b_wave_kernel<L, S>, dim3((max_b + 1) / 2, b.nfields),
+                     block, 0, s, g, b);
+    }
+    LCODE_HIP(hipGetLastError());
+    const dim3 gc((g.N + 1) / 2, b.nfields);
+    {
+        ProfScope ps(prof, "solve_pass_c", s, true);
+        launch_timed(ps, pass_c_wave_kernel<L, S, G>, gc, block, 0, s, g, b);
+    }
+    LCODE_HIP(hipGetLastError());
+    if (b.resid) {
+        ProfScope ps(prof, "resid_reduce", s);
+        resid_reduce_kernel<<<1, 256, 0, s>>>(b.resid_part, (int)(gc.x * gc.y), b.resid);
+        LCODE_HIP(hipGetLastError());
+    }
+    return LCODE_OK;
+}
+
+bool solver_wave_length(int L) {
+    return LCODE_FFT_WAVE && (L == 1280 || L == 2560);
+}
+
 template <typename S, typename G>
 static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
                           hipStream_t s, Profiler* prof) {
@@ -1033,6 +1510,22 @@ static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
     for (int f = 0; f < b.nfields; ++f) {
         max_a = max_a > b.f[f].rowsA ? max_a : b.f[f].rowsA;
         max_b = max_b > b.f[f].lenA ? max_b : b.f[f].lenA;
+    }
+    if (g.wave_tw) {
+        // the wave passes take a cosine transform of N points from column
+        // (or line) 0 and a sine transform of N - 2 from 1
+        for (int f = 0; f < b.nfields; ++f) {
+            const FieldPlan& p = b.f[f];
+            const bool ok_a = p.xfA == XF_DCT ? p.lenA == g.N && p.colA0 == 0
+                                              : p.lenA == g.N - 2 && p.colA0 == 1;
+            const bool ok_b = p.rowsA == (p.xfB == XF_DCT ? g.N : g.N - 2);
+            if (!ok_a || !ok_b) {
+                set_error("internal: field plan outside the wave FFT layout");
+                return LCODE_E_INVALID;
+            }
+        }
+        if (g.fft.L == 1280) return launch_wave_passes<1280, S, G>(g, b, s, prof, max_a, max_b);
+        if (g.fft.L == 2560) return launch_wave_passes<2560, S, G>(g, b, s, prof, max_a, max_b);
     }
     const dim3 ga((max_a + per_block - 1) / per_block, b.nfields);
     const dim3 gb((max_b + per_block - 1) / per_block, b.nfields);

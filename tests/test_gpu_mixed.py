@@ -34,7 +34,7 @@ def _cfg(N, precision="fp32", **kw):
     return types.SimpleNamespace(**base)
 
 
-@pytest.mark.parametrize("N", [33, 41, 129, 641])
+@pytest.mark.parametrize("N", [33, 41, 129, 641, 1281])
 def test_fp32_solves(lc, oracle, N):
     say(f"fp32 solves N={N}")
     rng = np.random.default_rng(N)

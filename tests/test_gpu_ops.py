@@ -29,7 +29,8 @@ def _cfg(N, h=.05, dxi=.05, trick=1, variant_A=True, c=2, f=2, pad=10):
         plasma_padding_steps=pad, plasma_coarseness=c, plasma_fineness=f)
 
 
-SIZES = [17, 25, 33, 41, 129, 641]   # 25, 41: runtime FFT plans (L = 48, 80)
+SIZES = [17, 25, 33, 41, 129, 641, 1281]   # 25, 41: runtime FFT plans (L = 48, 80);
+# 641, 1281: the wave FFT (L = 1280, 2560)
 
 
 @pytest.mark.parametrize("N", SIZES)
