@@ -383,7 +383,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     }
 
     g.wave_tw = nullptr;
-    if (solver_wave_length(L)) {
+    if (solver_wave_length(L) && !ws->fp32) {   // FP64 transforms only
         // W_L^t = exp(-2 pi i t / L) for every t < L
         std::vector<double2> wt(L);
         for (int t = 0; t < L; ++t) {
@@ -542,8 +542,8 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     }
     {
         // last FFT pass: ceil(N / (2 P)) blocks per field (one line pair per
-        // block in the wave FFT), six fields
-        const int per_block = ws->geom.wave_tw ? 2 : 2 * ws->geom.P;
+        // block in the wave FFT), six fields: sized for P = 1, the most
+        const int per_block = 2;
         const size_t nblk = (size_t)6 * ((N + per_block - 1) / per_block);
         if ((rc = dev_alloc(ws, (void**)&ws->resid_part, 2 * nblk * sizeof(double))))
             return fail(rc);

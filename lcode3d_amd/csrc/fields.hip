@@ -1141,7 +1141,7 @@ __device__ __forceinline__ void wext_out(const V (&x)[kWR1], int xf, int lane,
 
 // Waves per SIMD the LDS allows (160 KiB per CU, T * 20 points per block):
 // two with FP64 points, four with FP32 (three asked for: 168 registers
-// keep the FP32 passes from spilling).
+// keep the FP32 passes from spilling; the FP32 forms are not launched).
 #define LCODE_WAVE_ATTR(S) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 2 : 3)))
 
 __device__ __forceinline__ double block_sum_waves(double v, double* scratch) {
@@ -1450,8 +1450,8 @@ int solver_static_radix(int rem) { return next_radix(rem); }
 #define LCODE_FFT_STATIC_PLANS 1
 #endif
 
-// LCODE_FFT_WAVE (1): the wave FFT passes for L = 1280 and 2560 (grids of
-// 641^2 and 1281^2); 0 keeps the radix-4/5 block plan there too.
+// LCODE_FFT_WAVE (1): the wave FFT passes for the FP64 transforms of L = 1280
+// and 2560 (grids of 641^2 and 1281^2); 0 keeps the radix-4/5 plan there too.
 #ifndef LCODE_FFT_WAVE
 #define LCODE_FFT_WAVE 1
 #endif
@@ -1511,7 +1511,11 @@ static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
         max_a = max_a > b.f[f].rowsA ? max_a : b.f[f].rowsA;
         max_b = max_b > b.f[f].lenA ? max_b : b.f[f].lenA;
     }
-    if (g.wave_tw) {
+    // the wave FFT runs the FP64 transforms only: in FP32 the radix-4/5 plan,
+    // at three waves per line pair with the twiddles in LDS, measured faster
+    // (mixed six-field solve at 641^2: 67.8 against 71.0 us; at 1281^2 the
+    // mixed step 846 against 798 xi-steps/s)
+    if constexpr (std::is_same<S, double>::value) if (g.wave_tw) {
         // the wave passes take a cosine transform of N points from column
         // (or line) 0 and a sine transform of N - 2 from 1
         for (int f = 0; f < b.nfields; ++f) {
