@@ -370,8 +370,13 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     __shared__ double acc[4][kTile][kTile];
     __shared__ int last_block;
     __shared__ FineTab ti, tj;
-    __shared__ double cpos[2][kCoarseMax * kCoarseMax];   // x_offt, y_offt
-    __shared__ A cval[5][kCoarseMax * kCoarseMax];        // m q px py pz
+    // the staged coarse particles in 16-byte pairs where the arithmetic
+    // type allows, so that one LDS read brings two of a corner's values:
+    // (x_offt, y_offt), (m, q), (px, py), pz
+    using A2 = typename std::conditional<std::is_same<A, double>::value, double2, float2>::type;
+    __shared__ double2 cpos[kCoarseMax * kCoarseMax];
+    __shared__ A2 cmq[kCoarseMax * kCoarseMax], cpxy[kCoarseMax * kCoarseMax];
+    __shared__ A cpz[kCoarseMax * kCoarseMax];
     __shared__ int range[4];
     __shared__ int tile_nan;
     const int N = a.N;
@@ -471,10 +476,12 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             const double* src[7] = {a.x_offt, a.y_offt, a.m, a.q, a.px, a.py, a.pz};
             for (int k = threadIdx.x; k < cH * cW; k += blockDim.x) {
                 const size_t g = (size_t)(cI0 + k / cW) * a.nc + cJ0 + k % cW;
-                cpos[0][k] = src[0][g];
-                cpos[1][k] = src[1][g];
-#pragma unroll
-                for (int c = 0; c < 5; ++c) cval[c][k] = (A)src[2 + c][g];
+                cpos[k] = make_double2(src[0][g], src[1][g]);
+                cmq[k].x = (A)src[2][g];
+                cmq[k].y = (A)src[3][g];
+                cpxy[k].x = (A)src[4][g];
+                cpxy[k].y = (A)src[5][g];
+                cpz[k] = (A)src[6][g];
             }
             // every thread has read the rectangle's corners: turn the
             // neighbour indices into offsets in it, rows scaled by its
@@ -507,9 +514,10 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
                 // rectangle above)
                 const int pp = ti.P[li] + tj.P[lj], pn = ti.P[li] + tj.Nn[lj];
                 const int np = ti.Nn[li] + tj.P[lj], nn = ti.Nn[li] + tj.Nn[lj];
-#define LCODE_MIXP(c) (wA * cpos[c][pp] + wB * cpos[c][pn] + wC * cpos[c][np] + wD * cpos[c][nn])
-                x = ti.fg[li] + LCODE_MIXP(0);
-                y = tj.fg[lj] + LCODE_MIXP(1);
+                const double2 cA = cpos[pp], cB = cpos[pn], cC = cpos[np], cD = cpos[nn];
+#define LCODE_MIXP(c) (wA * cA.c + wB * cB.c + wC * cC.c + wD * cD.c)
+                x = ti.fg[li] + LCODE_MIXP(x);
+                y = tj.fg[lj] + LCODE_MIXP(y);
 #undef LCODE_MIXP
             } else {
                 const int nc = a.nc;
@@ -527,9 +535,14 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             if constexpr (STAGED) {
                 const int pp = ti.P[li] + tj.P[lj], pn = ti.P[li] + tj.Nn[lj];
                 const int np = ti.Nn[li] + tj.P[lj], nn = ti.Nn[li] + tj.Nn[lj];
-#define LCODE_MIX(c) (Ag * cval[c][pp] + Bg * cval[c][pn] + Cg * cval[c][np] + Dg * cval[c][nn])
-                deposit_values<A>(sm * LCODE_MIX(0), sm * LCODE_MIX(1), sm * LCODE_MIX(2),
-                                  sm * LCODE_MIX(3), sm * LCODE_MIX(4), v);
+                const A2 mA = cmq[pp], mB = cmq[pn], mC = cmq[np], mD = cmq[nn];
+                const A2 pA = cpxy[pp], pB = cpxy[pn], pC = cpxy[np], pD = cpxy[nn];
+#define LCODE_MIX(a, b, c, d) (Ag * (a) + Bg * (b) + Cg * (c) + Dg * (d))
+                deposit_values<A>(sm * LCODE_MIX(mA.x, mB.x, mC.x, mD.x),
+                                  sm * LCODE_MIX(mA.y, mB.y, mC.y, mD.y),
+                                  sm * LCODE_MIX(pA.x, pB.x, pC.x, pD.x),
+                                  sm * LCODE_MIX(pA.y, pB.y, pC.y, pD.y),
+                                  sm * LCODE_MIX(cpz[pp], cpz[pn], cpz[np], cpz[nn]), v);
 #undef LCODE_MIX
             } else {
                 const int nc = a.nc;
