@@ -290,8 +290,9 @@ constexpr int kCoarseMax = kDepCoarseMax;   // coarse particles per side staged 
 constexpr int kExtMax = kTile + 14;   // initial cells per side: kTile + 2 halo + 2, halo <= 6
 
 struct FineTab {                 // per fine index along one axis
-    double ip[kFineMax], in[kFineMax], fg[kFineMax];
-    int P[kFineMax], Nn[kFineMax], cell[kFineMax];
+    double2 w[kFineMax];         // interpolation weights (ip, in)
+    double fg[kFineMax];         // initial position
+    int4 idx[kFineMax];          // coarse neighbours (P, Nn), initial cell, -
 };
 
 // First fine index whose initial cell is >= c, found by one wave (every
@@ -451,12 +452,10 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             const int f = isj ? fj_a + k - ni : fi_a + k;
             FineTab& t = isj ? tj : ti;
             const int s = isj ? k - ni : k;
-            t.ip[s] = a.ip[f];
-            t.in[s] = a.in[f];
+            t.w[s] = make_double2(a.ip[f], a.in[f]);
             t.fg[s] = a.fg[f];
-            t.P[s] = clampi((int)a.P[f], a.nc);
-            t.Nn[s] = clampi((int)a.Nn[f], a.nc);
-            t.cell[s] = cell_of(a.fg[f], N, a.h);
+            t.idx[s] = make_int4(clampi((int)a.P[f], a.nc), clampi((int)a.Nn[f], a.nc),
+                                 cell_of(a.fg[f], N, a.h), 0);
         }
         __syncthreads();
         // coarse rectangle referenced by this block's fine particles
@@ -464,10 +463,10 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         if constexpr (STAGED) {
             // indices_prev/next grow with the fine index, so the rectangle
             // is [P[first], Nn[last]] on each axis; the host checked the fit
-            cI0 = ni ? ti.P[0] : 0;
-            cJ0 = nj ? tj.P[0] : 0;
-            int cH = ni ? ti.Nn[ni - 1] - cI0 + 1 : 0;
-            cW = nj ? tj.Nn[nj - 1] - cJ0 + 1 : 0;
+            cI0 = ni ? ti.idx[0].x : 0;
+            cJ0 = nj ? tj.idx[0].x : 0;
+            int cH = ni ? ti.idx[ni - 1].y - cI0 + 1 : 0;
+            cW = nj ? tj.idx[nj - 1].y - cJ0 + 1 : 0;
             if (cH > kCoarseMax || cW > kCoarseMax || cH < 0 || cW < 0) {
                 if (threadIdx.x == 0) atomicAdd(a.fault, 1);
                 cH = cH < 0 ? 0 : (cH > kCoarseMax ? kCoarseMax : cH);
@@ -491,12 +490,12 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             const int hmax = max(cH - 1, 0), wmax = max(cW - 1, 0);
             for (int k = threadIdx.x; k < ni + nj; k += blockDim.x) {
                 if (k < ni) {
-                    ti.P[k] = min(max(ti.P[k] - cI0, 0), hmax) * cW;
-                    ti.Nn[k] = min(max(ti.Nn[k] - cI0, 0), hmax) * cW;
+                    ti.idx[k].x = min(max(ti.idx[k].x - cI0, 0), hmax) * cW;
+                    ti.idx[k].y = min(max(ti.idx[k].y - cI0, 0), hmax) * cW;
                 } else {
                     const int s = k - ni;
-                    tj.P[s] = min(max(tj.P[s] - cJ0, 0), wmax);
-                    tj.Nn[s] = min(max(tj.Nn[s] - cJ0, 0), wmax);
+                    tj.idx[s].x = min(max(tj.idx[s].x - cJ0, 0), wmax);
+                    tj.idx[s].y = min(max(tj.idx[s].y - cJ0, 0), wmax);
                 }
             }
         }
@@ -507,13 +506,15 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         // particles whose stencil reaches the tile) its deposition values
         // (lcode.py:651-655)
         auto position = [&](int li, int lj, double& x, double& y) {
-            const double wA = ti.ip[li] * tj.ip[lj], wB = ti.ip[li] * tj.in[lj];
-            const double wC = ti.in[li] * tj.ip[lj], wD = ti.in[li] * tj.in[lj];
+            const double2 wi = ti.w[li], wj = tj.w[lj];
+            const int4 xi = ti.idx[li], xj = tj.idx[lj];
+            const double wA = wi.x * wj.x, wB = wi.x * wj.y;
+            const double wC = wi.y * wj.x, wD = wi.y * wj.y;
             if constexpr (STAGED) {
                 // row offset + column offset (both clamped into the staged
                 // rectangle above)
-                const int pp = ti.P[li] + tj.P[lj], pn = ti.P[li] + tj.Nn[lj];
-                const int np = ti.Nn[li] + tj.P[lj], nn = ti.Nn[li] + tj.Nn[lj];
+                const int pp = xi.x + xj.x, pn = xi.x + xj.y;
+                const int np = xi.y + xj.x, nn = xi.y + xj.y;
                 const double2 cA = cpos[pp], cB = cpos[pn], cC = cpos[np], cD = cpos[nn];
 #define LCODE_MIXP(c) (wA * cA.c + wB * cB.c + wC * cC.c + wD * cD.c)
                 x = ti.fg[li] + LCODE_MIXP(x);
@@ -521,8 +522,8 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
 #undef LCODE_MIXP
             } else {
                 const int nc = a.nc;
-                const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
-                const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
+                const int pp = xi.x * nc + xj.x, pn = xi.x * nc + xj.y;
+                const int np = xi.y * nc + xj.x, nn = xi.y * nc + xj.y;
 #define LCODE_MIXP(arr) (wA * arr[pp] + wB * arr[pn] + wC * arr[np] + wD * arr[nn])
                 x = ti.fg[li] + LCODE_MIXP(a.x_offt);
                 y = tj.fg[lj] + LCODE_MIXP(a.y_offt);
@@ -530,11 +531,13 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
             }
         };
         auto values = [&](int li, int lj, A* v) {
-            const A Ag = (A)(ti.ip[li] * tj.ip[lj]), Bg = (A)(ti.ip[li] * tj.in[lj]);
-            const A Cg = (A)(ti.in[li] * tj.ip[lj]), Dg = (A)(ti.in[li] * tj.in[lj]);
+            const double2 wi = ti.w[li], wj = tj.w[lj];
+            const int4 xi = ti.idx[li], xj = tj.idx[lj];
+            const A Ag = (A)(wi.x * wj.x), Bg = (A)(wi.x * wj.y);
+            const A Cg = (A)(wi.y * wj.x), Dg = (A)(wi.y * wj.y);
             if constexpr (STAGED) {
-                const int pp = ti.P[li] + tj.P[lj], pn = ti.P[li] + tj.Nn[lj];
-                const int np = ti.Nn[li] + tj.P[lj], nn = ti.Nn[li] + tj.Nn[lj];
+                const int pp = xi.x + xj.x, pn = xi.x + xj.y;
+                const int np = xi.y + xj.x, nn = xi.y + xj.y;
                 const A2 mA = cmq[pp], mB = cmq[pn], mC = cmq[np], mD = cmq[nn];
                 const A2 pA = cpxy[pp], pB = cpxy[pn], pC = cpxy[np], pD = cpxy[nn];
 #define LCODE_MIX(a, b, c, d) (Ag * (a) + Bg * (b) + Cg * (c) + Dg * (d))
@@ -546,8 +549,8 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
 #undef LCODE_MIX
             } else {
                 const int nc = a.nc;
-                const int pp = ti.P[li] * nc + tj.P[lj], pn = ti.P[li] * nc + tj.Nn[lj];
-                const int np = ti.Nn[li] * nc + tj.P[lj], nn = ti.Nn[li] * nc + tj.Nn[lj];
+                const int pp = xi.x * nc + xj.x, pn = xi.x * nc + xj.y;
+                const int np = xi.y * nc + xj.x, nn = xi.y * nc + xj.y;
 #define LCODE_MIX(arr) (Ag * (A)arr[pp] + Bg * (A)arr[pn] + Cg * (A)arr[np] + Dg * (A)arr[nn])
                 deposit_values<A>(sm * LCODE_MIX(a.m), sm * LCODE_MIX(a.q),
                                   sm * LCODE_MIX(a.px), sm * LCODE_MIX(a.py),
@@ -562,7 +565,7 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
         // TSC centre and weights of the particle; false when it moved more
         // than the halo from its initial cell (its home tile lists it)
         auto locate = [&](int li, int lj, double x, double y, TscT<A>& t) {
-            const int ci = ti.cell[li], cj = tj.cell[lj];   // initial cell
+            const int ci = ti.idx[li].z, cj = tj.idx[lj].z;   // initial cell
             const bool home = ci >= ci0 && ci < ci0 + kTile &&
                               cj >= cj0 && cj < cj0 + kTile;
             if (!(fabs(x) <= kFarCells * a.h && fabs(y) <= kFarCells * a.h)) {
