@@ -49,7 +49,7 @@ struct lcode_workspace {
     double* lam_i = nullptr;       // [M] eigenvalues k = 1 .. N - 2 (DST)
     double* lam_j = nullptr;       // [N] eigenvalues k = 0 .. N - 1 (DCT)
     double lam_add = 0, lam_norm = 1;
-    void* T = nullptr;             // 6 * N * N transform scratch (S)
+    void* T = nullptr;             // 6 * N * (N + 1) transform scratch (S)
     double* partial = nullptr;     // [N]
     double* p1[5] = {};            // predictor particles (nc * nc)
     double* p2[5] = {};            // corrector particles
@@ -240,7 +240,7 @@ FieldPlan plan_for(lcode_workspace* ws, int kind, int slot) {
     FieldPlan p{};
     p.kind = kind;
     p.T = static_cast<char*>(ws->T) +
-          (size_t)slot * N * N * (ws->fp32 ? sizeof(float) : sizeof(double));
+          (size_t)slot * N * (N + 1) * (ws->fp32 ? sizeof(float) : sizeof(double));
     switch (kind) {
         case F_EY:
         case F_BX:  // DST along x (rows 1..N-2), DCT along y (lcode.py:275-286)
@@ -522,7 +522,8 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     }
 
     const size_t n2 = (size_t)N * N, np = (size_t)cfg->nc * cfg->nc;
-    if ((rc = dev_alloc(ws, &ws->T, 6 * n2 * (ws->fp32 ? sizeof(float) : sizeof(double)))))
+    // six N x (N + 1) slots: the wave FFT keeps rows N + 1 long (even)
+    if ((rc = dev_alloc(ws, &ws->T, 6 * (n2 + N) * (ws->fp32 ? sizeof(float) : sizeof(double)))))
         return fail(rc);
     if ((rc = dev_alloc(ws, (void**)&ws->partial, N * sizeof(double)))) return fail(rc);
     for (int k = 0; k < 5; ++k) {

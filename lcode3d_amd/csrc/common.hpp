@@ -475,7 +475,8 @@ struct FieldPlan {
     const double *lamL, *lamK;
     double lam_add, lam_norm;
     int zero_mode;
-    void* T;                  // [lenA][rowsA] scratch spectra (S)
+    void* T;                  // scratch spectra (S): [lenA][rowsA]; in the wave
+                              // FFT [lenA][N + 1], indexed by grid row
     const double* colsum;     // Bz only: column sums of the last DCT
     double* partial;          // Bz only: [lenA] mean partials
     const double* prevF;      // Variant A / averaging source (or NULL)

@@ -980,37 +980,53 @@ guess_kernel(GuessArgs p) {
 // of six, no barrier across waves at L = 1280, a third less VALU work.
 // ---------------------------------------------------------------------------
 
-constexpr int kWR1 = 20;   // points per lane
+// LCODE_FFT_WAVE_POINTS_1280: points per lane at L = 1280, 20 (one wave per
+// line pair) or 10 (two waves: a Good-Thomas 2 x 5 DFT-10 per lane, stages
+// 4, 4, 4, 2 across 128 lanes, four waves per SIMD).  L = 2560 takes 20.
+#ifndef LCODE_FFT_WAVE_POINTS_1280
+#define LCODE_FFT_WAVE_POINTS_1280 20
+#endif
 
 template <int L> struct WaveLanes {
-    static constexpr int T = L / kWR1;
-    static_assert(T == 64 || T == 128, "wave FFT: L = 1280 or 2560");
+    static constexpr int R1 = L == 1280 ? LCODE_FFT_WAVE_POINTS_1280 : 20;   // points per lane
+    static constexpr int T = L / R1;                                          // lanes per pair
+    static_assert(R1 == 20 || R1 == 10, "wave FFT: 20 or 10 points per lane");
+    static_assert(T == 64 || T == 128, "wave FFT: one or two waves per line pair");
 };
 
-__device__ __forceinline__ int wswz(int row) { return (row >> 1) & 3; }
+// LDS address of point (row, k1) of the batched transforms: rows of R1
+// points, k1 XOR-swizzled within the row (20: by (row >> 1) & 3, 10: by
+// (row >> 2) & 1) so that the first exchange's stride-R1 stores of eight
+// consecutive lanes hit eight different 16-byte bank groups.
+template <int R1>
 __device__ __forceinline__ int waddr(int row, int k1) {
-    return row * kWR1 + (k1 ^ wswz(row));
+    const int sw = R1 == 20 ? (row >> 1) & 3 : (row >> 2) & 1;
+    return row * R1 + (k1 ^ sw);
 }
 
-// In-place DFT of 20 points: Good-Thomas with n = (5 n1 + 4 n2) mod 20,
-// k = (5 k1 + 16 k2) mod 20 (W_20^(nk) = W_4^(n1 k1) W_5^(n2 k2)).
-template <typename V>
-__device__ __forceinline__ void dft20(V (&x)[kWR1]) {
-    V a[5][4];
+// In-place DFT of R1 = N1 N2 points (gcd 1) by Good-Thomas: n = (N2 n1 + N1
+// n2) mod R1, k = (N2 (N2^-1 mod N1) k1 + N1 (N1^-1 mod N2) k2) mod R1, so
+// W_R1^(nk) = W_N1^(n1 k1) W_N2^(n2 k2) and no twiddles are needed: 20 = 4
+// x 5 (k = 5 k1 + 16 k2), 10 = 2 x 5 (k = 5 k1 + 6 k2).
+template <int R1, typename V>
+__device__ __forceinline__ void dft_lane(V (&x)[R1]) {
+    constexpr int N1 = R1 == 20 ? 4 : 2, N2 = 5;
+    constexpr int K1 = R1 == 20 ? 5 : 5, K2 = R1 == 20 ? 16 : 6;
+    V a[N2][N1];
 #pragma unroll
-    for (int n2 = 0; n2 < 5; ++n2) {
+    for (int n2 = 0; n2 < N2; ++n2) {
 #pragma unroll
-        for (int n1 = 0; n1 < 4; ++n1) a[n2][n1] = x[(5 * n1 + 4 * n2) % 20];
-        Dft<4, V>::run(a[n2]);
+        for (int n1 = 0; n1 < N1; ++n1) a[n2][n1] = x[(N2 * n1 + N1 * n2) % R1];
+        Dft<N1, V>::run(a[n2]);
     }
 #pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-        V b[5];
+    for (int k1 = 0; k1 < N1; ++k1) {
+        V b[N2];
 #pragma unroll
-        for (int n2 = 0; n2 < 5; ++n2) b[n2] = a[n2][k1];
-        Dft<5, V>::run(b);
+        for (int n2 = 0; n2 < N2; ++n2) b[n2] = a[n2][k1];
+        Dft<N2, V>::run(b);
 #pragma unroll
-        for (int k2 = 0; k2 < 5; ++k2) x[(5 * k1 + 16 * k2) % 20] = b[k2];
+        for (int k2 = 0; k2 < N2; ++k2) x[(K1 * k1 + K2 * k2) % R1] = b[k2];
     }
 }
 
@@ -1020,14 +1036,14 @@ __device__ __forceinline__ void dft20(V (&x)[kWR1]) {
 // 0 at p = 0 and L/2.
 template <int L, typename V>
 __device__ __forceinline__ void wext_load(const V* e, int xf, int lane,
-                                          V (&x)[kWR1]) {
-    constexpr int T = WaveLanes<L>::T, H = L / 2;
+                                          V (&x)[WaveLanes<L>::R1]) {
+    constexpr int T = WaveLanes<L>::T, R1 = WaveLanes<L>::R1, H = L / 2;
 #pragma unroll
-    for (int m = 0; m < kWR1; ++m) {
-        const int p = lane + T * m;        // p < H exactly for m < 10
+    for (int m = 0; m < R1; ++m) {
+        const int p = lane + T * m;        // p < H exactly for m < R1 / 2
         if (xf == XF_DCT) {
-            x[m] = e[m < kWR1 / 2 ? p : L - p];
-        } else if (m < kWR1 / 2) {
+            x[m] = e[m < R1 / 2 ? p : L - p];
+        } else if (m < R1 / 2) {
             const V v = e[max(p - 1, 0)];
             x[m] = p == 0 ? mk<V>(0, 0) : v;
         } else {
@@ -1037,25 +1053,31 @@ __device__ __forceinline__ void wext_load(const V* e, int xf, int lane,
     }
 }
 
-// One Stockham stage of radix R and span SPAN over the 20 transforms of
-// length T in z; the last one leaves output (r, q) in x[r * R + q].
+// One Stockham stage of radix R and span SPAN over the R1 transforms of
+// length T in z: lane l takes butterflies beta = l + T r, r < ceil(R1 / R),
+// of the R1 T / R (the last r partial when R does not divide R1); the last
+// stage (R dividing R1) leaves output (r, q) in x[r * R + q].
 template <int L, int R, int SPAN, bool LAST, typename V>
-__device__ __forceinline__ void wstage(V* z, V (&x)[kWR1],
+__device__ __forceinline__ void wstage(V* z, V (&x)[WaveLanes<L>::R1],
                                        const double2* __restrict__ tw, int lane) {
-    constexpr int T = WaveLanes<L>::T, NB = kWR1 / R, STRIDE = T / R;
+    constexpr int T = WaveLanes<L>::T, R1 = WaveLanes<L>::R1;
+    constexpr int NB = (R1 + R - 1) / R, NBT = R1 * T / R, STRIDE = T / R;
+    static_assert(!LAST || R1 % R == 0, "the last stage's radix divides R1");
     V v[NB][R];
     double2 w[NB];   // W_(SPAN R)^k = W_L^(k L / (SPAN R)) of each butterfly
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
-        const int beta = lane + T * r;
-        const int k1 = beta % kWR1, b = beta / kWR1, k = b % SPAN;
+        const int beta = min(lane + T * r, NBT - 1);   // a partial r repeats the last
+        const int k1 = beta % R1, b = beta / R1, k = b % SPAN;
         if constexpr (SPAN > 1) w[r] = tw[k * (L / (SPAN * R))];
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[r][q] = z[waddr(b + q * STRIDE, k1)];
+        for (int q = 0; q < R; ++q) v[r][q] = z[waddr<R1>(b + q * STRIDE, k1)];
     }
     __syncthreads();   // every read of the stage before its first write
 #pragma unroll
     for (int r = 0; r < NB; ++r) {
+        const int beta = lane + T * r;
+        if (NBT % T != 0 && beta >= NBT) continue;   // (only in a partial r)
         if constexpr (SPAN > 1) {
             // the powers formed in FP64, each rounded once to the transform's
             // precision
@@ -1071,43 +1093,45 @@ __device__ __forceinline__ void wstage(V* z, V (&x)[kWR1],
 #pragma unroll
             for (int q = 0; q < R; ++q) x[r * R + q] = v[r][q];
         } else {
-            const int beta = lane + T * r;
-            const int k1 = beta % kWR1, b = beta / kWR1, k = b % SPAN;
+            const int k1 = beta % R1, b = beta / R1, k = b % SPAN;
             const int j = (b - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) z[waddr(j + q * SPAN, k1)] = v[r][q];
+            for (int q = 0; q < R; ++q) z[waddr<R1>(j + q * SPAN, k1)] = v[r][q];
         }
     }
     if constexpr (!LAST) __syncthreads();
 }
 
 // Forward FFT of length L of the extension points in x (from wext_load).
-// z: T * 20 points of LDS, free on entry (the caller synchronised after its
+// z: T * R1 points of LDS, free on entry (the caller synchronised after its
 // last read).  On return x[r * R + q] = X[lane + T r + q L / R], R the last
-// radix (4 at L = 1280, 2 at 2560), and z is free again.
+// radix (4 for one wave per pair at L = 1280, 2 otherwise), and z is free
+// again.
 template <int L, typename V>
-__device__ __forceinline__ void wave_fft(V* z, V (&x)[kWR1],
+__device__ __forceinline__ void wave_fft(V* z, V (&x)[WaveLanes<L>::R1],
                                          const double2* __restrict__ tw, int lane) {
-    // W_L^(lane k1) = W_L^(lane 4a) W_L^(lane c) for k1 = 4a + c: seven
+    constexpr int T = WaveLanes<L>::T, R1 = WaveLanes<L>::R1;
+    // W_L^(lane k1) = W_L^(lane 4a) W_L^(lane c) for k1 = 4a + c: a few
     // table entries, one FP64 product each
-    double2 lo[4], hi[5];
+    constexpr int NA = (R1 + 3) / 4;
+    double2 lo[4], hi[NA];
 #pragma unroll
     for (int c = 1; c < 4; ++c) lo[c] = tw[lane * c];
 #pragma unroll
-    for (int a = 1; a < 5; ++a) hi[a] = tw[lane * 4 * a];
-    dft20(x);
+    for (int a = 1; a < NA; ++a) hi[a] = tw[lane * 4 * a];
+    dft_lane<R1>(x);
 #pragma unroll
-    for (int k1 = 1; k1 < kWR1; ++k1) {
+    for (int k1 = 1; k1 < R1; ++k1) {
         const int a = k1 / 4, c = k1 % 4;
         const double2 w = a == 0 ? lo[c] : c == 0 ? hi[a] : cmul(hi[a], lo[c]);
         x[k1] = cmul(x[k1], mk<V>(w.x, w.y));
     }
 #pragma unroll
-    for (int k1 = 0; k1 < kWR1; ++k1) z[waddr(lane, k1)] = x[k1];
+    for (int k1 = 0; k1 < R1; ++k1) z[waddr<R1>(lane, k1)] = x[k1];
     __syncthreads();
     wstage<L, 4, 1, false>(z, x, tw, lane);
     wstage<L, 4, 4, false>(z, x, tw, lane);
-    if constexpr (WaveLanes<L>::T == 64) {
+    if constexpr (T == 64) {
         wstage<L, 4, 16, true>(z, x, tw, lane);
     } else {
         wstage<L, 4, 16, false>(z, x, tw, lane);
@@ -1119,17 +1143,19 @@ template <int L> struct WaveOut {
     static constexpr int R = WaveLanes<L>::T == 64 ? 4 : 2;   // last radix
 };
 
+#define LCODE_WAVE_R1 WaveLanes<L>::R1
+
 // Calls f(u, k, a, b) for every output k of the type-1 transform held by
 // this lane (register u): cosine (L / 2 + 1 outputs) a = Re X[k],
 // b = Im X[k]; sine (L / 2 - 1 outputs) a = -Im X[k + 1], b = Re X[k + 1].
 // The registers with q < R / 2 hold X[idx] for idx < L / 2; X[L / 2] is
 // lane 0's register R / 2.
 template <int L, typename V, typename F>
-__device__ __forceinline__ void wext_out(const V (&x)[kWR1], int xf, int lane,
-                                         F&& f) {
+__device__ __forceinline__ void wext_out(const V (&x)[LCODE_WAVE_R1], int xf,
+                                         int lane, F&& f) {
     constexpr int T = WaveLanes<L>::T, R = WaveOut<L>::R, H = L / 2;
 #pragma unroll
-    for (int u = 0; u < kWR1; ++u) {
+    for (int u = 0; u < LCODE_WAVE_R1; ++u) {
         const int r = u / R, q = u % R;
         if (q >= R / 2) continue;
         const int idx = lane + T * r + q * (L / R);
@@ -1139,10 +1165,12 @@ __device__ __forceinline__ void wext_out(const V (&x)[kWR1], int xf, int lane,
     if (xf == XF_DCT && lane == 0) f(R / 2, H, (double)x[R / 2].x, (double)x[R / 2].y);
 }
 
-// Waves per SIMD the LDS allows (160 KiB per CU, T * 20 points per block):
-// two with FP64 points, four with FP32 (three asked for: 168 registers
-// keep the FP32 passes from spilling; the FP32 forms are not launched).
-#define LCODE_WAVE_ATTR(S) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 2 : 3)))
+// Waves per SIMD the LDS allows (160 KiB per CU, L points per block of T
+// lanes): 40 / R1 with FP64 points (two at 20 points per lane, four at 10),
+// twice that with FP32 (three asked for: 168 registers keep the FP32 passes
+// from spilling; the FP32 forms are not launched).
+#define LCODE_WAVE_ATTR(S) \
+    __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 40 / WaveLanes<L>::R1 : 3)))
 
 __device__ __forceinline__ double block_sum_waves(double v, double* scratch) {
     v = wave_sum(v);
@@ -1163,33 +1191,36 @@ pass_a_wave_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
     constexpr int T = WaveLanes<L>::T;
     constexpr int kPer = (L / 2 + 1 + T - 1) / T;   // inputs per lane
-    __shared__ V z[T * kWR1];
+    __shared__ V z[T * LCODE_WAVE_R1];
     if (!gate_open(b.gate)) return;
     const BlockPos bp = xcd_block_rows();
     const FieldPlan& fp = b.f[bp.field];
-    const int N = g.N, lines = fp.rowsA, lane = threadIdx.x;
-    const int ra = bp.block * 2, rb = ra + 1;
-    if (ra >= lines) return;
-    const bool vb = rb < lines;
+    const int N = g.N, lane = threadIdx.x;
+    // grid rows ga (even) and ga + 1; the spectra keep grid row order with
+    // rows N + 1 long, so the pair is one 16-byte element per k
+    const int ga = bp.block * 2, gb = ga + 1;
+    const int ta = ga - fp.rowsA0, tb = gb - fp.rowsA0;   // rows of the transform
+    const bool va = ta >= 0 && ta < fp.rowsA, vb = tb >= 0 && tb < fp.rowsA;
+    if (!va && !vb) return;
     {
         double a[kPer], c[kPer];
-        rhs_rows<kPer, G>(fp.kind, fp.rowsA0 + ra, fp.rowsA0 + (vb ? rb : ra), fp.colA0,
-                          fp.lenA, lane, T, N, b.in, a, c);
+        const int gc = va ? ga : gb;   // rows without data read a valid one
+        rhs_rows<kPer, G>(fp.kind, gc, vb ? gb : gc, fp.colA0, fp.lenA, lane, T, N, b.in, a, c);
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int n = lane + u * T;
-            if (n < fp.lenA) z[n] = mk<V>((S)a[u], vb ? (S)c[u] : (S)0);
+            if (n < fp.lenA) z[n] = mk<V>(va ? (S)a[u] : (S)0, vb ? (S)c[u] : (S)0);
         }
     }
     __syncthreads();
-    V x[kWR1];
+    V x[LCODE_WAVE_R1];
     wext_load<L>(z, fp.xfA, lane, x);
     __syncthreads();
     wave_fft<L>(z, x, g.wave_tw, lane);
-    S* Tsp = static_cast<S*>(fp.T);
+    V* Tsp = static_cast<V*>(fp.T);
+    const int rowp = (N + 1) / 2;   // pairs per row of the spectra
     wext_out<L>(x, fp.xfA, lane, [&](int, int k, double a, double c) {
-        Tsp[(size_t)k * lines + ra] = (S)a;
-        if (vb) Tsp[(size_t)k * lines + rb] = (S)c;
+        Tsp[(size_t)k * rowp + ga / 2] = mk<V>((S)a, (S)c);
     });
 }
 
@@ -1199,7 +1230,7 @@ pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
     constexpr int T = WaveLanes<L>::T;
     constexpr int kPer = (L / 2 + 1 + T - 1) / T;
-    __shared__ V z[T * kWR1];
+    __shared__ V z[T * LCODE_WAVE_R1];
     if (!gate_open(b.gate)) return;
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
@@ -1207,8 +1238,11 @@ pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
     const int la = bp.block * 2, lb = la + 1;
     if (la >= lines) return;
     const bool vb = lb < lines;
-    S* rowa = static_cast<S*>(fp.T) + (size_t)la * len;
-    S* rowb = static_cast<S*>(fp.T) + (size_t)(vb ? lb : la) * len;
+    // spectra rows N + 1 long in grid row order: line k's transform inputs
+    // start at grid row rowsA0
+    const size_t stride = (size_t)g.N + 1;
+    S* rowa = static_cast<S*>(fp.T) + (size_t)la * stride + fp.rowsA0;
+    S* rowb = static_cast<S*>(fp.T) + (size_t)(vb ? lb : la) * stride + fp.rowsA0;
     {
         S xa[kPer], xb[kPer];
 #pragma unroll
@@ -1224,7 +1258,7 @@ pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
         }
     }
     __syncthreads();
-    V x[kWR1];
+    V x[LCODE_WAVE_R1];
     wext_load<L>(z, xf, lane, x);
     __syncthreads();
     wave_fft<L>(z, x, g.wave_tw, lane);
@@ -1233,7 +1267,7 @@ pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
         const S* tab = static_cast<const S*>(fp.table);
         const S* ta = tab + (size_t)la * len;
         const S* tb = tab + (size_t)(vb ? lb : la) * len;
-        S ma[kWR1], mb[kWR1];
+        S ma[LCODE_WAVE_R1], mb[LCODE_WAVE_R1];
         wext_out<L>(x, xf, lane, [&](int u, int k, double, double) {
             ma[u] = ta[k];
             mb[u] = tb[k];
@@ -1271,7 +1305,7 @@ pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
     constexpr int T = WaveLanes<L>::T;
     constexpr int kPer = (L / 2 + 1 + T - 1) / T;
-    __shared__ V z[T * kWR1];
+    __shared__ V z[T * LCODE_WAVE_R1];
     if (!gate_open(b.gate)) return;
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
@@ -1281,24 +1315,23 @@ pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
     const bool ha = ca >= 0 && ca < fp.rowsA;
     const bool hb = rb < N && cb >= 0 && cb < fp.rowsA;
     {
-        const S* Tsp = static_cast<const S*>(fp.T);
-        const int stride = fp.rowsA;
-        const int cac = ha ? ca : 0, cbc = hb ? cb : 0;
-        S xa[kPer], xb[kPer];
+        // the pair of grid rows (ra even) is one 16-byte element per k of the
+        // spectra (rows N + 1 long, grid row order); rows without data hold
+        // stale values and are dropped
+        const V* Tsp = static_cast<const V*>(fp.T);
+        const int rowp = (N + 1) / 2;
+        V xv[kPer];
 #pragma unroll
-        for (int u = 0; u < kPer; ++u) {
-            const S* col = Tsp + (size_t)min(lane + u * T, fp.lenA - 1) * stride;
-            xa[u] = col[cac];
-            xb[u] = col[cbc];
-        }
+        for (int u = 0; u < kPer; ++u)
+            xv[u] = Tsp[(size_t)min(lane + u * T, fp.lenA - 1) * rowp + ra / 2];
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int n = lane + u * T;
-            if (n < fp.lenA) z[n] = mk<V>(ha ? xa[u] : (S)0, hb ? xb[u] : (S)0);
+            if (n < fp.lenA) z[n] = mk<V>(ha ? xv[u].x : (S)0, hb ? xv[u].y : (S)0);
         }
     }
     __syncthreads();
-    V x[kWR1];
+    V x[LCODE_WAVE_R1];
     wext_load<L>(z, fp.xfA, lane, x);
     __syncthreads();
     wave_fft<L>(z, x, g.wave_tw, lane);
@@ -1352,7 +1385,7 @@ pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
         if (!hb) vb = 0.;
     };
     {
-        constexpr int kOut = kWR1 / 2;   // registers with q < R / 2
+        constexpr int kOut = LCODE_WAVE_R1 / 2;   // registers with q < R / 2
         G pa[kOut], pb[kOut], qa[kOut], qb[kOut];
 #pragma unroll
         for (int s = 0; s < kOut; ++s) {
@@ -1469,11 +1502,12 @@ bool solver_static_length(int L) {
 
 template <int L, typename S, typename G>
 static int launch_wave_passes(const SolverGeom& g, const SolveBatch& b,
-                              hipStream_t s, Profiler* prof, int max_a, int max_b) {
+                              hipStream_t s, Profiler* prof, int max_b) {
     const dim3 block(WaveLanes<L>::T);
     {
         ProfScope ps(prof, "solve_pass_a", s, true);
-        launch_timed(ps, pass_a_wave_kernel<L, S, G>, dim3((max_a + 1) / 2, b.nfields),
+        // one block per pair of grid rows
+        launch_timed(ps, pass_a_wave_kernel<L, S, G>, dim3((g.N + 1) / 2, b.nfields),
                      block, 0, s, g, b);
     }
     LCODE_HIP(hipGetLastError());
@@ -1528,8 +1562,8 @@ static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
                 return LCODE_E_INVALID;
             }
         }
-        if (g.fft.L == 1280) return launch_wave_passes<1280, S, G>(g, b, s, prof, max_a, max_b);
-        if (g.fft.L == 2560) return launch_wave_passes<2560, S, G>(g, b, s, prof, max_a, max_b);
+        if (g.fft.L == 1280) return launch_wave_passes<1280, S, G>(g, b, s, prof, max_b);
+        if (g.fft.L == 2560) return launch_wave_passes<2560, S, G>(g, b, s, prof, max_b);
     }
     const dim3 ga((max_a + per_block - 1) / per_block, b.nfields);
     const dim3 gb((max_b + per_block - 1) / per_block, b.nfields);
