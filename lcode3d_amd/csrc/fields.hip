@@ -1224,6 +1224,12 @@ pass_a_wave_kernel(SolverGeom g, SolveBatch b) {
     });
 }
 
+// LCODE_WAVE_TABLE_INLINE (0): the wave pass B evaluates its spectral
+// multipliers instead of reading the table.
+#ifndef LCODE_WAVE_TABLE_INLINE
+#define LCODE_WAVE_TABLE_INLINE 0
+#endif
+
 template <int L, typename S>
 __global__ void __launch_bounds__(WaveLanes<L>::T) LCODE_WAVE_ATTR(S)
 pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
@@ -1263,15 +1269,27 @@ pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
     __syncthreads();
     wave_fft<L>(z, x, g.wave_tw, lane);
     {
-        // the spectral multipliers of this lane's outputs, loaded first
-        const S* tab = static_cast<const S*>(fp.table);
-        const S* ta = tab + (size_t)la * len;
-        const S* tb = tab + (size_t)(vb ? lb : la) * len;
+        // the spectral multipliers of this lane's outputs, loaded first (or,
+        // LCODE_WAVE_TABLE_INLINE, evaluated from the eigenvalues as the
+        // host builds the table: the same correctly rounded divisions)
         S ma[LCODE_WAVE_R1], mb[LCODE_WAVE_R1];
-        wext_out<L>(x, xf, lane, [&](int u, int k, double, double) {
-            ma[u] = ta[k];
-            mb[u] = tb[k];
-        });
+        if (LCODE_WAVE_TABLE_INLINE) {
+            const double lla = fp.lamL[la], llb = fp.lamL[vb ? lb : la];
+            wext_out<L>(x, xf, lane, [&](int u, int k, double, double) {
+                const double lk = fp.lamK[k];
+                const double va = 1 / (lla + lk + fp.lam_add) / fp.lam_norm;
+                ma[u] = (S)(fp.zero_mode && la == 0 && k == 0 ? 0. : va);
+                mb[u] = (S)(1 / (llb + lk + fp.lam_add) / fp.lam_norm);
+            });
+        } else {
+            const S* tab = static_cast<const S*>(fp.table);
+            const S* ta = tab + (size_t)la * len;
+            const S* tb = tab + (size_t)(vb ? lb : la) * len;
+            wext_out<L>(x, xf, lane, [&](int u, int k, double, double) {
+                ma[u] = ta[k];
+                mb[u] = tb[k];
+            });
+        }
         wext_out<L>(x, xf, lane, [&](int u, int k, double a, double c) {
             z[k] = mk<V>((S)a * ma[u], vb ? (S)c * mb[u] : (S)0);
         });
