@@ -124,16 +124,60 @@ constexpr int kDepositHalo = 6;
 #define LCODE_DEP_REGIONS 1
 #endif
 constexpr bool kDepRegions = LCODE_DEP_REGIONS;
-// 1: deposition blocks take the tiles in order of Chebyshev distance from
-// the grid centre (the wake of an axial beam makes those the slowest, so
-// they would start first when there are more tiles than block slots); 0:
-// row-major (build option).  Measured at 641^2 in the developed wake: no
+// Order of the deposition tiles over the blocks (build option).  0:
+// row-major.  1: by Chebyshev distance from the grid centre (the wake of an
+// axial beam makes those the slowest, so they would start first when there
+// are more tiles than block slots).  Measured at 641^2 in the developed wake: no
 // change with 24-cell tiles (43.03 us either way: every tile is resident
-// at once), slower with 20-cell ones (51.2 against 46.9 us); off.
+// at once), slower with 20-cell ones (51.2 against 46.9 us).
+// 2: XCD clusters: workgroups go to the 8 XCDs round-robin by linear id and
+// each XCD has its own L2, so row-major tiles spread every tile's
+// neighbours (which share its staged coarse particles, fine-index tables
+// and the 128-byte lines at its edges) over all eight; here the tiles come
+// in 3 x 3 clusters (LCODE_DEP_CLUSTER), each cluster on one XCD, the clusters dealt to the
+// XCDs round-robin in row-major order (adjacent clusters on different
+// XCDs, so the wake's slow central tiles still spread over them).
+// Measured at 641^2 (24-cell tiles, developed wake, A/B on one box): order 2
+// took the deposition from 41.6 to 41.0 us and its HBM traffic from 31.2 to
+// 22.7 MB per launch (reads 16.4 -> 9.0 MB, writes 14.8 -> 13.7 MB); 4 x 4
+// and 5 x 5 clusters 41.1-41.3 and 41.5 us at about the same traffic.
+// Default 2.
 #ifndef LCODE_DEP_ORDER
-#define LCODE_DEP_ORDER 0
+#define LCODE_DEP_ORDER 2
 #endif
-constexpr bool kDepOrder = LCODE_DEP_ORDER;
+constexpr int kDepOrder = LCODE_DEP_ORDER;
+
+// The XCD-cluster tile order: order[lin] = tile for linear block id lin
+// (XCD lin % 8), a permutation of the nt * nt tiles.
+#ifndef LCODE_DEP_CLUSTER
+#define LCODE_DEP_CLUSTER 3
+#endif
+std::vector<int> xcd_cluster_order(int nt) {
+    constexpr int kXcd = 8, kC = LCODE_DEP_CLUSTER;
+    const int nc = (nt + kC - 1) / kC;
+    std::vector<std::vector<int>> per(kXcd);
+    for (int cy = 0, j = 0; cy < nc; ++cy)
+        for (int cx = 0; cx < nc; ++cx, ++j)
+            for (int dy = 0; dy < kC; ++dy)
+                for (int dx = 0; dx < kC; ++dx) {
+                    const int ty = cy * kC + dy, tx = cx * kC + dx;
+                    if (ty < nt && tx < nt) per[j % kXcd].push_back(ty * nt + tx);
+                }
+    const int n = nt * nt;
+    std::vector<int> order(n, -1), spare;
+    std::vector<size_t> used(kXcd, 0);
+    for (int lin = 0; lin < n; ++lin) {   // each XCD's own clusters first
+        auto& v = per[lin % kXcd];
+        size_t& u = used[lin % kXcd];
+        if (u < v.size()) order[lin] = v[u++];
+    }
+    for (int x = 0; x < kXcd; ++x)        // what did not fit, to the free slots
+        for (size_t u = used[x]; u < per[x].size(); ++u) spare.push_back(per[x][u]);
+    size_t k = 0;
+    for (int lin = 0; lin < n; ++lin)
+        if (order[lin] < 0) order[lin] = spare[k++];
+    return order;
+}
 
 // Round-to-nearest-even BF16 bits of a float (finite inputs).
 uint16_t bf16_bits(float f) {
@@ -567,12 +611,16 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     if (kDepOrder) {
         std::vector<int> order((size_t)nt * nt);
         for (int k = 0; k < nt * nt; ++k) order[k] = k;
-        const double mid = (nt - 1) / 2.;
-        auto dist = [&](int k) {
-            return std::max(std::fabs(k / nt - mid), std::fabs(k % nt - mid));
-        };
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int u, int v) { return dist(u) < dist(v); });
+        if (kDepOrder == 2) {
+            order = xcd_cluster_order(nt);
+        } else {
+            const double mid = (nt - 1) / 2.;
+            auto dist = [&](int k) {
+                return std::max(std::fabs(k / nt - mid), std::fabs(k % nt - mid));
+            };
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int u, int v) { return dist(u) < dist(v); });
+        }
         if ((rc = dev_alloc(ws, (void**)&ws->tile_order, order.size() * sizeof(int))) ||
             (rc = hipMemcpy(ws->tile_order, order.data(), order.size() * sizeof(int),
                             hipMemcpyHostToDevice) == hipSuccess ? LCODE_OK : LCODE_E_HIP)) {
