@@ -75,13 +75,16 @@ constexpr int kRegionRowMax = 128;
 #define LCODE_PUSH_ATTR
 #endif
 
-// One coarse particle's push; returns max(|x_offt|, |y_offt|) after it.
+// One coarse particle's push; returns max(|x_offt|, |y_offt|) after it and
+// the particle's initial position in xi0, yi0.
 // G: storage type of the field grids (the momentum update is FP64 in both).
 template <typename G>
-__device__ __forceinline__ double push_one(const PushArgs& a, int k) {
+__device__ __forceinline__ double push_one(const PushArgs& a, int k, double& xi0,
+                                           double& yi0) {
     const double m = a.m[k], q = a.q[k];
     const double opx = a.px[k], opy = a.py[k], opz = a.pz[k];
-    const double xi0 = a.x_init[k], yi0 = a.y_init[k];
+    xi0 = a.x_init[k];
+    yi0 = a.y_init[k];
     double x_offt = a.prev_x_offt[k], y_offt = a.prev_y_offt[k];
     double ex, ey;
     if (a.est_x) {
@@ -145,8 +148,12 @@ template <typename G>
 __global__ void __launch_bounds__(kPushBlock) LCODE_PUSH_ATTR
 push_kernel(PushArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double d = 0.;
-    if (k < a.n && gate_open(a.gate)) d = push_one<G>(a, k);
+    const int k0 = blockIdx.x * blockDim.x;
+    // the block's first initial row, for the per-tile bound table: loaded
+    // with the particle data, not after the stores (which may alias it)
+    const double x_first = a.region_max ? a.x_init[k0] : 0.;
+    double d = 0., xi0 = 0., yi0 = 0.;
+    if (k < a.n && gate_open(a.gate)) d = push_one<G>(a, k, xi0, yi0);
     if (a.max_cells) {
         // cells the particles moved from their initial positions, at most
         // (the deposition sizes its tile halo by it); a non-finite state,
@@ -164,13 +171,12 @@ push_kernel(PushArgs a) {
             // tiles instead cost the push ~3 us of dependent latency
             __shared__ int rtab[2 * kRegionRowMax];
             const int nr = a.region_n, rn = nr - 1;
-            const int k0 = blockIdx.x * blockDim.x;
-            const int ri0 = min(max(cell_of(a.x_init[k0], a.N, a.h) / a.region_t, 0), rn);
+            const int ri0 = min(max(cell_of(x_first, a.N, a.h) / a.region_t, 0), rn);
             for (int e = threadIdx.x; e < 2 * kRegionRowMax; e += blockDim.x) rtab[e] = 0;
             __syncthreads();
-            if (cells > 0) {
-                const int ri = min(max(cell_of(a.x_init[k], a.N, a.h) / a.region_t, 0), rn);
-                const int rj = min(max(cell_of(a.y_init[k], a.N, a.h) / a.region_t, 0), rn);
+            if (cells > 0) {   // (a particle that moved exists: k < n)
+                const int ri = min(max(cell_of(xi0, a.N, a.h) / a.region_t, 0), rn);
+                const int rj = min(max(cell_of(yi0, a.N, a.h) / a.region_t, 0), rn);
                 const int dr = ri - ri0;
                 if (dr >= 0 && dr < 2 && nr <= kRegionRowMax)
                     atomicMax(&rtab[dr * kRegionRowMax + rj], cells);
