@@ -1145,24 +1145,34 @@ template <int L> struct WaveOut {
 
 #define LCODE_WAVE_R1 WaveLanes<L>::R1
 
-// Calls f(u, k, a, b) for every output k of the type-1 transform held by
-// this lane (register u): cosine (L / 2 + 1 outputs) a = Re X[k],
-// b = Im X[k]; sine (L / 2 - 1 outputs) a = -Im X[k + 1], b = Re X[k + 1].
-// The registers with q < R / 2 hold X[idx] for idx < L / 2; X[L / 2] is
-// lane 0's register R / 2.
-template <int L, typename V, typename F>
-__device__ __forceinline__ void wext_out(const V (&x)[LCODE_WAVE_R1], int xf,
-                                         int lane, F&& f) {
+// The outputs of the type-1 transform held by this lane: f(u, k) for each
+// (register u, output k).
+template <int L, typename F>
+__device__ __forceinline__ void wext_index(int xf, int lane, F&& f) {
     constexpr int T = WaveLanes<L>::T, R = WaveOut<L>::R, H = L / 2;
 #pragma unroll
     for (int u = 0; u < LCODE_WAVE_R1; ++u) {
         const int r = u / R, q = u % R;
         if (q >= R / 2) continue;
         const int idx = lane + T * r + q * (L / R);
-        if (xf == XF_DCT) f(u, idx, (double)x[u].x, (double)x[u].y);
-        else if (idx >= 1) f(u, idx - 1, -(double)x[u].y, (double)x[u].x);
+        if (xf == XF_DCT) f(u, idx);
+        else if (idx >= 1) f(u, idx - 1);
     }
-    if (xf == XF_DCT && lane == 0) f(R / 2, H, (double)x[R / 2].x, (double)x[R / 2].y);
+    if (xf == XF_DCT && lane == 0) f(R / 2, H);
+}
+
+// Calls f(u, k, a, b) for every output k of the type-1 transform held by
+// this lane (register u; wext_index gives the (u, k) pairs alone): cosine
+// (L / 2 + 1 outputs) a = Re X[k], b = Im X[k]; sine (L / 2 - 1 outputs)
+// a = -Im X[k + 1], b = Re X[k + 1].  The registers with q < R / 2 hold
+// X[idx] for idx < L / 2; X[L / 2] is lane 0's register R / 2.
+template <int L, typename V, typename F>
+__device__ __forceinline__ void wext_out(const V (&x)[LCODE_WAVE_R1], int xf,
+                                         int lane, F&& f) {
+    wext_index<L>(xf, lane, [&](int u, int k) {
+        if (xf == XF_DCT) f(u, k, (double)x[u].x, (double)x[u].y);
+        else f(u, k, -(double)x[u].y, (double)x[u].x);
+    });
 }
 
 // Waves per SIMD the LDS allows (160 KiB per CU, L points per block of T
@@ -1225,9 +1235,13 @@ pass_a_wave_kernel(SolverGeom g, SolveBatch b) {
 }
 
 // LCODE_WAVE_TABLE_INLINE (0): the wave pass B evaluates its spectral
-// multipliers instead of reading the table.
+// multipliers instead of reading the table.  LCODE_WAVE_TABLE_EARLY: it
+// loads them before its first transform.
 #ifndef LCODE_WAVE_TABLE_INLINE
 #define LCODE_WAVE_TABLE_INLINE 0
+#endif
+#ifndef LCODE_WAVE_TABLE_EARLY
+#define LCODE_WAVE_TABLE_EARLY 0
 #endif
 
 template <int L, typename S>
@@ -1263,19 +1277,16 @@ pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
             if (n < len) z[n] = mk<V>(xa[u], vb ? xb[u] : (S)0);
         }
     }
-    __syncthreads();
-    V x[LCODE_WAVE_R1];
-    wext_load<L>(z, xf, lane, x);
-    __syncthreads();
-    wave_fft<L>(z, x, g.wave_tw, lane);
-    {
-        // the spectral multipliers of this lane's outputs, loaded first (or,
-        // LCODE_WAVE_TABLE_INLINE, evaluated from the eigenvalues as the
-        // host builds the table: the same correctly rounded divisions)
-        S ma[LCODE_WAVE_R1], mb[LCODE_WAVE_R1];
+    // the spectral multipliers of this lane's outputs (or,
+    // LCODE_WAVE_TABLE_INLINE, evaluated from the eigenvalues as the host
+    // builds the table: the same correctly rounded divisions); with
+    // LCODE_WAVE_TABLE_EARLY their loads are in flight during the first
+    // transform
+    S ma[LCODE_WAVE_R1], mb[LCODE_WAVE_R1];
+    auto multipliers = [&]() {
         if (LCODE_WAVE_TABLE_INLINE) {
             const double lla = fp.lamL[la], llb = fp.lamL[vb ? lb : la];
-            wext_out<L>(x, xf, lane, [&](int u, int k, double, double) {
+            wext_index<L>(xf, lane, [&](int u, int k) {
                 const double lk = fp.lamK[k];
                 const double va = 1 / (lla + lk + fp.lam_add) / fp.lam_norm;
                 ma[u] = (S)(fp.zero_mode && la == 0 && k == 0 ? 0. : va);
@@ -1285,11 +1296,20 @@ pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
             const S* tab = static_cast<const S*>(fp.table);
             const S* ta = tab + (size_t)la * len;
             const S* tb = tab + (size_t)(vb ? lb : la) * len;
-            wext_out<L>(x, xf, lane, [&](int u, int k, double, double) {
+            wext_index<L>(xf, lane, [&](int u, int k) {
                 ma[u] = ta[k];
                 mb[u] = tb[k];
             });
         }
+    };
+    if (LCODE_WAVE_TABLE_EARLY) multipliers();
+    __syncthreads();
+    V x[LCODE_WAVE_R1];
+    wext_load<L>(z, xf, lane, x);
+    __syncthreads();
+    wave_fft<L>(z, x, g.wave_tw, lane);
+    {
+        if (!LCODE_WAVE_TABLE_EARLY) multipliers();
         wext_out<L>(x, xf, lane, [&](int u, int k, double a, double c) {
             z[k] = mk<V>((S)a * ma[u], vb ? (S)c * mb[u] : (S)0);
         });
