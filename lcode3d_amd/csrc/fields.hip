@@ -1179,6 +1179,20 @@ __device__ __forceinline__ void wext_out(const V (&x)[LCODE_WAVE_R1], int xf,
 // lanes): 40 / R1 with FP64 points (two at 20 points per lane, four at 10),
 // twice that with FP32 (three asked for: 168 registers keep the FP32 passes
 // from spilling; the FP32 forms are not launched).
+// LCODE_FFT_WAVE_PAIRS (4): line pairs per block of the wave pass C: a block
+// reads the spectra of P consecutive row pairs together, P 16-byte elements
+// per k in one wave-wide access (pass C 24.2 -> 22.6 us at 641^2 with 4; the
+// same for pass A's stores cost it 26.4 -> 30.0 us: the block barriers of the
+// transform tie its waves' long right-hand-side loads together).
+// At L = 2560 one pair per block: four would take all of a CU's LDS (the
+// 1281^2 step 717 -> 712 xi-steps/s).
+#ifndef LCODE_FFT_WAVE_PAIRS
+#define LCODE_FFT_WAVE_PAIRS 4
+#endif
+template <int L> struct WavePairs {
+    static constexpr int P = L == 1280 ? LCODE_FFT_WAVE_PAIRS : 1;
+};
+
 #define LCODE_WAVE_ATTR(S) \
     __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 40 / WaveLanes<L>::R1 : 3)))
 
@@ -1338,34 +1352,44 @@ pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
 }
 
 template <int L, typename S, typename G>
-__global__ void __launch_bounds__(WaveLanes<L>::T) LCODE_WAVE_ATTR(S)
+__global__ void __launch_bounds__(WaveLanes<L>::T * WavePairs<L>::P) LCODE_WAVE_ATTR(S)
 pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
-    constexpr int T = WaveLanes<L>::T;
+    constexpr int T = WaveLanes<L>::T, P = WavePairs<L>::P, NT = T * P;
     constexpr int kPer = (L / 2 + 1 + T - 1) / T;
-    __shared__ V z[T * LCODE_WAVE_R1];
+    constexpr int kZ = T * LCODE_WAVE_R1;   // LDS points per line pair
+    __shared__ V zall[P * kZ];
     if (!gate_open(b.gate)) return;
     const BlockPos bp = xcd_block();
     const FieldPlan& fp = b.f[bp.field];
-    const int N = g.N, lane = threadIdx.x;
-    const int ra = bp.block * 2, rb = ra + 1;   // ra < N: the grid has ceil(N / 2) blocks
-    const int ca = ra - fp.rowsA0, cb = rb - fp.rowsA0;
-    const bool ha = ca >= 0 && ca < fp.rowsA;
-    const bool hb = rb < N && cb >= 0 && cb < fp.rowsA;
+    const int N = g.N, w = threadIdx.x / T, lane = threadIdx.x % T;
+    V* z = zall + w * kZ;
+    // this wave's pair of output rows; a pair beyond the grid (the last
+    // block) runs through the barriers and emits nothing
+    const int ra = (bp.block * P + w) * 2, rb = ra + 1;
+    const bool live = ra < N;
+    auto has = [&](int r) { return r < N && r - fp.rowsA0 >= 0 && r - fp.rowsA0 < fp.rowsA; };
+    const bool ha = has(ra), hb = has(rb);
     {
-        // the pair of grid rows (ra even) is one 16-byte element per k of the
-        // spectra (rows N + 1 long, grid row order); rows without data hold
-        // stale values and are dropped
+        // the block's P pairs of grid rows are P consecutive 16-byte
+        // elements per k of the spectra (rows N + 1 long, grid row order):
+        // the block reads them together, 16 k per wave-wide load, into each
+        // pair's LDS; rows without data hold stale values and are dropped
         const V* Tsp = static_cast<const V*>(fp.T);
         const int rowp = (N + 1) / 2;
         V xv[kPer];
 #pragma unroll
-        for (int u = 0; u < kPer; ++u)
-            xv[u] = Tsp[(size_t)min(lane + u * T, fp.lenA - 1) * rowp + ra / 2];
+        for (int u = 0; u < kPer; ++u) {
+            const int idx = threadIdx.x + u * NT;
+            const int k = min(idx / P, fp.lenA - 1), pr = bp.block * P + idx % P;
+            xv[u] = Tsp[(size_t)k * rowp + min(pr, rowp - 1)];
+        }
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
-            const int n = lane + u * T;
-            if (n < fp.lenA) z[n] = mk<V>(ha ? xv[u].x : (S)0, hb ? xv[u].y : (S)0);
+            const int idx = threadIdx.x + u * NT;
+            const int k = idx / P, p = idx % P, r0 = (bp.block * P + p) * 2;
+            if (k < fp.lenA && r0 < N)
+                zall[p * kZ + k] = mk<V>(has(r0) ? xv[u].x : (S)0, has(r0 + 1) ? xv[u].y : (S)0);
         }
     }
     __syncthreads();
@@ -1377,8 +1401,8 @@ pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
     double mean = 0.;
     if (fp.partial) {
         double s = 0.;
-        for (int k = lane; k < fp.lenA; k += T) s += fp.partial[k];
-        mean = block_sum_waves(s, reinterpret_cast<double*>(z)) / ((double)N * N);
+        for (int k = threadIdx.x; k < fp.lenA; k += NT) s += fp.partial[k];
+        mean = block_sum_waves(s, reinterpret_cast<double*>(zall)) / ((double)N * N);
     }
     const bool transverse = fp.kind == F_EX || fp.kind == F_EY ||
                             fp.kind == F_BX || fp.kind == F_BY;
@@ -1422,7 +1446,7 @@ pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
         if (!ha) va = 0.;
         if (!hb) vb = 0.;
     };
-    {
+    if (live) {
         constexpr int kOut = LCODE_WAVE_R1 / 2;   // registers with q < R / 2
         G pa[kOut], pb[kOut], qa[kOut], qb[kOut];
 #pragma unroll
@@ -1450,7 +1474,7 @@ pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
             if (rb < N) emit(rb, j, vb, pb[s], qb[s]);
         }
     }
-    if (lane == 0) {
+    if (live && lane == 0) {
         double va, vb;
         value(R / 2, H, va, vb);
         emit(ra, H, va, prevF ? (double)prevF[oa + H] : 0., guess ? (double)guess[oa + H] : 0.);
@@ -1458,10 +1482,10 @@ pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
             emit(rb, H, vb, prevF ? (double)prevF[ob + H] : 0., guess ? (double)guess[ob + H] : 0.);
     }
     if (b.resid) {
-        double* scratch = reinterpret_cast<double*>(z);
+        double* scratch = reinterpret_cast<double*>(zall);
         r_num = block_sum_waves(r_num, scratch);
         r_den = block_sum_waves(r_den, scratch);
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
             const int lin = blockIdx.y * gridDim.x + blockIdx.x;
             b.resid_part[2 * lin] = r_num;
             b.resid_part[2 * lin + 1] = r_den;
@@ -1542,6 +1566,8 @@ template <int L, typename S, typename G>
 static int launch_wave_passes(const SolverGeom& g, const SolveBatch& b,
                               hipStream_t s, Profiler* prof, int max_b) {
     const dim3 block(WaveLanes<L>::T);
+    constexpr int PC = WavePairs<L>::P;
+    const int nblk_c = ((g.N + 1) / 2 + PC - 1) / PC;
     {
         ProfScope ps(prof, "solve_pass_a", s, true);
         // one block per pair of grid rows
@@ -1555,10 +1581,11 @@ static int launch_wave_passes(const SolverGeom& g, const SolveBatch& b,
                      block, 0, s, g, b);
     }
     LCODE_HIP(hipGetLastError());
-    const dim3 gc((g.N + 1) / 2, b.nfields);
+    const dim3 gc(nblk_c, b.nfields);
     {
         ProfScope ps(prof, "solve_pass_c", s, true);
-        launch_timed(ps, pass_c_wave_kernel<L, S, G>, gc, block, 0, s, g, b);
+        launch_timed(ps, pass_c_wave_kernel<L, S, G>, gc, dim3(WaveLanes<L>::T * PC), 0, s,
+                     g, b);
     }
     LCODE_HIP(hipGetLastError());
     if (b.resid) {
