@@ -1358,6 +1358,10 @@ pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
     constexpr int T = WaveLanes<L>::T, P = WavePairs<L>::P, NT = T * P;
     constexpr int kPer = (L / 2 + 1 + T - 1) / T;
     constexpr int kZ = T * LCODE_WAVE_R1;   // LDS points per line pair
+    // pair p's staged inputs start kStage p points into its LDS (64 bytes
+    // apart: the four lanes storing one k hit different banks)
+    constexpr int kStage = P > 1 ? 4 : 0;
+    static_assert(L / 2 + 1 + kStage * (P - 1) <= kZ, "staged inputs fit the pair's LDS");
     __shared__ V zall[P * kZ];
     if (!gate_open(b.gate)) return;
     const BlockPos bp = xcd_block();
@@ -1389,12 +1393,13 @@ pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
             const int idx = threadIdx.x + u * NT;
             const int k = idx / P, p = idx % P, r0 = (bp.block * P + p) * 2;
             if (k < fp.lenA && r0 < N)
-                zall[p * kZ + k] = mk<V>(has(r0) ? xv[u].x : (S)0, has(r0 + 1) ? xv[u].y : (S)0);
+                zall[p * kZ + kStage * p + k] =
+                    mk<V>(has(r0) ? xv[u].x : (S)0, has(r0 + 1) ? xv[u].y : (S)0);
         }
     }
     __syncthreads();
     V x[LCODE_WAVE_R1];
-    wext_load<L>(z, fp.xfA, lane, x);
+    wext_load<L>(z + kStage * w, fp.xfA, lane, x);
     __syncthreads();
     wave_fft<L>(z, x, g.wave_tw, lane);
 
