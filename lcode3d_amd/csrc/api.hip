@@ -427,7 +427,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
     }
 
     g.wave_tw = nullptr;
-    if (solver_wave_length(L) && !ws->fp32) {   // FP64 transforms only
+    if (solver_wave_length(L) && (!ws->fp32 || (LCODE_FFT_WAVE_FP32 && L == 1280))) {
         // W_L^t = exp(-2 pi i t / L) for every t < L
         std::vector<double2> wt(L);
         for (int t = 0; t < L; ++t) {

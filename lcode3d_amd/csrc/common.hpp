@@ -595,6 +595,13 @@ int launch_solve(const SolverGeom& g, const SolveBatch& b, hipStream_t s,
                  Profiler* prof);
 bool solver_static_length(int L);
 bool solver_wave_length(int L);   // L has the wave FFT passes
+// LCODE_FFT_WAVE_FP32 (1): the FP32 transforms (FP32 and mixed solvers) of
+// L = 1280 run the wave FFT too (mixed six-field solve at 641^2 67.5 ->
+// 61.9 us); 0 keeps them on the radix-4/5 plan, as at L = 2560 always (there
+// the FP32 wave passes spill: the 1281^2 mixed step 871 -> 826 xi-steps/s).
+#ifndef LCODE_FFT_WAVE_FP32
+#define LCODE_FFT_WAVE_FP32 1
+#endif
 int solver_padded_len(int L);
 int solver_tw_count(int L);   // static plan: twiddle table entries
 int solver_threads_per_pair(int L, bool fp32, bool static_plan);   // threads of one line pair

@@ -1177,8 +1177,8 @@ __device__ __forceinline__ void wext_out(const V (&x)[LCODE_WAVE_R1], int xf,
 
 // Waves per SIMD the LDS allows (160 KiB per CU, L points per block of T
 // lanes): 40 / R1 with FP64 points (two at 20 points per lane, four at 10),
-// twice that with FP32 (three asked for: 168 registers keep the FP32 passes
-// from spilling; the FP32 forms are not launched).
+// twice that with FP32 (three asked for: 168 registers, where the FP32
+// passes of L = 1280 spill at most 6; those of L = 2560 are not launched).
 // LCODE_FFT_WAVE_PAIRS (4): line pairs per block of the wave pass C: a block
 // reads the spectra of P consecutive row pairs together, P 16-byte elements
 // per k in one wave-wide access (pass C 24.2 -> 22.6 us at 641^2 with 4; the
@@ -1615,11 +1615,9 @@ static int launch_solve_t(const SolverGeom& g, const SolveBatch& b,
         max_a = max_a > b.f[f].rowsA ? max_a : b.f[f].rowsA;
         max_b = max_b > b.f[f].lenA ? max_b : b.f[f].lenA;
     }
-    // the wave FFT runs the FP64 transforms only: in FP32 the radix-4/5 plan,
-    // at three waves per line pair with the twiddles in LDS, measured faster
-    // (mixed six-field solve at 641^2: 67.8 against 71.0 us; at 1281^2 the
-    // mixed step 846 against 798 xi-steps/s)
-    if constexpr (std::is_same<S, double>::value) if (g.wave_tw) {
+    // the wave FFT runs the FP64 transforms and, with LCODE_FFT_WAVE_FP32, the
+    // FP32 ones of L = 1280 (the workspace sets wave_tw only then)
+    if constexpr (std::is_same<S, double>::value || LCODE_FFT_WAVE_FP32) if (g.wave_tw) {
         // the wave passes take a cosine transform of N points from column
         // (or line) 0 and a sine transform of N - 2 from 1
         for (int f = 0; f < b.nfields; ++f) {
