@@ -963,8 +963,9 @@ guess_kernel(GuessArgs p) {
 
 
 // ---------------------------------------------------------------------------
-// Wave FFT (L = 1280 and 2560): one line pair per block of T = L / 20 lanes
-// (one wave at L = 1280, two at 2560).  With n = l + T m and k = k1 + 20 k2,
+// Wave FFT (L = 1280 and 2560): one line pair per T = L / 20 lanes (one
+// wave at L = 1280, two at 2560), one pair per block (four in pass C at
+// L = 1280).  With n = l + T m and k = k1 + 20 k2,
 //   X[k1 + 20 k2] = sum_l W_T^(l k2) W_L^(l k1) sum_m x[l + T m] W_20^(m k1):
 // lane l holds the 20 points x[l + T m] in registers and transforms them
 // there (a Good-Thomas 4 x 5 DFT, no internal twiddles), multiplies by
@@ -977,7 +978,10 @@ guess_kernel(GuessArgs p) {
 // registers: lane l's output (r, q) is X[l + T r + q L / R], consecutive
 // across lanes, so the passes read and write global memory coalesced.
 // Against the radix-4/5 plan above: four LDS exchanges per transform instead
-// of six, no barrier across waves at L = 1280, a third less VALU work.
+// of six, no barrier across waves at L = 1280, a quarter less VALU work.
+// Between the passes the spectra are kept in grid-row order, rows N + 1
+// long, so a pass-A or pass-C line pair (two grid rows) is one aligned
+// 16-byte element per k.
 // ---------------------------------------------------------------------------
 
 // LCODE_FFT_WAVE_POINTS_1280: points per lane at L = 1280, 20 (one wave per
