@@ -75,6 +75,10 @@ constexpr int kRegionRowMax = 128;
 #define LCODE_PUSH_ATTR
 #endif
 
+#ifndef LCODE_PUSH_BOUND_READ
+#define LCODE_PUSH_BOUND_READ 1
+#endif
+
 // One coarse particle's push; returns max(|x_offt|, |y_offt|) after it and
 // the particle's initial position in xi0, yi0.
 // G: storage type of the field grids (the momentum update is FP64 in both).
@@ -194,10 +198,13 @@ push_kernel(PushArgs a) {
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) cells = max(cells, __shfl_xor(cells, off, 64));
-        // one atomic per wave, and only when it raises the bound: after the
-        // first waves the bound is reached and the rest only read it
+        // one atomic per wave, and (LCODE_PUSH_BOUND_READ) only when it raises
+        // the bound: after the first waves the bound is reached and the rest
+        // only read it (an atomic from every wave to the one address took the
+        // push at 641^2 from 9.0 to 13.7 us)
         if ((threadIdx.x & 63) == 0) {
-            if (cells > __hip_atomic_load(a.max_cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            if (!LCODE_PUSH_BOUND_READ ||
+                cells > __hip_atomic_load(a.max_cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 atomicMax(a.max_cells, cells);
             if (cells == kNonFinite) {
                 if (a.host_flag)
