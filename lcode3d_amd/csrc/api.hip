@@ -65,6 +65,13 @@ struct lcode_workspace {
                                    // two slots of region_n^2 ints
     int region_n = 0;              // tiles per side (0: no per-tile bounds)
     int* tile_order = nullptr;     // deposition tiles, centre first
+    // deposition tiles taken by two blocks: split_tile[s] for s < nsplit,
+    // then the slot of each tile (-1: not split); the two parts' sums
+    // (nsplit x 2 x 4 x tile^2 doubles) and a count per slot
+    int nsplit = 0;
+    int* split_tile = nullptr;
+    double* split_part = nullptr;
+    int* split_count = nullptr;
     int bound_slot = 0;            // slot the next push raises
     int spill_slot = 0;            // count slot of the next deposition
     int grid_spill_left = 0;       // depositions still followed by the spill
@@ -146,6 +153,16 @@ constexpr bool kDepRegions = LCODE_DEP_REGIONS;
 #define LCODE_DEP_ORDER 2
 #endif
 constexpr int kDepOrder = LCODE_DEP_ORDER;
+
+// Deposition tiles taken by two blocks each (build option; -1: as many as
+// the launch leaves block slots free when all tile blocks are resident at
+// once, kDepResident per CU; 0: none).  The tiles nearest the grid centre
+// are split: an axisymmetric beam's wake compresses and displaces the plasma
+// there, so those tiles have the largest halos and the most particles.
+#ifndef LCODE_DEP_SPLIT
+#define LCODE_DEP_SPLIT -1
+#endif
+constexpr int kDepResident = 3;   // tile blocks per CU (165 VGPRs, 4 waves)
 
 // The XCD-cluster tile order: order[lin] = tile for linear block id lin
 // (XCD lin % 8), a permutation of the nt * nt tiles.
@@ -628,6 +645,42 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
             return fail(rc);
         }
     }
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device) !=
+            hipSuccess) {
+            set_error("hipDeviceGetAttribute (multiprocessor count) failed");
+            return fail(LCODE_E_HIP);
+        }
+        const int free_slots = std::max(kDepResident * cus - nt * nt, 0);
+        const int ns = std::min(LCODE_DEP_SPLIT < 0 ? free_slots : LCODE_DEP_SPLIT, nt * nt);
+        if (ns > 0) {
+            std::vector<int> byd((size_t)nt * nt);
+            for (int k = 0; k < nt * nt; ++k) byd[k] = k;
+            const double mid = (nt - 1) / 2.;
+            auto d2 = [&](int k) {
+                const double y = k / nt - mid, x = k % nt - mid;
+                return x * x + y * y;
+            };
+            std::stable_sort(byd.begin(), byd.end(),
+                             [&](int u, int v) { return d2(u) < d2(v); });
+            std::vector<int> tab((size_t)ns + (size_t)nt * nt, -1);
+            for (int s = 0; s < ns; ++s) {
+                tab[s] = byd[s];
+                tab[ns + byd[s]] = s;
+            }
+            ws->nsplit = ns;
+            const size_t part = 2 * 4 * (size_t)kDepTile * kDepTile;
+            if ((rc = upload(ws, &ws->split_tile, tab)) ||
+                (rc = dev_alloc(ws, (void**)&ws->split_part, ns * part * sizeof(double))) ||
+                (rc = dev_alloc(ws, (void**)&ws->split_count, ns * sizeof(int))))
+                return fail(rc);
+            if (hipMemset(ws->split_count, 0, ns * sizeof(int)) != hipSuccess) {
+                set_error("hipMemset of the split counts failed");
+                return fail(LCODE_E_HIP);
+            }
+        }
+    }
     if (hipHostMalloc((void**)&ws->host_flag, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
         ws->host_flag = nullptr;
         set_error("hipHostMalloc of the host flags failed");
@@ -859,6 +912,7 @@ static DepositArgs deposit_args(const lcode_config* cfg,
     a.ro_initial = ro_initial;
     a.ro = out[0]; a.jx = out[1]; a.jy = out[2]; a.jz = out[3];
     a.halo = kDepositHalo;
+    a.ntx = (a.N + kDepTile - 1) / kDepTile;
     // the tile's coarse rectangle has at most ext / coarseness + 3 rows
     a.staged = (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= kDepCoarseMax;
     a.group = cfg->plasma_fineness;
@@ -881,6 +935,11 @@ static int do_deposit(lcode_workspace* ws, const lcode_config* cfg,
     a.spill_count = ws->spill_ctr + ws->spill_slot;
     a.spill_next = ws->spill_ctr + (1 - ws->spill_slot);
     a.tile_order = ws->tile_order;
+    a.nsplit = ws->nsplit;
+    a.split_tile = ws->split_tile;
+    a.split_slot = ws->split_tile ? ws->split_tile + ws->nsplit : nullptr;
+    a.split_part = ws->split_part;
+    a.split_count = ws->split_count;
     // the spill list is deposited by the tile kernel's last block, unless
     // a deposition listed particles recently: then for the next 64 the
     // grid-wide spill kernel follows (a long list is many blocks' work)

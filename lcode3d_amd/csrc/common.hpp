@@ -312,14 +312,28 @@ struct DepositArgs {
     int* halo_cells;               // NULL, or the pushes' displacement bound
     int* halo_next;                // NULL, or the bound slot of the next
                                    // pushes: set to 0 by the deposition
-    // NULL, or the pushes' bound per tile ([gridDim.y][gridDim.x]): a tile's
+    // NULL, or the pushes' bound per tile ([ntx][ntx]): a tile's
     // halo is the largest bound of its 3 x 3 tile neighbourhood; and the
     // next pushes' per-tile slots, each zeroed by its tile
     const int* halo_regions;
     int* halo_regions_next;
-    // NULL, or the tile each block takes ([gridDim.y * gridDim.x], a
+    // NULL, or the tile each of the first ntx * ntx blocks takes (a
     // permutation of the row-major tile indices)
     const int* tile_order;
+    int ntx;                       // tiles per side; the tile kernel runs
+                                   // ntx * ntx + nsplit one-dimensional blocks
+    // split tiles (nsplit 0: none).  Block ntx * ntx + s is the second part
+    // of tile split_tile[s], and split_slot[tile] is s for that tile (-1 for
+    // the others).  The two parts of a split tile each deposit half of its
+    // particle groups and write their sums to split_part (2 x 4 x tile^2
+    // doubles per slot); the second to finish, counted in split_count[s]
+    // (zero between calls: that block resets it), adds the other's sums to
+    // its own and stores the tile.
+    int nsplit;
+    const int* split_tile;
+    const int* split_slot;
+    double* split_part;
+    int* split_count;
     // 1: no spill kernel follows; when a particle may spill (the pushes'
     // bound exceeds a.halo) every tile block ends with a fence and a count
     // in *done, and the last one deposits the spill list and resets *done
