@@ -162,7 +162,7 @@ constexpr int kDepOrder = LCODE_DEP_ORDER;
 #ifndef LCODE_DEP_SPLIT
 #define LCODE_DEP_SPLIT -1
 #endif
-constexpr int kDepResident = 3;   // tile blocks per CU (165 VGPRs, 4 waves)
+constexpr int kDepResident = 3;   // tile blocks per CU (158 VGPRs, 4 waves)
 
 // The XCD-cluster tile order: order[lin] = tile for linear block id lin
 // (XCD lin % 8), a permutation of the nt * nt tiles.
@@ -194,6 +194,38 @@ std::vector<int> xcd_cluster_order(int nt) {
     for (int lin = 0; lin < n; ++lin)
         if (order[lin] < 0) order[lin] = spare[k++];
     return order;
+}
+
+// Within an XCD the k-th block of a launch goes to CU k mod (CUs per XCD)
+// (read from the hardware ids of a one-round launch), so CU c runs slots
+// c, c + C, c + 2C, ...  LCODE_DEP_BALANCE (build option) reorders each
+// XCD's tiles over its slots, keeping every tile on its XCD: by distance
+// from the grid centre (the wake's tiles, the heaviest, first), in snake
+// order over the rounds, so that a CU's blocks mix heavy and light tiles.
+#ifndef LCODE_DEP_BALANCE
+#define LCODE_DEP_BALANCE 1
+#endif
+void balance_xcd_slots(std::vector<int>& order, int nt, int per_xcd) {
+    constexpr int kXcd = 8;
+    const int n = (int)order.size();
+    const double mid = (nt - 1) / 2.;
+    auto d2 = [&](int t) {
+        const double y = t / nt - mid, x = t % nt - mid;
+        return x * x + y * y;
+    };
+    for (int x = 0; x < kXcd; ++x) {
+        std::vector<int> tiles;
+        for (int lin = x; lin < n; lin += kXcd) tiles.push_back(order[lin]);
+        const int m = (int)tiles.size();
+        std::stable_sort(tiles.begin(), tiles.end(),
+                         [&](int u, int v) { return d2(u) < d2(v); });
+        for (int i = 0; i < m; ++i) {
+            const int r = i / per_xcd, c = i % per_xcd;
+            const int width = std::min(per_xcd, m - r * per_xcd);   // slots of round r
+            const int k = r * per_xcd + (r % 2 == 0 ? c : width - 1 - c);
+            order[x + kXcd * k] = tiles[i];
+        }
+    }
 }
 
 // Round-to-nearest-even BF16 bits of a float (finite inputs).
@@ -625,11 +657,20 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         if ((rc = dev_alloc(ws, (void**)&ws->regions, 2 * (size_t)nt * nt * sizeof(int))))
             return fail(rc);
     }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device) !=
+        hipSuccess) {
+        set_error("hipDeviceGetAttribute (multiprocessor count) failed");
+        return fail(LCODE_E_HIP);
+    }
     if (kDepOrder) {
         std::vector<int> order((size_t)nt * nt);
         for (int k = 0; k < nt * nt; ++k) order[k] = k;
         if (kDepOrder == 2) {
             order = xcd_cluster_order(nt);
+            // (when every tile block is resident at once: one round)
+            if (LCODE_DEP_BALANCE && cus >= 8 && cus % 8 == 0 && nt * nt <= kDepResident * cus)
+                balance_xcd_slots(order, nt, cus / 8);
         } else {
             const double mid = (nt - 1) / 2.;
             auto dist = [&](int k) {
@@ -646,12 +687,6 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         }
     }
     {
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device) !=
-            hipSuccess) {
-            set_error("hipDeviceGetAttribute (multiprocessor count) failed");
-            return fail(LCODE_E_HIP);
-        }
         const int free_slots = std::max(kDepResident * cus - nt * nt, 0);
         const int ns = std::min(LCODE_DEP_SPLIT < 0 ? free_slots : LCODE_DEP_SPLIT, nt * nt);
         if (ns > 0) {
