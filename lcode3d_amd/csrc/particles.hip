@@ -400,9 +400,9 @@ __device__ __forceinline__ void deposit_tile_body(const DepositArgs& a) {
     __shared__ int tile_nan;
     const int N = a.N;
     const double inv_h = 1. / a.h;
-    // tile of this block: row-major, or the order of a.tile_order (tiles
-    // near the grid centre, where the wake displaces and compresses the
-    // plasma and tiles take longest, first)
+    // tile of this block: row-major, or the host's order in a.tile_order
+    // (which keeps neighbouring tiles on one XCD and gives each CU a mix of
+    // the wake's heavy tiles and light ones)
     // A split tile's two parts (the tile's own block, part 0, and one of the
     // nsplit blocks after the tiles, part 1) take alternate runs of
     // blockDim.x particle groups.
