@@ -118,7 +118,9 @@ __device__ __forceinline__ bool gate_open(const Gate& g) {
 // HIP events on a sample of the launches, three boxes): 24 cells (729
 // tiles, all resident at once) 43.0-43.2 us per launch, 28 cells (529
 // tiles) 43.7-47.8 us, 30-36 cells 49-58 us, 16-22 cells 44-51 us (more
-// tiles than block slots); 512 threads 54-59 us.
+// tiles than block slots); 512 threads 54-59 us.  With the central tiles
+// split over the free block slots: 24 cells 39.3 us, 25 cells 39.8, 26
+// cells (143 split tiles) 42.5, 27 cells 43.3.
 #ifndef LCODE_DEP_TILE
 #define LCODE_DEP_TILE 24
 #endif
