@@ -461,34 +461,136 @@ __device__ __forceinline__ double rhs_value(int kind, int i, int j, int N,
     return rhs_value_dev<G>(kind, i, j, N, in);
 }
 
+// LCODE_RHS_BATCH (build option, the FP64 wave pass A only; the other pass
+// A kernels keep 0, which their register budgets need): 0, each
+// right-hand-side value loads and combines its inputs in one expression, and
+// the compiler streams the loads (about a dozen in flight per wave); B > 0,
+// the inputs of B columns of both rows are loaded (rhs_load_k) before any is
+// combined (rhs_combine_k, the same arithmetic as rhs_value_k), so that up
+// to 18 B loads are in flight.
+#ifndef LCODE_RHS_BATCH
+#define LCODE_RHS_BATCH 0
+#endif
+
+// The loads of rhs_value_k<KIND>(i, j): r[0..2] and r[3..5] the source
+// (density or jz, its previous-pass value, beam) at the forward and backward
+// neighbour, r[6..8] the previous-slice current, the current and the trick
+// field at the point (Ex, Ey, Bx, By); r[0..3] the four current neighbours
+// (Ez, Bz).
+template <int KIND, typename G>
+__device__ __forceinline__ void rhs_load_k(int i, int j, int N, const RhsInputs& in,
+                                           double* r) {
+    const size_t idx = (size_t)i * N + j;
+    const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
+    const size_t oi = inner ? (size_t)N : 0, oj = inner ? 1 : 0;
+    const G *jx = gp<G>(in.jx), *jy = gp<G>(in.jy);
+    if constexpr (KIND == F_EZ) {
+        r[0] = (double)jx[idx + oi];
+        r[1] = (double)jx[idx - oi];
+        r[2] = (double)jy[idx + oj];
+        r[3] = (double)jy[idx - oj];
+    } else if constexpr (KIND == F_BZ) {
+        r[0] = (double)jx[idx + oj];
+        r[1] = (double)jx[idx - oj];
+        r[2] = (double)jy[idx + oi];
+        r[3] = (double)jy[idx - oi];
+    } else {
+        constexpr bool rho = KIND == F_EX || KIND == F_EY;
+        constexpr bool along_i = KIND == F_EX || KIND == F_BY;
+        const size_t o = along_i ? oi : oj;
+        const G* src = gp<G>(rho ? in.ro : in.jz);
+        const double* sp0 = rho ? in.ro_prev : in.jz_prev;
+        const G* prev = sp0 ? gp<G>(sp0) : src;
+        const double* dummy = std::is_same<G, double>::value ? in.ro : in.zero64;
+        const double* b = in.beam ? in.beam : dummy;
+        r[0] = (double)src[idx + o];
+        r[1] = (double)prev[idx + o];
+        r[2] = b[idx + o];
+        r[3] = (double)src[idx - o];
+        r[4] = (double)prev[idx - o];
+        r[5] = b[idx - o];
+        constexpr bool xcur = KIND == F_EX || KIND == F_BY;
+        const G* jp = gp<G>(xcur ? in.jx_prev : in.jy_prev);
+        const G* jc = xcur ? jx : jy;
+        constexpr int ft = KIND == F_EX ? 0 : KIND == F_EY ? 1 : KIND == F_BX ? 2 : 3;
+        r[6] = (double)jp[idx];
+        r[7] = (double)jc[idx];
+        r[8] = (double)gp<G>(in.Ftrick[ft])[idx];
+    }
+}
+
+// rhs_value_k<KIND>(i, j) from the values rhs_load_k loaded.
+template <int KIND>
+__device__ __forceinline__ double rhs_combine_k(int i, int j, int N, const RhsInputs& in,
+                                                const double* r) {
+    const bool inner = i >= 1 && i <= N - 2 && j >= 1 && j <= N - 2;
+    const double h2 = in.h * 2;
+    if constexpr (KIND == F_EZ) {
+        return -((r[0] - r[1]) + (r[2] - r[3])) / h2;
+    } else if constexpr (KIND == F_BZ) {
+        const double djx_dy = r[0] - r[1];
+        const double djy_dx = r[2] - r[3];
+        return -((inner ? djx_dy : 0.) - (inner ? djy_dx : 0.)) / h2;
+    } else {
+        auto source = [&](double v, double p, double b) {
+            const double s = (v + p) / 2;
+            return s + (b * (in.beam ? 1. : 0.) + (in.beam ? 0. : in.beam_scalar));
+        };
+        const double d = (source(r[0], r[1], r[2]) - source(r[3], r[4], r[5])) / h2;
+        const double dj = (r[6] - r[7]) / in.dxi;
+        if constexpr (KIND == F_BX) return +(((inner ? d : 0.) - dj) + r[8] * in.trick);
+        else return -(((inner ? d : 0.) - dj) - r[8] * in.trick);
+    }
+}
+
 // Right-hand sides of rows ia and ib at columns j0 + t + u Tp, u < K, as
 // straight-line code: columns clamped to [j0, j0 + len), so all loads of
 // the thread are issued before their values are used.
-template <int KIND, int K, typename G>
+template <int KIND, int K, typename G, int B>
 __device__ __forceinline__ void rhs_rows_k(int ia, int ib, int j0, int len,
                                            int t, int Tp, int N,
                                            const RhsInputs& in, double* a,
                                            double* c) {
+    if constexpr (B > 0) {
 #pragma unroll
-    for (int u = 0; u < K; ++u) {
-        const int j = j0 + min(t + u * Tp, len - 1);
-        a[u] = rhs_value_k<KIND, G>(ia, j, N, in);
-        c[u] = rhs_value_k<KIND, G>(ib, j, N, in);
+        for (int u0 = 0; u0 < K; u0 += B) {
+            double ra[B][9], rc[B][9];
+#pragma unroll
+            for (int u = u0; u < u0 + B && u < K; ++u) {
+                const int j = j0 + min(t + u * Tp, len - 1);
+                rhs_load_k<KIND, G>(ia, j, N, in, ra[u - u0]);
+                rhs_load_k<KIND, G>(ib, j, N, in, rc[u - u0]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = u0; u < u0 + B && u < K; ++u) {
+                const int j = j0 + min(t + u * Tp, len - 1);
+                a[u] = rhs_combine_k<KIND>(ia, j, N, in, ra[u - u0]);
+                c[u] = rhs_combine_k<KIND>(ib, j, N, in, rc[u - u0]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < K; ++u) {
+            const int j = j0 + min(t + u * Tp, len - 1);
+            a[u] = rhs_value_k<KIND, G>(ia, j, N, in);
+            c[u] = rhs_value_k<KIND, G>(ib, j, N, in);
+        }
     }
 }
 
-template <int K, typename G>
+template <int K, typename G, int B = 0>
 __device__ __forceinline__ void rhs_rows(int kind, int ia, int ib, int j0,
                                          int len, int t, int Tp, int N,
                                          const RhsInputs& in, double* a,
                                          double* c) {
     switch (kind) {
-        case F_EX: rhs_rows_k<F_EX, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        case F_EY: rhs_rows_k<F_EY, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        case F_BX: rhs_rows_k<F_BX, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        case F_BY: rhs_rows_k<F_BY, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        case F_EZ: rhs_rows_k<F_EZ, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
-        default: rhs_rows_k<F_BZ, K, G>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_EX: rhs_rows_k<F_EX, K, G, B>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_EY: rhs_rows_k<F_EY, K, G, B>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_BX: rhs_rows_k<F_BX, K, G, B>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_BY: rhs_rows_k<F_BY, K, G, B>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        case F_EZ: rhs_rows_k<F_EZ, K, G, B>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
+        default: rhs_rows_k<F_BZ, K, G, B>(ia, ib, j0, len, t, Tp, N, in, a, c); break;
     }
 }
 
@@ -1233,7 +1335,9 @@ pass_a_wave_kernel(SolverGeom g, SolveBatch b) {
     {
         double a[kPer], c[kPer];
         const int gc = va ? ga : gb;   // rows without data read a valid one
-        rhs_rows<kPer, G>(fp.kind, gc, vb ? gb : gc, fp.colA0, fp.lenA, lane, T, N, b.in, a, c);
+        rhs_rows<kPer, G, sizeof(S) == 8 ? LCODE_RHS_BATCH : 0>(fp.kind, gc, vb ? gb : gc,
+                                                                fp.colA0, fp.lenA, lane, T, N,
+                                                                b.in, a, c);
 #pragma unroll
         for (int u = 0; u < kPer; ++u) {
             const int n = lane + u * T;
