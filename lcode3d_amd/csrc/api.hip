@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "tile_order.hpp"
 
 namespace lcode {
 
@@ -164,69 +165,16 @@ constexpr int kDepOrder = LCODE_DEP_ORDER;
 #endif
 constexpr int kDepResident = 3;   // tile blocks per CU (158 VGPRs, 4 waves)
 
-// The XCD-cluster tile order: order[lin] = tile for linear block id lin
-// (XCD lin % 8), a permutation of the nt * nt tiles.
+// Side of the XCD clusters of the tile order (tile_order.hpp).
 #ifndef LCODE_DEP_CLUSTER
 #define LCODE_DEP_CLUSTER 3
 #endif
-std::vector<int> xcd_cluster_order(int nt) {
-    constexpr int kXcd = 8, kC = LCODE_DEP_CLUSTER;
-    const int nc = (nt + kC - 1) / kC;
-    std::vector<std::vector<int>> per(kXcd);
-    for (int cy = 0, j = 0; cy < nc; ++cy)
-        for (int cx = 0; cx < nc; ++cx, ++j)
-            for (int dy = 0; dy < kC; ++dy)
-                for (int dx = 0; dx < kC; ++dx) {
-                    const int ty = cy * kC + dy, tx = cx * kC + dx;
-                    if (ty < nt && tx < nt) per[j % kXcd].push_back(ty * nt + tx);
-                }
-    const int n = nt * nt;
-    std::vector<int> order(n, -1), spare;
-    std::vector<size_t> used(kXcd, 0);
-    for (int lin = 0; lin < n; ++lin) {   // each XCD's own clusters first
-        auto& v = per[lin % kXcd];
-        size_t& u = used[lin % kXcd];
-        if (u < v.size()) order[lin] = v[u++];
-    }
-    for (int x = 0; x < kXcd; ++x)        // what did not fit, to the free slots
-        for (size_t u = used[x]; u < per[x].size(); ++u) spare.push_back(per[x][u]);
-    size_t k = 0;
-    for (int lin = 0; lin < n; ++lin)
-        if (order[lin] < 0) order[lin] = spare[k++];
-    return order;
-}
 
-// Within an XCD the k-th block of a launch goes to CU k mod (CUs per XCD)
-// (read from the hardware ids of a one-round launch), so CU c runs slots
-// c, c + C, c + 2C, ...  LCODE_DEP_BALANCE (build option) reorders each
-// XCD's tiles over its slots, keeping every tile on its XCD: by distance
-// from the grid centre (the wake's tiles, the heaviest, first), in snake
-// order over the rounds, so that a CU's blocks mix heavy and light tiles.
+// 1: each XCD's tiles spread over its CUs in one-round launches
+// (balance_xcd_slots, tile_order.hpp)
 #ifndef LCODE_DEP_BALANCE
 #define LCODE_DEP_BALANCE 1
 #endif
-void balance_xcd_slots(std::vector<int>& order, int nt, int per_xcd) {
-    constexpr int kXcd = 8;
-    const int n = (int)order.size();
-    const double mid = (nt - 1) / 2.;
-    auto d2 = [&](int t) {
-        const double y = t / nt - mid, x = t % nt - mid;
-        return x * x + y * y;
-    };
-    for (int x = 0; x < kXcd; ++x) {
-        std::vector<int> tiles;
-        for (int lin = x; lin < n; lin += kXcd) tiles.push_back(order[lin]);
-        const int m = (int)tiles.size();
-        std::stable_sort(tiles.begin(), tiles.end(),
-                         [&](int u, int v) { return d2(u) < d2(v); });
-        for (int i = 0; i < m; ++i) {
-            const int r = i / per_xcd, c = i % per_xcd;
-            const int width = std::min(per_xcd, m - r * per_xcd);   // slots of round r
-            const int k = r * per_xcd + (r % 2 == 0 ? c : width - 1 - c);
-            order[x + kXcd * k] = tiles[i];
-        }
-    }
-}
 
 // Round-to-nearest-even BF16 bits of a float (finite inputs).
 uint16_t bf16_bits(float f) {
@@ -667,7 +615,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         std::vector<int> order((size_t)nt * nt);
         for (int k = 0; k < nt * nt; ++k) order[k] = k;
         if (kDepOrder == 2) {
-            order = xcd_cluster_order(nt);
+            order = xcd_cluster_order(nt, LCODE_DEP_CLUSTER);
             // (when every tile block is resident at once: one round)
             if (LCODE_DEP_BALANCE && cus >= 8 && cus % 8 == 0 && nt * nt <= kDepResident * cus)
                 balance_xcd_slots(order, nt, cus / 8);
@@ -690,20 +638,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         const int free_slots = std::max(kDepResident * cus - nt * nt, 0);
         const int ns = std::min(LCODE_DEP_SPLIT < 0 ? free_slots : LCODE_DEP_SPLIT, nt * nt);
         if (ns > 0) {
-            std::vector<int> byd((size_t)nt * nt);
-            for (int k = 0; k < nt * nt; ++k) byd[k] = k;
-            const double mid = (nt - 1) / 2.;
-            auto d2 = [&](int k) {
-                const double y = k / nt - mid, x = k % nt - mid;
-                return x * x + y * y;
-            };
-            std::stable_sort(byd.begin(), byd.end(),
-                             [&](int u, int v) { return d2(u) < d2(v); });
-            std::vector<int> tab((size_t)ns + (size_t)nt * nt, -1);
-            for (int s = 0; s < ns; ++s) {
-                tab[s] = byd[s];
-                tab[ns + byd[s]] = s;
-            }
+            const std::vector<int> tab = split_tile_table(nt, ns);
             ws->nsplit = ns;
             const size_t part = 2 * 4 * (size_t)kDepTile * kDepTile;
             if ((rc = upload(ws, &ws->split_tile, tab)) ||
