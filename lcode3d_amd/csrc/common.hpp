@@ -148,7 +148,9 @@ constexpr int kDepCoarseMax = LCODE_DEP_COARSE_MAX;
 #define LCODE_DEP_GROUP 1
 #endif
 // 1: the lanes of a deposition wave take groups a prime stride apart
-// instead of adjacent ones (build option)
+// instead of adjacent ones (build option; at 641^2 39.6 -> 44.7 us per
+// launch: adjacent lanes' LDS reads of the staged tables lose their
+// locality)
 #ifndef LCODE_DEP_SPREAD
 #define LCODE_DEP_SPREAD 0
 #endif
