@@ -238,8 +238,10 @@ int lcode_deposit(lcode_workspace *ws, const lcode_config *cfg,
 /* lcode_deposit without a workspace and without any allocation.  No
  * counterpart in the reference: it is for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
- * harnesses that forbid hipMalloc).  Same arguments, kernels and results as
- * lcode_deposit, except that clamped lookups are not counted and a
+ * harnesses that forbid hipMalloc).  Same arguments and kernels as
+ * lcode_deposit and the same results up to the order of the floating-point
+ * sums (it takes every tile with one block, where lcode_deposit may split
+ * tiles over two), except that clamped lookups are not counted and a
  * non-finite particle is not flagged to the host (its tile is still written
  * as NaN).  Its scratch - the spill list of particles displaced beyond the
  * deposition tiles' halo and its counters - is a fixed device array of
