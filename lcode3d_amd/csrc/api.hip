@@ -157,13 +157,21 @@ constexpr int kDepOrder = LCODE_DEP_ORDER;
 
 // Deposition tiles taken by two blocks each (build option; -1: as many as
 // the launch leaves block slots free when all tile blocks are resident at
-// once, kDepResident per CU; 0: none).  The tiles nearest the grid centre
+// once, at the tile blocks per CU the runtime reports for the kernel, or
+// kDepResident when it reports none; 0: none).  The tiles nearest the grid centre
 // are split: an axisymmetric beam's wake compresses and displaces the plasma
 // there, so those tiles have the largest halos and the most particles.
 #ifndef LCODE_DEP_SPLIT
 #define LCODE_DEP_SPLIT -1
 #endif
-constexpr int kDepResident = 3;   // tile blocks per CU (158 VGPRs, 4 waves)
+constexpr int kDepResident = 3;   // tile blocks per CU (158 VGPRs, 4 waves) when
+                                  // the runtime reports none
+
+// The deposition stages each tile's coarse rectangle in LDS: it has at
+// most ext / coarseness + 3 rows (ext: the tile and its largest halo).
+static bool deposit_staged(const lcode_config* cfg) {
+    return (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= kDepCoarseMax;
+}
 
 // Side of the XCD clusters of the tile order (tile_order.hpp).
 #ifndef LCODE_DEP_CLUSTER
@@ -611,13 +619,15 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         set_error("hipDeviceGetAttribute (multiprocessor count) failed");
         return fail(LCODE_E_HIP);
     }
+    int resident = deposit_blocks_per_cu(deposit_staged(cfg));
+    if (resident <= 0) resident = kDepResident;
     if (kDepOrder) {
         std::vector<int> order((size_t)nt * nt);
         for (int k = 0; k < nt * nt; ++k) order[k] = k;
         if (kDepOrder == 2) {
             order = xcd_cluster_order(nt, LCODE_DEP_CLUSTER);
             // (when every tile block is resident at once: one round)
-            if (LCODE_DEP_BALANCE && cus >= 8 && cus % 8 == 0 && nt * nt <= kDepResident * cus)
+            if (LCODE_DEP_BALANCE && cus >= 8 && cus % 8 == 0 && nt * nt <= resident * cus)
                 balance_xcd_slots(order, nt, cus / 8);
         } else {
             const double mid = (nt - 1) / 2.;
@@ -635,7 +645,7 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         }
     }
     {
-        const int free_slots = std::max(kDepResident * cus - nt * nt, 0);
+        const int free_slots = std::max(resident * cus - nt * nt, 0);
         const int ns = std::min(LCODE_DEP_SPLIT < 0 ? free_slots : LCODE_DEP_SPLIT, nt * nt);
         if (ns > 0) {
             const std::vector<int> tab = split_tile_table(nt, ns);
@@ -883,8 +893,7 @@ static DepositArgs deposit_args(const lcode_config* cfg,
     a.ro = out[0]; a.jx = out[1]; a.jy = out[2]; a.jz = out[3];
     a.halo = kDepositHalo;
     a.ntx = (a.N + kDepTile - 1) / kDepTile;
-    // the tile's coarse rectangle has at most ext / coarseness + 3 rows
-    a.staged = (kDepTile + 2 * kDepositHalo + 2) / cfg->plasma_coarseness + 3 <= kDepCoarseMax;
+    a.staged = deposit_staged(cfg);
     a.group = cfg->plasma_fineness;
     return a;
 }

@@ -355,6 +355,10 @@ struct DepositArgs {
 int launch_estimate(const EstimateArgs& a, hipStream_t s, Profiler* prof);
 int launch_push(const PushArgs& a, hipStream_t s, Profiler* prof);
 int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof);
+// Tile blocks of the deposition one CU holds at once (the smaller of its
+// FP64- and FP32-grid forms, staged or not), or 0 when the runtime cannot
+// say.
+int deposit_blocks_per_cu(bool staged);
 // make_plasma (lcode.py:452-573) on the device.
 constexpr double kElectronMass = 1.;      // lcode.py:47
 constexpr double kElectronCharge = -1.;   // lcode.py:46

@@ -934,6 +934,27 @@ int launch_zero_ints(int* p, int n, hipStream_t s) {
     return LCODE_OK;
 }
 
+int deposit_blocks_per_cu(bool staged) {
+    int a = 0, b = 0;
+    hipError_t e1, e2;
+    if (staged) {
+        e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &a, deposit_tile_kernel<true, double, double>, kDepBlock, 0);
+        e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &b, deposit_tile_kernel<true, double, float>, kDepBlock, 0);
+    } else {
+        e1 = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &a, deposit_tile_kernel<false, double, double>, kDepBlock, 0);
+        e2 = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &b, deposit_tile_kernel<false, double, float>, kDepBlock, 0);
+    }
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        (void)hipGetLastError();   // leave no error behind for later checks
+        return 0;
+    }
+    return std::min(a, b);
+}
+
 int launch_deposit(const DepositArgs& a, hipStream_t s, Profiler* prof) {
     LCODE_REQUIRE(a.ntx == (a.N + kTile - 1) / kTile, "deposition: tiles per side do not match N");
     LCODE_REQUIRE(a.nsplit >= 0 && a.nsplit <= a.ntx * a.ntx &&
