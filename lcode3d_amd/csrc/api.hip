@@ -179,7 +179,7 @@ static bool deposit_staged(const lcode_config* cfg) {
 #endif
 
 // 1: each XCD's tiles spread over its CUs in one-round launches
-// (balance_xcd_slots, tile_order.hpp)
+// (balance_xcd_slots, tile_order.hpp); 2: in every launch
 #ifndef LCODE_DEP_BALANCE
 #define LCODE_DEP_BALANCE 1
 #endif
@@ -627,7 +627,8 @@ int lcode_workspace_create(const lcode_config* cfg, lcode_workspace** out) {
         if (kDepOrder == 2) {
             order = xcd_cluster_order(nt, LCODE_DEP_CLUSTER);
             // (when every tile block is resident at once: one round)
-            if (LCODE_DEP_BALANCE && cus >= 8 && cus % 8 == 0 && nt * nt <= resident * cus)
+            if (LCODE_DEP_BALANCE && cus >= 8 && cus % 8 == 0 &&
+                (LCODE_DEP_BALANCE == 2 || nt * nt <= resident * cus))
                 balance_xcd_slots(order, nt, cus / 8);
         } else {
             const double mid = (nt - 1) / 2.;
