@@ -1366,8 +1366,21 @@ pass_a_wave_kernel(SolverGeom g, SolveBatch b) {
 #define LCODE_WAVE_TABLE_EARLY 0
 #endif
 
+// Waves per SIMD the FP64 wave pass B at L = 2560 is compiled for (build
+// option): 2 (the other passes' budget, 256 registers: 41 spill to
+// scratch) or 1 (up to 512: 256 VGPRs and 45 AGPRs, no scratch, half the
+// waves).  The 1281^2 plain run: 707.6-708.9 xi-steps/s with 2, 711.5-711.7
+// with 1 (A/B on one box, two runs each).
+#ifndef LCODE_WAVE_B2560_WAVES
+#define LCODE_WAVE_B2560_WAVES 1
+#endif
+#define LCODE_WAVE_B_ATTR(S)                                                        \
+    __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? (L == 2560 ? LCODE_WAVE_B2560_WAVES \
+                                                                    : 40 / WaveLanes<L>::R1) \
+                                                      : 3)))
+
 template <int L, typename S>
-__global__ void __launch_bounds__(WaveLanes<L>::T) LCODE_WAVE_ATTR(S)
+__global__ void __launch_bounds__(WaveLanes<L>::T) LCODE_WAVE_B_ATTR(S)
 pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
     constexpr int T = WaveLanes<L>::T;
