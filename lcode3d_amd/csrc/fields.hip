@@ -1374,10 +1374,17 @@ pass_a_wave_kernel(SolverGeom g, SolveBatch b) {
 #ifndef LCODE_WAVE_B2560_WAVES
 #define LCODE_WAVE_B2560_WAVES 1
 #endif
+// Waves per SIMD of the FP32 wave pass B (build option): 3, as the other
+// FP32 passes, spills 6 registers at L = 1280; 2 (184 VGPRs) spills none:
+// the mixed six-field solve 61.5 -> 59.7-59.9 us at 641^2 (A/B on one
+// box, two runs each).
+#ifndef LCODE_WAVE_B_F32_WAVES
+#define LCODE_WAVE_B_F32_WAVES 2
+#endif
 #define LCODE_WAVE_B_ATTR(S)                                                        \
     __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? (L == 2560 ? LCODE_WAVE_B2560_WAVES \
                                                                     : 40 / WaveLanes<L>::R1) \
-                                                      : 3)))
+                                                      : LCODE_WAVE_B_F32_WAVES)))
 
 template <int L, typename S>
 __global__ void __launch_bounds__(WaveLanes<L>::T) LCODE_WAVE_B_ATTR(S)
