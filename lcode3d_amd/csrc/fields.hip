@@ -1356,6 +1356,14 @@ pass_a_wave_kernel(SolverGeom g, SolveBatch b) {
     });
 }
 
+// Waves per SIMD of the FP32 wave pass C (build option; 3 as the other
+// FP32 passes: at 2 the mixed six-field solve measured the same).
+#ifndef LCODE_WAVE_C_F32_WAVES
+#define LCODE_WAVE_C_F32_WAVES 3
+#endif
+#define LCODE_WAVE_C_ATTR(S) \
+    __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 40 / WaveLanes<L>::R1 : LCODE_WAVE_C_F32_WAVES)))
+
 // LCODE_WAVE_TABLE_INLINE (0): the wave pass B evaluates its spectral
 // multipliers instead of reading the table.  LCODE_WAVE_TABLE_EARLY: it
 // loads them before its first transform.
@@ -1480,7 +1488,7 @@ pass_b_wave_kernel(SolverGeom g, SolveBatch b) {
 }
 
 template <int L, typename S, typename G>
-__global__ void __launch_bounds__(WaveLanes<L>::T * WavePairs<L>::P) LCODE_WAVE_ATTR(S)
+__global__ void __launch_bounds__(WaveLanes<L>::T * WavePairs<L>::P) LCODE_WAVE_C_ATTR(S)
 pass_c_wave_kernel(SolverGeom g, SolveBatch b) {
     using V = typename Vec<S>::T;
     constexpr int T = WaveLanes<L>::T, P = WavePairs<L>::P, NT = T * P;
